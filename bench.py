@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: cone-beam FDK on a 384^3 volume with 512 views on one MI355X.
+
+One step = CUDAmex_FP (cone, step 1.0, clamp) -> FDK pre-filter (cosine
+weight, Hamming-windowed ramp, angle step / 2) -> CUDAmex_BP (cone), plain
+FP32, inputs resident in HBM.  The headline value is FP+BP GUPS:
+  (U_FP + U_BP) / (t_FP + t_BP) / 1e9,  U = 384^3 * 512 = 2.899e10 per
+operator (every voxel, every view).  The same line reports each operator's
+GUPS, the FP16-storage form's GUPS and its relative RMSE against the FP64
+oracle on a sample of rays/voxel columns, the roofline of the FP kernel
+and a CPU baseline (the oracle on a bounded sample, this host's cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mex-cuda-ct_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=384)
+    ap.add_argument("--views", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mexct
+    from mexct import phantoms
+    from mexct.device import Context
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    W = H = args.size
+    n = args.views
+    SAD = 1000.0
+    log(f"[bench] building inputs W=H={W} views={n}")
+    vol_np = phantoms.cylinder(W, H, 140.0 * W / 384.0, 0.02)
+    ang = phantoms.angles_linspace(n, 2 * np.pi)
+    vol = torch.from_numpy(vol_np.T.copy()).to(dev)  # (H, W, W) == MATLAB [W, W, H]
+    sino = torch.empty((n, H, W), dtype=torch.float32, device=dev)
+    filt = torch.empty_like(sino)
+    rec = torch.empty((H, W, W), dtype=torch.float32, device=dev)
+    ctx = Context(0)
+
+    def run_step(storage, v, events=None):
+        if events is not None:
+            events[0].record()
+        ctx.fp(v, SAD, ang, n, 2, sino, storage=storage)
+        if events is not None:
+            events[1].record()
+        ctx.prefilter(sino, 2, SAD, ang, 3, 1.0, filt)
+        if events is not None:
+            events[2].record()
+        ctx.bp(filt, SAD, ang, 2, rec, storage=storage)
+        if events is not None:
+            events[3].record()
+
+    def timed(storage, v, steps, warmup):
+        for _ in range(warmup):
+            run_step(storage, v)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            run_step(storage, v, evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fp_ms = np.mean([e[0].elapsed_time(e[1]) for e in evs])
+        fl_ms = np.mean([e[1].elapsed_time(e[2]) for e in evs])
+        bp_ms = np.mean([e[2].elapsed_time(e[3]) for e in evs])
+        return (t1 - t0) * 1e3 / steps, fp_ms, fl_ms, bp_ms
+
+    U = float(W) * W * H * n
+    log(f"[bench] fp32: warmup {args.warmup}, steps {args.steps}")
+    step_ms, fp_ms, fl_ms, bp_ms = timed(mexct.F32, vol, args.steps, args.warmup)
+    fp_gups = U / (fp_ms * 1e-3) / 1e9
+    bp_gups = U / (bp_ms * 1e-3) / 1e9
+    value = 2 * U / ((fp_ms + bp_ms) * 1e-3) / 1e9
+    log(f"[bench] fp32 step {step_ms:.2f} ms: FP {fp_ms:.2f} ms ({fp_gups:.0f} GUPS), "
+        f"filter {fl_ms:.2f} ms, BP {bp_ms:.2f} ms ({bp_gups:.0f} GUPS)")
+    sino32 = sino.clone()
+    filt32 = filt.clone()
+    rec32 = rec.clone()
+
+    # FP roofline: compulsory HBM bytes of a view-by-view projector, the
+    # volume read once per view plus the projections written once.
+    fp_bytes = n * (float(W) * W * H * 4) + n * float(W) * H * 4
+    fp_gbs = fp_bytes / (fp_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "fp_kernel<cone,f32>", "achieved": round(fp_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fp_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "algorithmic_bytes_per_launch": fp_bytes}
+
+    f16 = None
+    if not args.no_f16:
+        import oracle as O  # checker only: FP64 reference on a sample
+
+        vol16 = vol.half()
+        log("[bench] fp16-storage")
+        s16, f16_fp_ms, _, f16_bp_ms = timed(mexct.F16, vol16, args.steps, 1)
+        rng = np.random.Generator(np.random.PCG64(7))
+        nr = 2048
+        ix = rng.integers(0, W, nr)
+        iy = rng.integers(0, H, nr)
+        iv = rng.integers(0, n, nr)
+        ref = O.fp_f64_rays(vol_np, SAD, ang, 2, ix, iy, iv)
+        g16 = sino.cpu().numpy()[iv, iy, ix]
+        g32 = sino32.cpu().numpy()[iv, iy, ix]
+        rr16 = float(np.linalg.norm(g16 - ref) / np.linalg.norm(ref))
+        rr32 = float(np.linalg.norm(g32 - ref) / np.linalg.norm(ref))
+        # BP accuracy: same filtered FP32 input, columns inside the circle
+        hw = W // 2 - 1
+        cols = []
+        while len(cols) < 48:
+            a, b = rng.integers(0, W, 2)
+            if (a - hw) ** 2 + (b - hw) ** 2 < hw * hw:
+                cols.append((a, b))
+        cx = np.array([c[0] for c in cols])
+        cy = np.array([c[1] for c in cols])
+        fil_np = np.asfortranarray(filt32.cpu().numpy().transpose(2, 1, 0))
+        refb = O.bp_f64_cols(fil_np, SAD, ang, 2, cx, cy)  # [ncol, H]
+        rec16 = rec.cpu().numpy()[:, cy, cx].T
+        rec32n = rec32.cpu().numpy()[:, cy, cx].T
+        bb16 = float(np.linalg.norm(rec16 - refb) / np.linalg.norm(refb))
+        bb32 = float(np.linalg.norm(rec32n - refb) / np.linalg.norm(refb))
+        f16 = {"fp_ms": round(f16_fp_ms, 3), "bp_ms": round(f16_bp_ms, 3),
+               "fp_gups": round(U / (f16_fp_ms * 1e-3) / 1e9, 1),
+               "bp_gups": round(U / (f16_bp_ms * 1e-3) / 1e9, 1),
+               "fp_speedup_vs_f32": round(fp_ms / f16_fp_ms, 3),
+               "bp_speedup_vs_f32": round(bp_ms / f16_bp_ms, 3),
+               "fp_rel_rmse_vs_f64": rr16, "bp_rel_rmse_vs_f64": bb16,
+               "f32_fp_rel_rmse_vs_f64": rr32, "f32_bp_rel_rmse_vs_f64": bb32,
+               "sample": f"{nr} random rays, {len(cols)} voxel columns"}
+        log(f"[bench] fp16: FP {f16_fp_ms:.2f} ms BP {f16_bp_ms:.2f} ms; relRMSE FP {rr16:.2e} "
+            f"(f32 {rr32:.2e}) BP {bb16:.2e} (f32 {bb32:.2e})")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+
+        cores = os.cpu_count() or 1
+        O.set_threads(cores)
+        kv = 2  # views of the FP sample
+        log(f"[bench] cpu baseline: oracle FP on {kv} views, BP on 1024 columns, {cores} threads")
+        t0 = time.perf_counter()
+        O.fp(vol_np, SAD, ang[:kv], kv, 2)
+        tf = time.perf_counter() - t0
+        rng = np.random.Generator(np.random.PCG64(11))
+        ncol = 1024
+        cx = rng.integers(0, W, ncol)
+        cy = rng.integers(0, W, ncol)
+        fil_np = np.asfortranarray(filt32.cpu().numpy().transpose(2, 1, 0))
+        t0 = time.perf_counter()
+        O.bp_cols(fil_np, SAD, ang, 2, cx, cy)
+        tb = time.perf_counter() - t0
+        cfp = float(W) * W * H * kv / tf / 1e9
+        cbp = float(ncol) * H * n / tb / 1e9
+        cpu = {"value": round(2.0 / (1.0 / cfp + 1.0 / cbp), 4), "unit": "GUPS", "cores": cores,
+               "kind": "port", "fp_gups": round(cfp, 4), "bp_gups": round(cbp, 4),
+               "sample": f"FP of {kv} of {n} views ({tf:.1f} s); BP of {ncol} random voxel "
+                         f"columns x {H} z x {n} views ({tb:.1f} s); FP32 oracle"}
+        log(f"[bench] cpu: FP {cfp:.3f} GUPS, BP {cbp:.3f} GUPS")
+
+    line = {
+        "metric": "cone-beam FP/BP GUPS 384^3x512",
+        "value": round(value, 2),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"cone FDK: FP -> Hamming ramp filter -> BP, {W}^3 volume "
+                               f"(cylinder r={140.0 * W / 384.0:g}, 0.02), {n} views "
+                               "linspace(0,2pi), SAD 1000, detector WxH", "volume": [W, W, H],
+                   "views": n, "geometry": "cone", "parallelism": "single GPU"},
+        "fp_ms": round(fp_ms, 3), "filter_ms": round(fl_ms, 3), "bp_ms": round(bp_ms, 3),
+        "fp_gups": round(fp_gups, 1), "bp_gups": round(bp_gups, 1),
+        "roofline": roofline,
+        "fp16_storage": f16,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,183 @@
+/*
+ * mexct.h -- C ABI of the MI355X (gfx950) CT projection and
+ * iterative-reconstruction operators.
+ *
+ * Each entry point replaces one mexFunction of kurtishdekker/mex-cuda-CT
+ * (cited below) with the same argument meaning, array layout and result.
+ * Arrays are MATLAB `single`, column-major:
+ *   volume    vol[x + W*y + W*W*z]           dims [W, W, H]
+ *   sinogram  P[u + W*v + W*H*view]          dims [W, H, nProj]
+ *   rays      pts[ix + W*iy + W*H*k], k<6    dims [W, H, 6] (x1,y1,z1,x2,y2,z2)
+ * Angles are float radians; their sine/cosine is taken with host sincosf
+ * as in the reference.  geom: 0 parallel, 1 fan, 2 cone.
+ *
+ * Two families:
+ *   mexct_*      host pointers, synchronous (the MEX call contract: copy in,
+ *                compute, copy out).  Usable from any thread; each thread
+ *                uses its own implicit context on the current HIP device.
+ *   mexct_*_dev  device pointers, asynchronous on the context's stream.
+ *
+ * Return value: MEXCT_OK (0) or a negative MEXCT_E* code; the message of
+ * the last failure on the calling thread is mexct_last_error().
+ * Nothing in this header uses torch or HIP types.
+ */
+#ifndef MEXCT_H
+#define MEXCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEXCT_OK 0
+#define MEXCT_EINVAL (-1)  /* bad argument: null pointer, dims, counts      */
+#define MEXCT_EGEOM (-2)   /* geomFlag not 0/1/2 ("invalid geometry")         */
+#define MEXCT_EHIP (-3)    /* HIP runtime error                                */
+#define MEXCT_ENOMEM (-4)  /* device allocation failed or over the HBM budget */
+#define MEXCT_ENOTSUP (-5) /* unsupported option combination                  */
+
+/* storage type of the gathered operand (volume for FP, sinogram for BP);
+ * arithmetic (weights, ray sums, accumulators) is always FP32. */
+#define MEXCT_F32 0
+#define MEXCT_F16 1
+#define MEXCT_BF16 2
+
+typedef struct mexct_opts {
+  int32_t storage;  /* MEXCT_F32 (default, reference-exact), MEXCT_F16, MEXCT_BF16 */
+  int32_t tv_steps; /* TVmin gradient steps; 0 -> 20 as CUDAmex_TVmin_3D.cu:221  */
+  int32_t zrange;   /* BP z-voxels per thread; 0 -> autotuned                     */
+  int32_t reserved;
+} mexct_opts;
+
+typedef struct mexct_ctx mexct_ctx;
+
+/* Context = device + stream + workspace.  stream may be NULL (the context
+ * creates one at default priority) or an existing hipStream_t. */
+int mexct_ctx_create(mexct_ctx** ctx, int device, void* stream);
+int mexct_ctx_destroy(mexct_ctx* ctx);
+int mexct_ctx_sync(mexct_ctx* ctx);
+/* bytes of device workspace currently held by the context */
+int64_t mexct_ctx_workspace_bytes(const mexct_ctx* ctx);
+const char* mexct_last_error(void);
+int mexct_version(void);
+
+/* ------------------------------------------------------------------ FP
+ * fp = CUDAmex_FP(vol, SAD, angles, nProjections, geomFlag)
+ * replaces projection_kernels/CUDAmex_FP.cu:350-477 (kernels :56-346).
+ * vol [W,W,H] -> out [W,H,nProj].  Cone marches with step 1.0, par/fan 0.5,
+ * clamp addressing.  Detector rows iy >= 32*ceil(W/32) stay 0 as in the
+ * reference's grid (CUDAmex_FP.cu:432-440).  n_angles >= nProj is required
+ * (the reference reads past the end otherwise). */
+int mexct_fp(const float* vol, int64_t W, int64_t H, float sad, const float* angles,
+             int64_t n_angles, int32_t nProj, int32_t geom, float* out, const mexct_opts* opts);
+int mexct_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
+                 float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+                 const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ BP
+ * bp = CUDAmex_BP(P, SAD, angles, geomFlag)
+ * replaces projection_kernels/CUDAmex_BP.cu:223-324 (kernels :44-221).
+ * sino [W,H,nProj] -> out [W,W,H]; voxels outside the circle
+ * (ix-hw)^2+(iy-hw)^2 < hw^2, hw = W/2-1, are 0.  Views accumulate in
+ * view order; no 1/nProj or pi factor (as the reference). */
+int mexct_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, float sad,
+             const float* angles, int64_t n_angles, int32_t geom, float* out,
+             const mexct_opts* opts);
+int mexct_bp_dev(mexct_ctx* ctx, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
+                 int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+                 const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ general-3D FP
+ * fp = CUDAmex_general3D_FP(vol, angles, pts, nProjections)
+ * replaces projection_kernels/CUDAmex_general3D_FP.cu:154-252 (kernel :59-150).
+ * vol [side,side,nSlices], pts [W,H,6] rotated about z by each angle,
+ * step 0.5, border addressing -> out [W,H,nProj].
+ * pts_per_view = 1 is an additive extension: pts is [W,H,6,nProj] and
+ * view k uses its own slice (still rotated by angles[k]). */
+int mexct_g3d_fp(const float* vol, int64_t side, int64_t nSlices, const float* angles,
+                 int64_t n_angles, const float* pts, int64_t W, int64_t H, int32_t nProj,
+                 int32_t pts_per_view, float* out, const mexct_opts* opts);
+int mexct_g3d_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t side,
+                     int64_t nSlices, const float* angles, const float* pts_dev, int64_t W,
+                     int64_t H, int32_t nProj, int32_t pts_per_view, float* out,
+                     const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ general-3D BP
+ * bp = CUDAmex_general3D_BP(P, angles, pts, M, N)
+ * replaces projection_kernels/CUDAmex_general3D_BP.cu:194-282 (kernel :61-188).
+ * Ray-driven scatter with float atomics into [M,M,N]; two corners per
+ * 0.5-voxel step with the reference's weights.  Summation order is not
+ * fixed (as the reference's atomicAdd). */
+int mexct_g3d_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, const float* angles,
+                 int64_t n_angles, const float* pts, int32_t pts_per_view, int64_t M, int64_t N,
+                 float* out, const mexct_opts* opts);
+int mexct_g3d_bp_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                     const float* angles, const float* pts_dev, int32_t pts_per_view, int64_t M,
+                     int64_t N, float* out, const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ OSC
+ * CUDAmex_oscIter(mu, pre, post, numSubsets, subsetPicks, angles, SAD, geomFlag)
+ * replaces iterative_reconstruction/CUDAmex_oscIter.cu:362-584
+ * (kernels :94-358).  mu [W,W,H], pre/post [W,H,nProj] are updated IN
+ * PLACE exactly as the reference mutates its MATLAB inputs: for every
+ * processed view pre <- pre*exp(-l) - post and post <- pre*exp(-l)*l.
+ * picks are 0-based int32 subset offsets (n_picks >= numSubsets). */
+int mexct_osc_iter(float* mu, int64_t W, int64_t H, float* pre, float* post, int64_t nProj,
+                   int32_t numSubsets, const int32_t* picks, int64_t n_picks,
+                   const float* angles, int64_t n_angles, float sad, int32_t geom,
+                   const mexct_opts* opts);
+int mexct_osc_iter_dev(mexct_ctx* ctx, float* mu, int64_t W, int64_t H, float* pre,
+                       float* post, int64_t nProj, int32_t numSubsets, const int32_t* picks,
+                       const float* angles, float sad, int32_t geom, const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ TV
+ * out = CUDAmex_TVmin_3D(vol, c)
+ * replaces iterative_reconstruction/CUDAmex_TVmin_3D.cu:162-246
+ * (kernel :101-158).  opts->tv_steps steps (default 20) of
+ * x <- x - (c/|g|) g; |g| is reduced in FP64 on the device.  The gradient
+ * border, uninitialised in the reference, is 0. */
+int mexct_tvmin(const float* vol, int64_t Nx, int64_t Ny, int64_t Nz, float c, float* out,
+                const mexct_opts* opts);
+int mexct_tvmin_dev(mexct_ctx* ctx, float* vol, int64_t Nx, int64_t Ny, int64_t Nz, float c,
+                    const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ FDK filter
+ * filtered = preFilterSinogram(sino, geom, angles, filter, cutoff) without
+ * Parker weighting (fbp_reconstruction/preFilterSinogram.m:29-152):
+ * cosine weight (fan/cone), band-limited ramp x window on a zero-padded
+ * FFT grid of order max(64, 2^nextpow2(2W)), then x (angle step)/2.
+ * window: 0 ramp, 1 shepp-logan, 2 cosine, 3 hamming, 4 hann, 5 none.
+ * The convolution is evaluated in FP64 on the device. */
+int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int32_t geom,
+                    double sad, const float* angles, int64_t n_angles, int32_t window,
+                    double cutoff, float* out);
+int mexct_prefilter_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                        int32_t geom, double sad, const float* angles, int32_t window,
+                        double cutoff, float* out);
+
+/* ------------------------------------------------------------------ OSC-TV driver
+ * recon = OSC_TV_recon(pre, post, angles, geom, nIter, TV, sInit, sFinal, p)
+ * (iterative_reconstruction/OSC_TV_recon.m:1-102) with all state resident
+ * on the device.  first_picks[k] (1-based, k < nIter) replaces MATLAB's
+ * randperm(n,1) draw of iteration k.  pre/post are not modified. */
+int mexct_osc_tv_recon(const float* pre, const float* post, int64_t W, int64_t H,
+                       int64_t nProj, const float* angles, int64_t n_angles, float sad,
+                       int32_t geom, int32_t nIter, float tv, int32_t sInit, int32_t sFinal,
+                       double p, const int32_t* first_picks, float* mu_out,
+                       const mexct_opts* opts);
+int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, int64_t W,
+                           int64_t H, int64_t nProj, const float* angles, float sad,
+                           int32_t geom, int32_t nIter, float tv, int32_t sInit,
+                           int32_t sFinal, double p, const int32_t* first_picks, float* mu_out,
+                           const mexct_opts* opts);
+
+/* OSC_TV_recon.m:323-349 helpers (host only). */
+int mexct_calc_num_subsets(int32_t sInit, int32_t sFinal, double p, int32_t K, int32_t k);
+int mexct_calc_subset_order(int32_t n, int32_t first_pick, int32_t* picks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEXCT_H */

@@ -1,0 +1,974 @@
+// api.hip -- the C ABI (include/mexct.h): argument checks, workspace,
+// per-view geometry tables and kernel launches.
+//
+// The reference's mexFunctions allocate, copy and free device memory on
+// every call and synchronise after every view.  Here a context owns a
+// stream and a grow-only workspace reused across calls; one call is a short
+// fixed sequence of launches on that stream (halo/convert pass, projector
+// kernel), and the host-pointer entry points add one copy in and one copy
+// out around it.
+#include "../../include/mexct.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+using namespace mexct;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+// "HIP ERROR: <call> = <code> (<string>) at (<file>:<line>)", the format of
+// the reference's CHECK_CUDA_ERROR (CUDAmex_FP.cu:43-50).
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(MEXCT_EHIP, "HIP ERROR: %s = %d (%s) at (%s:%d)", #x, (int)e_,              \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                                  \
+  } while (0)
+
+#define RC(x)              \
+  do {                     \
+    int r_ = (x);          \
+    if (r_) return r_;     \
+  } while (0)
+
+enum Slot {
+  WS_HOST_A,  // host-entry staging of the first array argument
+  WS_HOST_B,
+  WS_HOST_C,
+  WS_HOST_OUT,
+  WS_VOL,      // haloed volume (storage dtype)
+  WS_SINO,     // haloed sinogram (storage dtype)
+  WS_PTS,
+  WS_PREPAD,
+  WS_POSTPAD,
+  WS_MUPAD,
+  WS_MUOLD,
+  WS_TVG,
+  WS_PARTIALS,
+  WS_SCALARS,
+  WS_FILT,
+  WS_WGT,
+  WS_DA,
+  WS_PRE0,
+  WS_POST0,
+  WS_PRE2,
+  WS_POST2,
+  WS_PREW,
+  WS_POSTW,
+  WS_PLAIN,
+  WS_COUNT
+};
+
+struct DevGuard {
+  int prev = -1;
+  explicit DevGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DevGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+int store_bytes(int dt) { return dt == MEXCT_F32 ? 4 : 2; }
+
+}  // namespace
+
+struct mexct_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+  } ws[WS_COUNT];
+  // per-view (sin, cos) table cache, keyed by its host contents
+  std::vector<float2> sc_key;
+  float2* sc_dev = nullptr;
+  size_t sc_cap = 0;
+  void* pinned = nullptr;
+  size_t pinned_cap = 0;
+  // FDK filter cache
+  int f_W = -1, f_window = -1;
+  double f_cut = 0.0;
+  int w_W = -1, w_H = -1, w_geom = -1;
+  double w_sad = 0.0;
+
+  int get(int slot, size_t bytes, void** out) {
+    Buf& b = ws[slot];
+    if (bytes == 0) bytes = 256;
+    if (b.cap < bytes) {
+      if (b.p) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+      }
+      size_t cap = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+      hipError_t e = hipMalloc(&b.p, cap);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return fail(MEXCT_ENOMEM, "device allocation of %zu bytes failed (%s)", cap,
+                    hipGetErrorString(e));
+      }
+      b.cap = cap;
+    }
+    *out = b.p;
+    return MEXCT_OK;
+  }
+
+  // Stage `bytes` of host data through pinned memory into device memory.
+  int upload(void* dst, const void* src, size_t bytes) {
+    if (pinned_cap < bytes) {
+      HIPCHK(hipStreamSynchronize(stream));
+      if (pinned) HIPCHK(hipHostFree(pinned));
+      pinned = nullptr;
+      pinned_cap = 0;
+      size_t cap = std::max<size_t>(bytes, 1 << 16);
+      HIPCHK(hipHostMalloc(&pinned, cap, hipHostMallocDefault));
+      pinned_cap = cap;
+    } else {
+      HIPCHK(hipStreamSynchronize(stream));  // the previous upload may still read it
+    }
+    memcpy(pinned, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, pinned, bytes, hipMemcpyHostToDevice, stream));
+    return MEXCT_OK;
+  }
+
+  // Device table of (sinf, cosf) of each angle, from host sincosf on the
+  // float angle as the reference's host loop (CUDAmex_FP.cu:447-449).
+  int sincos_table(const float* angles, int n, const float2** out) {
+    std::vector<float2> h((size_t)std::max(n, 0));
+    for (int i = 0; i < n; ++i) sincosf(angles[i], &h[i].x, &h[i].y);
+    if (sc_dev && h.size() == sc_key.size() &&
+        (h.empty() || memcmp(h.data(), sc_key.data(), h.size() * sizeof(float2)) == 0)) {
+      *out = sc_dev;
+      return MEXCT_OK;
+    }
+    if (sc_cap < h.size() || !sc_dev) {
+      if (sc_dev) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipFree(sc_dev));
+      }
+      sc_dev = nullptr;
+      size_t cap = std::max<size_t>(h.size(), 1024);
+      HIPCHK(hipMalloc((void**)&sc_dev, cap * sizeof(float2)));
+      sc_cap = cap;
+    }
+    if (!h.empty()) RC(upload(sc_dev, h.data(), h.size() * sizeof(float2)));
+    sc_key.swap(h);
+    *out = sc_dev;
+    return MEXCT_OK;
+  }
+
+  ~mexct_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& b : ws)
+      if (b.p) (void)hipFree(b.p);
+    if (sc_dev) (void)hipFree(sc_dev);
+    if (pinned) (void)hipHostFree(pinned);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int ctx_init(mexct_ctx* c, int device, void* stream) {
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fail(MEXCT_EINVAL, "device %d out of range (%d devices)", device, ndev);
+  c->device = device;
+  DevGuard g(device);
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+  }
+  return MEXCT_OK;
+}
+
+// Implicit per-thread context of the host-pointer entry points.
+thread_local std::vector<std::unique_ptr<mexct_ctx>> t_ctx;
+
+int default_ctx(mexct_ctx** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if ((int)t_ctx.size() <= dev) t_ctx.resize(dev + 1);
+  if (!t_ctx[dev]) {
+    std::unique_ptr<mexct_ctx> c(new mexct_ctx());
+    RC(ctx_init(c.get(), dev, nullptr));
+    t_ctx[dev] = std::move(c);
+  }
+  *out = t_ctx[dev].get();
+  return MEXCT_OK;
+}
+
+int check_geom(int geom) {
+  if (geom < 0 || geom > 2) return fail(MEXCT_EGEOM, "invalid geometry");
+  return MEXCT_OK;
+}
+
+int check_dtype(int dt, const char* what) {
+  if (dt != MEXCT_F32 && dt != MEXCT_F16 && dt != MEXCT_BF16)
+    return fail(MEXCT_EINVAL, "%s: unknown dtype %d", what, dt);
+  return MEXCT_OK;
+}
+
+// Padded operands are indexed with 32-bit element offsets.
+int check_padded(int64_t a, int64_t b, int64_t c, const char* what) {
+  double n = (double)(a + 2 * HALO) * (double)(b + 2 * HALO) * (double)(c + 2 * HALO);
+  if (n >= 2147483648.0) return fail(MEXCT_EINVAL, "%s too large (%g padded elements)", what, n);
+  return MEXCT_OK;
+}
+
+int storage_of(const mexct_opts* o) { return o ? o->storage : MEXCT_F32; }
+
+int tv_steps_of(const mexct_opts* o) { return (o && o->tv_steps > 0) ? o->tv_steps : 20; }
+
+int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
+  if (o && (o->zrange == 32 || o->zrange == 64)) return o->zrange;
+  (void)geom;
+  return H >= 256 ? 64 : 32;
+}
+
+// ---- FDK filter design (preFilterSinogram.m:95-152), host FP64
+int filter_order(int len) {
+  int o = 1;
+  while (o < 2 * len) o <<= 1;
+  return std::max(64, o);
+}
+
+int design_filter(int window, int len, double d, std::vector<double>& full) {
+  const int order = filter_order(len);
+  full.assign(order, 1.0);
+  if (window == 5) return order;
+  if (window < 0 || window > 4) return -1;
+  const int half = order / 2;
+  // band-limited ramp impulse response, Kak & Slaney eq. 61 (:117-121)
+  std::vector<double> imp(order, 0.0);
+  imp[0] = 0.25;
+  for (int n = 1; n <= half; n += 2) {
+    double v = -1.0 / ((M_PI * n) * (M_PI * n));
+    imp[n] = v;
+    if (n < half) imp[order - n] = v;
+  }
+  // 2*real(fft(imp)) on bins 0..half (:122-123)
+  std::vector<double> filt(half + 1);
+  for (int j = 0; j <= half; ++j) {
+    double s = 0.0;
+    for (int m = 0; m < order; ++m) {
+      if (imp[m] == 0.0) continue;
+      long long jm = ((long long)j * m) % order;
+      s += imp[m] * cos(2.0 * M_PI * (double)jm / (double)order);
+    }
+    filt[j] = 2.0 * s;
+  }
+  for (int j = 1; j <= half; ++j) {
+    const double w = 2.0 * M_PI * j / order;
+    switch (window) {
+      case 1: filt[j] *= sin(w / (2 * d)) / (w / (2 * d)); break;
+      case 2: filt[j] *= cos(w / (2 * d)); break;
+      case 3: filt[j] *= (.54 + .46 * cos(w / d)); break;
+      case 4: filt[j] *= (1 + cos(w / d)) / 2; break;
+      default: break;
+    }
+  }
+  for (int j = 0; j <= half; ++j)
+    if (2.0 * M_PI * j / order > M_PI * d) filt[j] = 0.0;
+  for (int j = 0; j <= half; ++j) full[j] = filt[j];
+  for (int j = half + 1; j < order; ++j) full[j] = filt[order - j];
+  return order;
+}
+
+// Circular impulse response of the (real, even) filter and its taps
+// g[d + W - 1] = h[d mod order], d in [-(W-1), W-1].
+void conv_taps(const std::vector<double>& full, int W, std::vector<double>& g) {
+  const int order = (int)full.size();
+  std::vector<double> h(order);
+  const int half = order / 2;
+  for (int m = 0; m < order; ++m) {
+    double s = full[0] + ((m & 1) ? -full[half] : full[half]);
+    for (int j = 1; j < half; ++j) {
+      long long jm = ((long long)j * m) % order;
+      s += 2.0 * full[j] * cos(2.0 * M_PI * (double)jm / (double)order);
+    }
+    h[m] = s / (double)order;
+  }
+  g.resize(2 * W - 1);
+  for (int d = -(W - 1); d <= W - 1; ++d) g[d + W - 1] = h[((d % order) + order) % order];
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int mexct_version(void) { return 10000; }
+
+const char* mexct_last_error(void) { return g_err.c_str(); }
+
+int mexct_ctx_create(mexct_ctx** ctx, int device, void* stream) {
+  if (!ctx) return fail(MEXCT_EINVAL, "ctx is null");
+  std::unique_ptr<mexct_ctx> c(new mexct_ctx());
+  RC(ctx_init(c.get(), device, stream));
+  *ctx = c.release();
+  return MEXCT_OK;
+}
+
+int mexct_ctx_destroy(mexct_ctx* ctx) {
+  if (!ctx) return MEXCT_OK;
+  DevGuard g(ctx->device);
+  delete ctx;
+  return MEXCT_OK;
+}
+
+int mexct_ctx_sync(mexct_ctx* ctx) {
+  if (!ctx) return fail(MEXCT_EINVAL, "ctx is null");
+  DevGuard g(ctx->device);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MEXCT_OK;
+}
+
+int64_t mexct_ctx_workspace_bytes(const mexct_ctx* ctx) {
+  if (!ctx) return 0;
+  int64_t s = 0;
+  for (auto& b : ctx->ws) s += (int64_t)b.cap;
+  return s + (int64_t)ctx->sc_cap * 8;
+}
+
+// ------------------------------------------------------------------ FP
+int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
+                 float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+                 const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_geom(geom));
+  if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims W=%lld H=%lld nProj=%d", (long long)W, (long long)H, nProj);
+  if (nProj > 0 && (!vol || !angles || !out)) return fail(MEXCT_EINVAL, "null array argument");
+  RC(check_dtype(vol_dtype, "vol"));
+  const int st = storage_of(opts);
+  RC(check_dtype(st, "storage"));
+  RC(check_padded(W, W, H, "volume"));
+  if (nProj == 0) return MEXCT_OK;
+  DevGuard g(c->device);
+  void* pad;
+  RC(c->get(WS_VOL, (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st), &pad));
+  HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
+  const float2* sc;
+  RC(c->sincos_table(angles, nProj, &sc));
+  FPLaunch a;
+  a.vol = pad;
+  a.st_dt = st;
+  a.out = out;
+  a.sc = sc;
+  a.geom = geom;
+  a.mode = ADDR_CLAMP;
+  a.W = (int)W;
+  a.H = (int)H;
+  a.nviews = nProj;
+  a.NX = (int)W;
+  a.NY = (int)W;
+  a.NZ = (int)H;
+  a.sad = sad;
+  a.step = geom == GEOM_CONE ? 1.0f : 0.5f;  // CUDAmex_FP.cu:133,231,328
+  a.pts = nullptr;
+  a.pts_view_stride = 0;
+  HIPCHK(launch_fp(a, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_fp(const float* vol, int64_t W, int64_t H, float sad, const float* angles,
+             int64_t n_angles, int32_t nProj, int32_t geom, float* out, const mexct_opts* opts) {
+  RC(check_geom(geom));
+  if (nProj < 0 || n_angles < nProj)
+    return fail(MEXCT_EINVAL, "nProj (%d) exceeds the number of angles (%lld)", nProj, (long long)n_angles);
+  if (W < 1 || H < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!vol || (nProj > 0 && (!angles || !out))) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t vb = (size_t)W * W * H * 4, ob = (size_t)W * H * nProj * 4;
+  void *dv, *dout;
+  RC(c->get(WS_HOST_A, vb, &dv));
+  RC(c->get(WS_HOST_OUT, ob, &dout));
+  HIPCHK(hipMemcpyAsync(dv, vol, vb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_fp_dev(c, dv, MEXCT_F32, W, H, sad, angles, nProj, geom, (float*)dout, opts));
+  if (ob) HIPCHK(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ BP
+int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
+                 int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+                 const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_geom(geom));
+  if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (!out || (nProj > 0 && (!sino || !angles))) return fail(MEXCT_EINVAL, "null array argument");
+  RC(check_dtype(sino_dtype, "sino"));
+  const int st = storage_of(opts);
+  RC(check_dtype(st, "storage"));
+  RC(check_padded(W, H, nProj, "sinogram"));
+  if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
+  DevGuard g(c->device);
+  void* pad;
+  RC(c->get(WS_SINO, (size_t)(W + 4) * (H + 4) * std::max<int64_t>(nProj, 1) * store_bytes(st), &pad));
+  if (nProj > 0)
+    HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream));
+  const float2* sc = nullptr;
+  RC(c->sincos_table(angles, (int)nProj, &sc));
+  BPLaunch a;
+  a.sino = pad;
+  a.st_dt = st;
+  a.out = out;
+  a.sc = sc;
+  a.geom = geom;
+  a.W = (int)W;
+  a.H = (int)H;
+  a.nviews = (int)nProj;
+  a.sad = sad;
+  a.zr = pick_zrange(opts, geom, H);
+  HIPCHK(launch_bp(a, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, float sad,
+             const float* angles, int64_t n_angles, int32_t geom, float* out,
+             const mexct_opts* opts) {
+  RC(check_geom(geom));
+  if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (n_angles < nProj)
+    return fail(MEXCT_EINVAL, "nProj (%lld) exceeds the number of angles (%lld)", (long long)nProj, (long long)n_angles);
+  if (!out || (nProj > 0 && (!sino || !angles))) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t sb = (size_t)W * H * nProj * 4, ob = (size_t)W * W * H * 4;
+  void *ds, *dout;
+  RC(c->get(WS_HOST_A, sb, &ds));
+  RC(c->get(WS_HOST_OUT, ob, &dout));
+  if (sb) HIPCHK(hipMemcpyAsync(ds, sino, sb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_bp_dev(c, ds, MEXCT_F32, W, H, nProj, sad, angles, geom, (float*)dout, opts));
+  HIPCHK(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ general-3D FP
+int mexct_g3d_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t side,
+                     int64_t nSlices, const float* angles, const float* pts_dev, int64_t W,
+                     int64_t H, int32_t nProj, int32_t ppv, float* out, const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  if (side < 1 || nSlices < 1 || W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (nProj > 0 && (!vol || !angles || !pts_dev || !out)) return fail(MEXCT_EINVAL, "null array argument");
+  RC(check_dtype(vol_dtype, "vol"));
+  const int st = storage_of(opts);
+  RC(check_dtype(st, "storage"));
+  RC(check_padded(side, side, nSlices, "volume"));
+  if (nProj == 0) return MEXCT_OK;
+  DevGuard g(c->device);
+  void* pad;
+  RC(c->get(WS_VOL, (size_t)(side + 4) * (side + 4) * (nSlices + 4) * store_bytes(st), &pad));
+  HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)side, (int)side, (int)nSlices, ADDR_BORDER, c->stream));
+  const float2* sc;
+  RC(c->sincos_table(angles, nProj, &sc));
+  FPLaunch a;
+  a.vol = pad;
+  a.st_dt = st;
+  a.out = out;
+  a.sc = sc;
+  a.geom = GEOM_G3D;
+  a.mode = ADDR_BORDER;
+  a.W = (int)W;
+  a.H = (int)H;
+  a.nviews = nProj;
+  a.NX = (int)side;
+  a.NY = (int)side;
+  a.NZ = (int)nSlices;
+  a.sad = 0.f;
+  a.step = 0.5f;  // CUDAmex_general3D_FP.cu:132
+  a.pts = pts_dev;
+  a.pts_view_stride = ppv ? (int64_t)6 * W * H : 0;
+  HIPCHK(launch_fp(a, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_g3d_fp(const float* vol, int64_t side, int64_t nSlices, const float* angles,
+                 int64_t n_angles, const float* pts, int64_t W, int64_t H, int32_t nProj,
+                 int32_t ppv, float* out, const mexct_opts* opts) {
+  if (nProj < 0 || n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  if (side < 1 || nSlices < 1 || W < 1 || H < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!vol || !pts || (nProj > 0 && (!angles || !out))) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t vb = (size_t)side * side * nSlices * 4;
+  const size_t pb = (size_t)W * H * 6 * 4 * (ppv ? std::max(nProj, 1) : 1);
+  const size_t ob = (size_t)W * H * nProj * 4;
+  void *dv, *dp, *dout;
+  RC(c->get(WS_HOST_A, vb, &dv));
+  RC(c->get(WS_HOST_B, pb, &dp));
+  RC(c->get(WS_HOST_OUT, ob, &dout));
+  HIPCHK(hipMemcpyAsync(dv, vol, vb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dp, pts, pb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_g3d_fp_dev(c, dv, MEXCT_F32, side, nSlices, angles, (const float*)dp, W, H, nProj, ppv,
+                      (float*)dout, opts));
+  if (ob) HIPCHK(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ general-3D BP
+int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                     const float* angles, const float* pts_dev, int32_t ppv, int64_t M, int64_t N,
+                     float* out, const mexct_opts* opts) {
+  (void)opts;
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  if (W < 1 || H < 1 || nProj < 0 || M < 1 || N < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!out || (nProj > 0 && (!sino || !angles || !pts_dev))) return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  HIPCHK(launch_fill(out, 0.f, (size_t)M * M * N, c->stream));
+  if (nProj == 0) return MEXCT_OK;
+  const float2* sc;
+  RC(c->sincos_table(angles, (int)nProj, &sc));
+  G3DBPLaunch a;
+  a.sino = sino;
+  a.pts = pts_dev;
+  a.pts_view_stride = ppv ? (int64_t)6 * W * H : 0;
+  a.sc = sc;
+  a.out = out;
+  a.W = (int)W;
+  a.H = (int)H;
+  a.nviews = (int)nProj;
+  a.M = (int)M;
+  a.N = (int)N;
+  HIPCHK(launch_g3d_bp(a, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_g3d_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, const float* angles,
+                 int64_t n_angles, const float* pts, int32_t ppv, int64_t M, int64_t N,
+                 float* out, const mexct_opts* opts) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  if (W < 1 || H < 1 || nProj < 0 || M < 1 || N < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!out || !pts || (nProj > 0 && (!sino || !angles))) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t sb = (size_t)W * H * nProj * 4;
+  const size_t pb = (size_t)W * H * 6 * 4 * (ppv ? std::max<int64_t>(nProj, 1) : 1);
+  const size_t ob = (size_t)M * M * N * 4;
+  void *ds, *dp, *dout;
+  RC(c->get(WS_HOST_A, sb, &ds));
+  RC(c->get(WS_HOST_B, pb, &dp));
+  RC(c->get(WS_HOST_OUT, ob, &dout));
+  if (sb) HIPCHK(hipMemcpyAsync(ds, sino, sb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dp, pts, pb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_g3d_bp_dev(c, (const float*)ds, W, H, nProj, angles, (const float*)dp, ppv, M, N,
+                      (float*)dout, opts));
+  HIPCHK(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ OSC
+namespace {
+
+// One CUDAmex_oscIter pass over the subsets on the device state: mu_pad is
+// the zero-haloed volume, pre/post [W,H,nProj] are rewritten in place.
+int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* post, int nProj,
+                int numSubsets, const int32_t* picks, const float2* sc, float sad, int geom) {
+  const int cmax = (nProj + numSubsets - 1) / numSubsets;
+  const size_t pv = (size_t)(W + 4) * (H + 4);
+  void *pp, *qp;
+  RC(c->get(WS_PREPAD, pv * std::max(cmax, 1) * 4, &pp));
+  RC(c->get(WS_POSTPAD, pv * std::max(cmax, 1) * 4, &qp));
+  // halos stay zero: the kernels only ever write the interior
+  HIPCHK(launch_fill((float*)pp, 0.f, pv * std::max(cmax, 1), c->stream));
+  HIPCHK(launch_fill((float*)qp, 0.f, pv * std::max(cmax, 1), c->stream));
+  for (int s = 0; s < numSubsets; ++s) {
+    const int sel = picks[s];
+    const int count = (sel >= 0 && sel < nProj) ? (nProj - 1 - sel) / numSubsets + 1 : 0;
+    if (count > 0) {
+      OSCFPLaunch f;
+      f.mu_pad = mu_pad;
+      f.pre = pre;
+      f.post = post;
+      f.pre_pad = (float*)pp;
+      f.post_pad = (float*)qp;
+      f.sc = sc;
+      f.geom = geom;
+      f.W = W;
+      f.H = H;
+      f.nProj = nProj;
+      f.first = sel;
+      f.stride = numSubsets;
+      f.count = count;
+      f.sad = sad;
+      HIPCHK(launch_osc_fp(f, c->stream));
+    }
+    OSCBPLaunch b;
+    b.pre_pad = (const float*)pp;
+    b.post_pad = (const float*)qp;
+    b.mu_pad = mu_pad;
+    b.sc = sc;
+    b.geom = geom;
+    b.W = W;
+    b.H = H;
+    b.first = sel;
+    b.stride = numSubsets;
+    b.count = count;
+    b.sad = sad;
+    HIPCHK(launch_osc_bp_update(b, c->stream));
+  }
+  return MEXCT_OK;
+}
+
+int osc_check(int64_t W, int64_t H, int64_t nProj, int32_t numSubsets, int32_t geom) {
+  RC(check_geom(geom));
+  if (W < 2 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (numSubsets < 1) return fail(MEXCT_EINVAL, "numSubsets must be >= 1 (got %d)", numSubsets);
+  RC(check_padded(W, W, H, "volume"));
+  return MEXCT_OK;
+}
+
+}  // namespace
+
+int mexct_osc_iter_dev(mexct_ctx* c, float* mu, int64_t W, int64_t H, float* pre, float* post,
+                       int64_t nProj, int32_t numSubsets, const int32_t* picks,
+                       const float* angles, float sad, int32_t geom, const mexct_opts* opts) {
+  (void)opts;
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(osc_check(W, H, nProj, numSubsets, geom));
+  if (!mu || !picks || (nProj > 0 && (!pre || !post || !angles)))
+    return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  void* mp;
+  RC(c->get(WS_MUPAD, (size_t)(W + 4) * (W + 4) * (H + 4) * 4, &mp));
+  HIPCHK(launch_pad3(mu, MEXCT_F32, mp, MEXCT_F32, (int)W, (int)W, (int)H, ADDR_BORDER, c->stream));
+  const float2* sc = nullptr;
+  RC(c->sincos_table(angles, (int)nProj, &sc));
+  RC(osc_subsets(c, (float*)mp, (int)W, (int)H, pre, post, (int)nProj, numSubsets, picks, sc, sad,
+                 geom));
+  HIPCHK(launch_unpad3((const float*)mp, mu, (int)W, (int)W, (int)H, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_osc_iter(float* mu, int64_t W, int64_t H, float* pre, float* post, int64_t nProj,
+                   int32_t numSubsets, const int32_t* picks, int64_t n_picks, const float* angles,
+                   int64_t n_angles, float sad, int32_t geom, const mexct_opts* opts) {
+  RC(osc_check(W, H, nProj, numSubsets, geom));
+  if (n_picks < numSubsets) return fail(MEXCT_EINVAL, "fewer subset picks than subsets");
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  if (!mu || !picks || (nProj > 0 && (!pre || !post || !angles)))
+    return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t vb = (size_t)W * W * H * 4, pb = (size_t)W * H * nProj * 4;
+  void *dm, *dpre, *dpost;
+  RC(c->get(WS_HOST_A, vb, &dm));
+  RC(c->get(WS_HOST_B, pb, &dpre));
+  RC(c->get(WS_HOST_C, pb, &dpost));
+  HIPCHK(hipMemcpyAsync(dm, mu, vb, hipMemcpyHostToDevice, c->stream));
+  if (pb) {
+    HIPCHK(hipMemcpyAsync(dpre, pre, pb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dpost, post, pb, hipMemcpyHostToDevice, c->stream));
+  }
+  RC(mexct_osc_iter_dev(c, (float*)dm, W, H, (float*)dpre, (float*)dpost, nProj, numSubsets, picks,
+                        angles, sad, geom, opts));
+  HIPCHK(hipMemcpyAsync(mu, dm, vb, hipMemcpyDeviceToHost, c->stream));
+  if (pb) {
+    HIPCHK(hipMemcpyAsync(pre, dpre, pb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(post, dpost, pb, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ TV
+namespace {
+int tv_run(mexct_ctx* c, Vol3 x, const float* c_dev, float cc, int steps) {
+  void *gp, *pp, *sp;
+  RC(c->get(WS_TVG, (size_t)x.nx * x.ny * x.nz * 4, &gp));
+  RC(c->get(WS_PARTIALS, tv_partials_count(x.nx, x.ny, x.nz) * 8, &pp));
+  RC(c->get(WS_SCALARS, 256, &sp));
+  HIPCHK(launch_tvmin(x, (float*)gp, (double*)pp, (float*)sp, c_dev, cc, steps, c->stream));
+  return MEXCT_OK;
+}
+}  // namespace
+
+int mexct_tvmin_dev(mexct_ctx* c, float* vol, int64_t Nx, int64_t Ny, int64_t Nz, float cc,
+                    const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  if (Nx < 1 || Ny < 1 || Nz < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!vol) return fail(MEXCT_EINVAL, "null array argument");
+  if (Nz > 65535) return fail(MEXCT_EINVAL, "Nz too large");
+  DevGuard g(c->device);
+  Vol3 x{vol, (int)Nx, (int)Ny, (int)Nz, (int64_t)Nx, (int64_t)Nx * Ny};
+  return tv_run(c, x, nullptr, cc, tv_steps_of(opts));
+}
+
+int mexct_tvmin(const float* vol, int64_t Nx, int64_t Ny, int64_t Nz, float cc, float* out,
+                const mexct_opts* opts) {
+  if (Nx < 1 || Ny < 1 || Nz < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!vol || !out) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t vb = (size_t)Nx * Ny * Nz * 4;
+  void* dv;
+  RC(c->get(WS_HOST_A, vb, &dv));
+  HIPCHK(hipMemcpyAsync(dv, vol, vb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_tvmin_dev(c, (float*)dv, Nx, Ny, Nz, cc, opts));
+  HIPCHK(hipMemcpyAsync(out, dv, vb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ FDK filter
+int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                        int32_t geom, double sad, const float* angles, int32_t window,
+                        double cutoff, float* out) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_geom(geom));
+  if (W < 1 || H < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (nProj < 2) return fail(MEXCT_EINVAL, "the angle step needs at least 2 views");
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  if (window < 0 || window > 5) return fail(MEXCT_EINVAL, "unknown filter window %d", window);
+  if (3 * W - 1 > 16384) return fail(MEXCT_EINVAL, "detector too wide for the LDS filter (W=%lld)", (long long)W);
+  DevGuard g(c->device);
+  void *gp, *wp = nullptr, *dp;
+  RC(c->get(WS_FILT, sizeof(double) * (2 * W - 1), &gp));
+  if (c->f_W != W || c->f_window != window || c->f_cut != cutoff) {
+    std::vector<double> full, taps;
+    design_filter(window, (int)W, cutoff, full);
+    conv_taps(full, (int)W, taps);
+    RC(c->upload(gp, taps.data(), taps.size() * sizeof(double)));
+    c->f_W = (int)W;
+    c->f_window = window;
+    c->f_cut = cutoff;
+  }
+  if (geom != GEOM_PAR) {
+    RC(c->get(WS_WGT, sizeof(float) * W * H, &wp));
+    if (c->w_W != W || c->w_H != H || c->w_geom != geom || c->w_sad != sad) {
+      // Distance = SAD ./ sqrt(SAD^2 + U.^2 [+ V.^2]), U = (1..W) - (W-1)/2
+      // (preFilterSinogram.m:40-42, :54-57), rounded to single.
+      std::vector<float> w((size_t)W * H);
+      for (int64_t v = 0; v < H; ++v)
+        for (int64_t u = 0; u < W; ++u) {
+          const double U = (double)(u + 1) - (double)(W - 1) / 2.0;
+          const double V = (double)(v + 1) - (double)(H - 1) / 2.0;
+          const double D = geom == GEOM_CONE ? sad / sqrt(sad * sad + U * U + V * V)
+                                             : sad / sqrt(sad * sad + U * U);
+          w[(size_t)u + (size_t)W * v] = (float)D;
+        }
+      RC(c->upload(wp, w.data(), w.size() * sizeof(float)));
+      c->w_W = (int)W;
+      c->w_H = (int)H;
+      c->w_geom = geom;
+      c->w_sad = sad;
+    }
+  }
+  RC(c->get(WS_DA, sizeof(float) * nProj, &dp));
+  std::vector<float> dA((size_t)nProj);
+  for (int64_t i = 0; i < nProj; ++i) dA[i] = i > 0 ? angles[i] - angles[i - 1] : angles[1] - angles[0];
+  RC(c->upload(dp, dA.data(), dA.size() * sizeof(float)));
+  HIPCHK(launch_prefilter(sino, out, (const float*)wp, (const double*)gp, (const float*)dp, (int)W,
+                          (int)H, (int)nProj, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int32_t geom,
+                    double sad, const float* angles, int64_t n_angles, int32_t window,
+                    double cutoff, float* out) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t sb = (size_t)W * H * nProj * 4;
+  void *ds, *dout;
+  RC(c->get(WS_HOST_A, sb, &ds));
+  RC(c->get(WS_HOST_OUT, sb, &dout));
+  HIPCHK(hipMemcpyAsync(ds, sino, sb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_prefilter_dev(c, (const float*)ds, W, H, nProj, geom, sad, angles, window, cutoff,
+                         (float*)dout));
+  HIPCHK(hipMemcpyAsync(out, dout, sb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ OSC-TV driver
+int mexct_calc_num_subsets(int32_t sInit, int32_t sFinal, double p, int32_t K, int32_t k) {
+  // round((sInit - sFinal)/(K-1)^p * (K-k)^p + sFinal)  (OSC_TV_recon.m:323-325)
+  const double v = (double)(sInit - sFinal) / pow((double)(K - 1), p) * pow((double)(K - k), p) +
+                   (double)sFinal;
+  return (int)round(v);
+}
+
+int mexct_calc_subset_order(int32_t n, int32_t first_pick, int32_t* picks) {
+  // calcSubsetOrder (OSC_TV_recon.m:327-349): start at the drawn subset, then
+  // repeatedly take the middle of the larger remaining side.
+  if (n < 1 || !picks) return fail(MEXCT_EINVAL, "bad subset count");
+  if (first_pick < 1 || first_pick > n) return fail(MEXCT_EINVAL, "first pick %d not in 1..%d", first_pick, n);
+  std::vector<int> nums(n);
+  for (int i = 0; i < n; ++i) nums[i] = i + 1;
+  int cur = first_pick;
+  for (int s = 1; s <= n; ++s) {
+    if (s > 1) {
+      std::vector<int> lo, hi;
+      for (int v : nums) {
+        if (v < cur) lo.push_back(v);
+        if (v > cur) hi.push_back(v);
+      }
+      std::vector<int>& side = lo.size() > hi.size() ? lo : hi;
+      cur = side[(side.size() + 1) / 2 - 1];
+      side.erase(std::find(side.begin(), side.end(), cur));
+      nums = lo;
+      nums.insert(nums.end(), hi.begin(), hi.end());
+    }
+    picks[s - 1] = cur - 1;
+  }
+  return MEXCT_OK;
+}
+
+namespace {
+__global__ void clamp_min_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                 float* __restrict__ out2, size_t n) {
+  // pre(pre<epsilon) = epsilon, epsilon = 1e-6 (OSC_TV_recon.m:282-284);
+  // out2 = copy with element 1 set to 0 (pre2(1) = 0, :288)
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float v = in[i];
+    if ((double)v < 1e-6) v = (float)1e-6;
+    out[i] = v;
+    out2[i] = i == 0 ? 0.0f : v;
+  }
+}
+}  // namespace
+
+int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, int64_t W,
+                           int64_t H, int64_t nProj, const float* angles, float sad, int32_t geom,
+                           int32_t nIter, float tv, int32_t sInit, int32_t sFinal, double p,
+                           const int32_t* first_picks, float* mu_out, const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_geom(geom));
+  if (W < 2 || H < 1 || nProj < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (nIter < 2) return fail(MEXCT_EINVAL, "nIter must be >= 2 (calcNumSubsets divides by (nIter-1)^p)");
+  if (!pre || !post || !angles || !first_picks || !mu_out) return fail(MEXCT_EINVAL, "null array argument");
+  RC(check_padded(W, W, H, "volume"));
+  DevGuard g(c->device);
+  hipStream_t st = c->stream;
+  const size_t psz = (size_t)W * H * nProj, vsz = (size_t)W * W * H;
+  const size_t padsz = (size_t)(W + 4) * (W + 4) * (H + 4);
+  void *pre0, *post0, *pre2, *post2, *prew, *postw, *mp, *mo, *plain, *pp, *sp;
+  RC(c->get(WS_PRE0, psz * 4, &pre0));
+  RC(c->get(WS_POST0, psz * 4, &post0));
+  RC(c->get(WS_PRE2, psz * 4, &pre2));
+  RC(c->get(WS_POST2, psz * 4, &post2));
+  RC(c->get(WS_PREW, psz * 4, &prew));
+  RC(c->get(WS_POSTW, psz * 4, &postw));
+  RC(c->get(WS_MUPAD, padsz * 4, &mp));
+  RC(c->get(WS_MUOLD, padsz * 4, &mo));
+  RC(c->get(WS_PLAIN, vsz * 4, &plain));
+  RC(c->get(WS_PARTIALS, tv_partials_count((int)W, (int)W, (int)H) * 8, &pp));
+  RC(c->get(WS_SCALARS, 256, &sp));
+  const unsigned cg = (unsigned)std::min<size_t>((psz + 255) / 256, 65536);
+  clamp_min_kernel<<<cg, 256, 0, st>>>(pre, (float*)pre0, (float*)pre2, psz);
+  clamp_min_kernel<<<cg, 256, 0, st>>>(post, (float*)post0, (float*)post2, psz);
+  HIPCHK(hipGetLastError());
+  // mu = ones(W,W,H,'single') .* 1e-10
+  HIPCHK(launch_fill((float*)plain, (float)(1.0 * 1e-10), vsz, st));
+  HIPCHK(launch_pad3(plain, MEXCT_F32, mp, MEXCT_F32, (int)W, (int)W, (int)H, ADDR_BORDER, st));
+  const float2* sc = nullptr;
+  RC(c->sincos_table(angles, (int)nProj, &sc));
+  const int64_t MPX = W + 4, MPXY = (W + 4) * (W + 4);
+  const int64_t interior = 2 + 2 * MPX + 2 * MPXY;
+  Vol3 mu_v{(float*)mp + interior, (int)W, (int)W, (int)H, MPX, MPXY};
+  Vol3 old_v{(float*)mo + interior, (int)W, (int)W, (int)H, MPX, MPXY};
+  double* sum = (double*)sp;
+  float* cdev = (float*)((char*)sp + 64);
+  float* adev = (float*)((char*)sp + 128);
+  const int steps = tv_steps_of(opts);
+  void* gp;
+  RC(c->get(WS_TVG, vsz * 4, &gp));
+  for (int k = 1; k <= nIter; ++k) {
+    HIPCHK(hipMemcpyAsync(mo, mp, padsz * 4, hipMemcpyDeviceToDevice, st));
+    const int n = mexct_calc_num_subsets(sInit, sFinal, p, nIter, k);
+    if (n < 1) return fail(MEXCT_EINVAL, "iteration %d has %d subsets", k, n);
+    std::vector<int32_t> picks(n);
+    RC(mexct_calc_subset_order(n, first_picks[k - 1], picks.data()));
+    float* wpre = (float*)(k == 1 ? pre0 : prew);
+    float* wpost = (float*)(k == 1 ? post0 : postw);
+    if (k > 1) {
+      HIPCHK(hipMemcpyAsync(prew, pre2, psz * 4, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(postw, post2, psz * 4, hipMemcpyDeviceToDevice, st));
+    }
+    RC(osc_subsets(c, (float*)mp, (int)W, (int)H, wpre, wpost, (int)nProj, n, picks.data(), sc,
+                   sad, geom));
+    if (k > 1) {
+      HIPCHK(launch_diff_norm(old_v, mu_v, (double*)pp, sum, st));
+      HIPCHK(launch_tv_const(sum, tv, cdev, st));
+      HIPCHK(launch_tvmin(mu_v, (float*)gp, (double*)pp, adev, cdev, 0.f, steps, st));
+      HIPCHK(launch_clean(mu_v, st));
+    }
+  }
+  HIPCHK(launch_unpad3((const float*)mp, mu_out, (int)W, (int)W, (int)H, st));
+  return MEXCT_OK;
+}
+
+int mexct_osc_tv_recon(const float* pre, const float* post, int64_t W, int64_t H, int64_t nProj,
+                       const float* angles, int64_t n_angles, float sad, int32_t geom,
+                       int32_t nIter, float tv, int32_t sInit, int32_t sFinal, double p,
+                       const int32_t* first_picks, float* mu_out, const mexct_opts* opts) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  if (W < 2 || H < 1 || nProj < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!pre || !post || !mu_out) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t pb = (size_t)W * H * nProj * 4, vb = (size_t)W * W * H * 4;
+  void *dpre, *dpost, *dout;
+  RC(c->get(WS_HOST_A, pb, &dpre));
+  RC(c->get(WS_HOST_B, pb, &dpost));
+  RC(c->get(WS_HOST_OUT, vb, &dout));
+  HIPCHK(hipMemcpyAsync(dpre, pre, pb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dpost, post, pb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_osc_tv_recon_dev(c, (const float*)dpre, (const float*)dpost, W, H, nProj, angles, sad,
+                            geom, nIter, tv, sInit, sFinal, p, first_picks, (float*)dout, opts));
+  HIPCHK(hipMemcpyAsync(mu_out, dout, vb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// bp.hip -- voxel-driven backprojectors (standard geometries and the OSC
+// dual backprojection with its update) and the general-3D ray-driven
+// scatter, for gfx950.
+//
+// Voxel-driven BP (CUDAmex_BP.cu:44-221): the reference launches one kernel
+// per view and read-modify-writes the whole volume each time (8 B per
+// update).  Here one launch covers every view: a 64-thread workgroup owns an
+// 8x8 tile of voxel columns and a run of ZR consecutive z-voxels, keeps the
+// ZR sums in registers while it walks the views in order, and writes each
+// voxel once.  Every voxel therefore receives its views in the reference's
+// order, starting from 0, so the float sum is the reference's sum.
+//
+// The reference steps the detector row by a running float sum,
+// proj_z += U per slice (CUDAmex_BP.cu:208-219).  For par and fan the
+// running sum holds k + 0.5 exactly and is computed directly; for cone the
+// thread replays the z0 additions that precede its first voxel, so every
+// sample lands exactly where the reference's sample lands.
+#include "common.h"
+#include "internal.h"
+
+namespace mexct {
+
+struct BPKArgs {
+  const void* sino;
+  float* out;
+  const float2* sc;
+  int W, H, nviews, PW;
+  int64_t PV;
+  float v0, SR;
+  uint32_t tiles_x, nz, n_logical;
+};
+
+// per-view detector geometry of one voxel column
+template <int GEOM>
+__device__ __forceinline__ void bp_view_geom(float x, float y, float sB, float cB, float SR,
+                                             float v0, uint32_t hw, float& U, float& py,
+                                             float& pz, float& dz) {
+  float s = x * cB + y * sB;
+  float t = -x * sB + y * cB;
+  if (GEOM == GEOM_PAR) {  // CUDAmex_BP.cu:92-95
+    U = 1.f;
+    py = t * U + (float)hw + 1.f;
+    pz = -v0 * U + v0 + 0.5f;
+    dz = U;
+  } else if (GEOM == GEOM_FAN) {  // :149-154
+    U = SR / (SR - s);
+    py = t * U + (float)hw + 1.f;
+    pz = 0.5f;
+    dz = 1.f;
+  } else {  // cone :208-212
+    U = SR / (SR - s);
+    py = t * U + (float)hw + 1.f;
+    pz = -v0 * U + v0 + 0.5f;
+    dz = U;
+  }
+}
+
+template <int GEOM>
+__device__ __forceinline__ float advance_rows(float pz, float dz, int z0) {
+  if (GEOM == GEOM_CONE) {
+    for (int m = 0; m < z0; ++m) pz = pz + dz;
+    return pz;
+  }
+  return pz + (float)z0;  // k + 0.5 held exactly by the running sum
+}
+
+template <int GEOM, int ZR, typename S>
+__global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  if (b >= a.n_logical) return;
+  const uint32_t zr = b % a.nz;
+  const uint32_t tile = b / a.nz;
+  const int ix = (int)(tile % a.tiles_x) * 8 + (int)(threadIdx.x & 7);
+  const int iy = (int)(tile / a.tiles_x) * 8 + (int)(threadIdx.x >> 3);
+  if (ix >= a.W || iy >= a.W) return;
+  const int z0 = (int)zr * ZR;
+  const int zc = min(ZR, a.H - z0);
+  const size_t WW = (size_t)a.W * a.W;
+  float* o = a.out + (size_t)ix + (size_t)a.W * iy + WW * z0;
+  const uint32_t hw = (uint32_t)a.W / 2u - 1u;
+  const uint32_t dx = (uint32_t)ix - hw, dy = (uint32_t)iy - hw;
+  if (!(dx * dx + dy * dy < hw * hw)) {  // outside the circle of reconstruction
+    for (int m = 0; m < zc; ++m) o[(size_t)m * WW] = 0.f;
+    return;
+  }
+  const float x = (float)ix - ((float)hw + 0.5f);
+  const float y = (float)iy - ((float)hw + 0.5f);
+  const S* __restrict__ sino = (const S*)a.sino;
+  float acc[ZR];
+#pragma unroll
+  for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
+
+  for (int v = 0; v < a.nviews; ++v) {
+    const float2 scv = a.sc[v];
+    float U, py, pz, dz;
+    bp_view_geom<GEOM>(x, y, scv.x, scv.y, a.SR, a.v0, hw, U, py, pz, dz);
+    const float UU = U * U;
+    const float xb = py - 0.5f;
+    const float fx = floorf(xb);
+    const float ax = xb - fx;
+    const int i0 = (int)fminf(fmaxf(fx, -2.f), (float)a.W);
+    const S* col = sino + (size_t)v * a.PV + (i0 + HALO);
+    pz = advance_rows<GEOM>(pz, dz, z0);
+#pragma unroll
+    for (int m = 0; m < ZR; ++m) {
+      const float zb = pz - 0.5f;
+      const float fz = floorf(zb);
+      const float az = zb - fz;
+      const int j0 = (int)fminf(fmaxf(fz, -2.f), (float)a.H);
+      const S* row = col + (j0 + HALO) * a.PW;
+      const float2 q0 = Store<S>::ld2(row);
+      const float2 q1 = Store<S>::ld2(row + a.PW);
+      const float c0 = lerp(q0.x, q0.y, ax);
+      const float c1 = lerp(q1.x, q1.y, ax);
+      acc[m] = acc[m] + UU * lerp(c0, c1, az);
+      pz = pz + dz;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < ZR; ++m)
+    if (m < zc) o[(size_t)m * WW] = acc[m];
+}
+
+static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
+
+template <int ZR, typename S>
+static void bp_dispatch(const BPKArgs& k, int geom, hipStream_t st) {
+  const unsigned g = padded_grid(k.n_logical);
+  switch (geom) {
+    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S><<<g, 64, 0, st>>>(k); break;
+    default: bp_kernel<GEOM_CONE, ZR, S><<<g, 64, 0, st>>>(k); break;
+  }
+}
+
+template <typename S>
+static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, hipStream_t st) {
+  if (zr == 64)
+    bp_dispatch<64, S>(k, geom, st);
+  else
+    bp_dispatch<32, S>(k, geom, st);
+}
+
+hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
+  const int zr = a.zr == 64 ? 64 : 32;
+  BPKArgs k;
+  k.sino = a.sino;
+  k.out = a.out;
+  k.sc = a.sc;
+  k.W = a.W;
+  k.H = a.H;
+  k.nviews = a.nviews;
+  k.PW = a.W + 2 * HALO;
+  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO);
+  k.v0 = float(a.H - 1) / 2;
+  k.SR = a.sad;
+  k.tiles_x = (uint32_t)((a.W + 7) / 8);
+  k.nz = (uint32_t)((a.H + zr - 1) / zr);
+  k.n_logical = k.tiles_x * k.tiles_x * k.nz;
+  if (k.n_logical == 0) return hipSuccess;
+  if (a.st_dt == 0)
+    bp_dispatch_zr<float>(k, a.geom, zr, st);
+  else if (a.st_dt == 1)
+    bp_dispatch_zr<_Float16>(k, a.geom, zr, st);
+  else
+    bp_dispatch_zr<bf16_t>(k, a.geom, zr, st);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// OSC dual backprojection + multiplicative update (CUDAmex_oscIter.cu:272-358
+// and :113-136) for one subset: num/den sums stay in registers over the
+// subset's views and the voxel is updated in place, inside the circle only.
+struct OSCBPKArgs {
+  const float* pre_pad;
+  const float* post_pad;
+  float* mu_pad;
+  const float2* sc;
+  int W, H, PW, first, stride, count;
+  int64_t PV, MPX, MPXY;
+  float v0, SR;
+  uint32_t tiles_x, nz, n_logical;
+};
+
+constexpr int OSC_ZR = 32;
+
+template <int GEOM>
+__global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  if (b >= a.n_logical) return;
+  const uint32_t zr = b % a.nz;
+  const uint32_t tile = b / a.nz;
+  const int ix = (int)(tile % a.tiles_x) * 8 + (int)(threadIdx.x & 7);
+  const int iy = (int)(tile / a.tiles_x) * 8 + (int)(threadIdx.x >> 3);
+  if (ix >= a.W || iy >= a.W) return;
+  const uint32_t hw = (uint32_t)a.W / 2u - 1u;
+  const uint32_t dx = (uint32_t)ix - hw, dy = (uint32_t)iy - hw;
+  if (!(dx * dx + dy * dy < hw * hw)) return;
+  const int z0 = (int)zr * OSC_ZR;
+  const int zc = min(OSC_ZR, a.H - z0);
+  const float x = (float)ix - ((float)hw + 0.5f);
+  const float y = (float)iy - ((float)hw + 0.5f);
+  float accN[OSC_ZR], accD[OSC_ZR];
+#pragma unroll
+  for (int m = 0; m < OSC_ZR; ++m) {
+    accN[m] = 0.f;
+    accD[m] = 0.f;
+  }
+  for (int j = 0; j < a.count; ++j) {
+    const float2 scv = a.sc[a.first + j * a.stride];
+    const float sB = scv.x, cB = scv.y;
+    const float s = x * cB + y * sB;
+    const float t = -x * sB + y * cB;
+    float U, UU, py, pz;
+    if (GEOM == GEOM_PAR) {  // :318-324
+      U = 1.f;
+      UU = 1.f;
+      py = t * UU + (float)hw + 1.f;
+      pz = -a.v0 * UU + a.v0 + 0.5f;
+    } else if (GEOM == GEOM_FAN) {  // :325-330
+      U = 1.f;
+      UU = a.SR / (a.SR - s);
+      py = t * UU + (float)hw + 1.f;
+      pz = 0.5f;
+    } else {  // :331-336
+      U = a.SR / (a.SR - s);
+      UU = a.SR / (a.SR - s);
+      py = t * U + (float)hw + 1.f;
+      pz = -a.v0 * UU + a.v0 + 0.5f;
+    }
+    const float W2 = UU * UU;
+    const float xb = py - 0.5f;
+    const float fx = floorf(xb);
+    const float ax = xb - fx;
+    const int i0 = (int)fminf(fmaxf(fx, -2.f), (float)a.W);
+    const size_t coff = (size_t)j * a.PV + (i0 + HALO);
+    const float* colN = a.pre_pad + coff;
+    const float* colD = a.post_pad + coff;
+    if (GEOM == GEOM_CONE) {
+      for (int m = 0; m < z0; ++m) pz = pz + U;
+    } else {
+      pz = pz + (float)z0;
+    }
+#pragma unroll
+    for (int m = 0; m < OSC_ZR; ++m) {
+      const float zb = pz - 0.5f;
+      const float fz = floorf(zb);
+      const float az = zb - fz;
+      const int j0 = (int)fminf(fmaxf(fz, -2.f), (float)a.H);
+      const size_t r = (size_t)(j0 + HALO) * a.PW;
+      const float2 n0 = Store<float>::ld2(colN + r);
+      const float2 n1 = Store<float>::ld2(colN + r + a.PW);
+      const float2 d0 = Store<float>::ld2(colD + r);
+      const float2 d1 = Store<float>::ld2(colD + r + a.PW);
+      const float tn = lerp(lerp(n0.x, n0.y, ax), lerp(n1.x, n1.y, ax), az);
+      const float td = lerp(lerp(d0.x, d0.y, ax), lerp(d1.x, d1.y, ax), az);
+      accN[m] = accN[m] + W2 * tn;
+      accD[m] = accD[m] + W2 * td;
+      pz = pz + U;
+    }
+  }
+  float* mu = a.mu_pad + (size_t)(ix + HALO) + (size_t)a.MPX * (iy + HALO) +
+              (size_t)a.MPXY * (z0 + HALO);
+#pragma unroll
+  for (int m = 0; m < OSC_ZR; ++m) {
+    if (m < zc) {
+      const float fval = mu[(size_t)m * a.MPXY];
+      mu[(size_t)m * a.MPXY] = fmaxf(fval + (fval * accN[m] / accD[m]), 0.f);
+    }
+  }
+}
+
+hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st) {
+  OSCBPKArgs k;
+  k.pre_pad = a.pre_pad;
+  k.post_pad = a.post_pad;
+  k.mu_pad = a.mu_pad;
+  k.sc = a.sc;
+  k.W = a.W;
+  k.H = a.H;
+  k.PW = a.W + 2 * HALO;
+  k.first = a.first;
+  k.stride = a.stride;
+  k.count = a.count;
+  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO);
+  k.MPX = a.W + 2 * HALO;
+  k.MPXY = (int64_t)(a.W + 2 * HALO) * (a.W + 2 * HALO);
+  k.v0 = float(a.H - 1) / 2;
+  k.SR = a.sad;
+  k.tiles_x = (uint32_t)((a.W + 7) / 8);
+  k.nz = (uint32_t)((a.H + OSC_ZR - 1) / OSC_ZR);
+  k.n_logical = k.tiles_x * k.tiles_x * k.nz;
+  if (k.n_logical == 0) return hipSuccess;
+  const unsigned g = padded_grid(k.n_logical);
+  switch (a.geom) {
+    case GEOM_PAR: osc_bp_update_kernel<GEOM_PAR><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: osc_bp_update_kernel<GEOM_FAN><<<g, 64, 0, st>>>(k); break;
+    default: osc_bp_update_kernel<GEOM_CONE><<<g, 64, 0, st>>>(k); break;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// General-3D ray-driven backprojection (CUDAmex_general3D_BP.cu:61-188):
+// one thread per (ray, view), 0.5-voxel steps, two corners per step with
+// the reference's weights, scattered with float atomics.
+struct G3DBPKArgs {
+  const float* sino;
+  const float* pts;
+  int64_t pts_view_stride;
+  const float2* sc;
+  float* out;
+  int W, H, M, N;
+  float bx, by, bz;
+  uint32_t tiles_x, tiles_per_view, n_logical;
+};
+
+__device__ __forceinline__ int floor_index(float v) {
+  return (int)fminf(fmaxf(floorf(v), -2.f), 1073741824.f);
+}
+__device__ __forceinline__ int ceil_index(float v) {
+  return (int)fminf(fmaxf(ceilf(v), -2.f), 1073741824.f);
+}
+
+__global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  if (b >= a.n_logical) return;
+  const uint32_t view = b / a.tiles_per_view;
+  const uint32_t t = b - view * a.tiles_per_view;
+  const int ix = (int)(t % a.tiles_x) * 16 + (int)(threadIdx.x & 15);
+  const int iy = (int)(t / a.tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  if (ix >= a.W || iy >= a.H) return;
+  const float2 scv = a.sc[view];
+  const float sB = scv.x, cB = scv.y;
+  const size_t npts = (size_t)a.W * a.H;
+  const float* p = a.pts + (size_t)view * a.pts_view_stride + (size_t)iy * a.W + ix;
+  R3 st, en;
+  st.x = (p[0]) * cB - (p[npts]) * sB;
+  st.y = (p[0]) * sB + (p[npts]) * cB;
+  st.z = (p[2 * npts]);
+  en.x = (p[3 * npts]) * cB - (p[4 * npts]) * sB;
+  en.y = (p[3 * npts]) * sB + (p[4 * npts]) * cB;
+  en.z = (p[5 * npts]);
+  // slab clip, identical to the forward projector's (:84-143)
+  const float TH = parallel_threshold();
+  const float qnan = __builtin_nanf("");
+  float aX1, aX2, aY1, aY2, aZ1, aZ2;
+  if (fabsf(en.x - st.x) >= TH) {
+    aX1 = (a.bx - st.x) / (en.x - st.x);
+    aX2 = (-a.bx - st.x) / (en.x - st.x);
+  } else {
+    aX1 = qnan;
+    aX2 = qnan;
+  }
+  if (fabsf(en.y - st.y) >= TH) {
+    aY1 = (a.by - st.y) / (en.y - st.y);
+    aY2 = (-a.by - st.y) / (en.y - st.y);
+  } else {
+    aY1 = qnan;
+    aY2 = qnan;
+  }
+  if (fabsf(en.z - st.z) >= TH) {
+    aZ1 = (a.bz - st.z) / (en.z - st.z);
+    aZ2 = (-a.bz - st.z) / (en.z - st.z);
+  } else {
+    aZ1 = qnan;
+    aZ2 = qnan;
+  }
+  float aXmin = fminf(aX1, aX2), aXmax = fmaxf(aX1, aX2);
+  float aYmin = fminf(aY1, aY2), aYmax = fmaxf(aY1, aY2);
+  float aZmin = fminf(aZ1, aZ2), aZmax = fmaxf(aZ1, aZ2);
+  float tmp = fmaxf(aXmin, fmaxf(aYmin, aZmin));
+  const float aMin = fmaxf(0.f, tmp);
+  tmp = fminf(aXmax, fminf(aYmax, aZmax));
+  const float aMax = fminf(1.f, tmp);
+  R3 cur, e2;
+  cur.x = st.x + aMin * (en.x - st.x);
+  cur.y = st.y + aMin * (en.y - st.y);
+  cur.z = st.z + aMin * (en.z - st.z);
+  e2.x = st.x + aMax * (en.x - st.x);
+  e2.y = st.y + aMax * (en.y - st.y);
+  e2.z = st.z + aMax * (en.z - st.z);
+  const float distance = sqrtf((e2.x - cur.x) * (e2.x - cur.x) + (e2.y - cur.y) * (e2.y - cur.y) +
+                               (e2.z - cur.z) * (e2.z - cur.z));
+  const float stepsize = 0.5f;
+  const float sX = (e2.x - cur.x) / distance * stepsize;
+  const float sY = (e2.y - cur.y) / distance * stepsize;
+  const float sZ = (e2.z - cur.z) / distance * stepsize;
+  int n = 0;
+  if (distance > 0.f) {
+    const float q = ceilf(distance / stepsize);
+    n = q < 16777216.f ? (int)q : 16777216;
+  }
+  const float pval = a.sino[(size_t)view * npts + (size_t)iy * a.W + ix];
+  const unsigned M = (unsigned)a.M, N = (unsigned)a.N;
+  const size_t MM = (size_t)a.M * a.M;
+  for (int k = 0; k < n; ++k) {
+    const float fxv = cur.x - a.bx, fyv = cur.y - a.by, fzv = cur.z - a.bz;
+    const int iL = floor_index(fxv), iU = ceil_index(fxv);
+    const int jL = floor_index(fyv), jU = ceil_index(fyv);
+    const int kL = floor_index(fzv), kU = ceil_index(fzv);
+    if (!(iL < 0 || jL < 0 || kL < 0 || (unsigned)iU >= M || (unsigned)jU >= M ||
+          (unsigned)kU >= N)) {
+      const float iXX = fxv - (float)iL + 0.5f;
+      const float iYY = fyv - (float)jL + 0.5f;
+      const float iZ = fzv - (float)kL + 0.5f;
+      const float iZZ = 1 - iZ;
+      const size_t col = (size_t)jL * a.M + (size_t)iL;
+      atomicAdd(a.out + col + MM * kL, pval * stepsize * iXX * iYY * iZZ);
+      atomicAdd(a.out + col + MM * kU, pval * stepsize * iXX * iYY * iZ);
+    }
+    cur.x = cur.x + sX;
+    cur.y = cur.y + sY;
+    cur.z = cur.z + sZ;
+  }
+}
+
+hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st) {
+  G3DBPKArgs k;
+  k.sino = a.sino;
+  k.pts = a.pts;
+  k.pts_view_stride = a.pts_view_stride;
+  k.sc = a.sc;
+  k.out = a.out;
+  k.W = a.W;
+  k.H = a.H;
+  k.M = a.M;
+  k.N = a.N;
+  k.bx = (float)(-(double)(float)a.M / 2.0);
+  k.by = k.bx;
+  k.bz = (float)(-(double)(float)a.N / 2.0);
+  k.tiles_x = (uint32_t)((a.W + 15) / 16);
+  k.tiles_per_view = k.tiles_x * (uint32_t)((a.H + 15) / 16);
+  k.n_logical = k.tiles_per_view * (uint32_t)a.nviews;
+  if (k.n_logical == 0) return hipSuccess;
+  g3d_bp_kernel<<<padded_grid(k.n_logical), 256, 0, st>>>(k);
+  return hipGetLastError();
+}
+
+}  // namespace mexct

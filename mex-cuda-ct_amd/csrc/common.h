@@ -1,0 +1,110 @@
+// common.h -- device helpers shared by the gfx950 kernels.
+//
+// Every kernel is compiled with -ffp-contract=off: a float expression is
+// rounded once per operation in source order, so the sample positions,
+// weights and sums reproduce the reference kernels' single-precision
+// arithmetic operation by operation.  FMAs appear only where written
+// explicitly (the texture lerp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mexct {
+
+enum Geom { GEOM_PAR = 0, GEOM_FAN = 1, GEOM_CONE = 2, GEOM_G3D = 3 };
+enum Mode { ADDR_CLAMP = 0, ADDR_BORDER = 1 };
+
+// Every gathered operand (volume for FP, sinogram for BP) is stored with a
+// 2-texel halo on each side of each axis: edge copies for clamp
+// addressing, zeros for border addressing.  A corner index clamped to
+// [-2, N] then always reads the texels the reference's address mode
+// selects, and the x-neighbour pair (i, i+1) is one 8-byte load.
+constexpr int HALO = 2;
+
+// storage tags
+struct bf16_t {
+  uint16_t v;
+};
+typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef _Float16 h2a2 __attribute__((ext_vector_type(2), aligned(2)));
+typedef uint16_t u2a2 __attribute__((ext_vector_type(2), aligned(2)));
+
+template <typename S>
+struct Store;
+
+template <>
+struct Store<float> {
+  static constexpr int bytes = 4;
+  __device__ __forceinline__ static float2 ld2(const float* p) {
+    f2a4 v = *(const f2a4*)p;
+    return make_float2(v.x, v.y);
+  }
+  __device__ __forceinline__ static float ld(const float* p) { return *p; }
+  __device__ __forceinline__ static float from(float f) { return f; }
+};
+
+template <>
+struct Store<_Float16> {
+  static constexpr int bytes = 2;
+  __device__ __forceinline__ static float2 ld2(const _Float16* p) {
+    h2a2 v = *(const h2a2*)p;
+    return make_float2((float)v.x, (float)v.y);
+  }
+  __device__ __forceinline__ static float ld(const _Float16* p) { return (float)*p; }
+  __device__ __forceinline__ static _Float16 from(float f) { return (_Float16)f; }
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+template <>
+struct Store<bf16_t> {
+  static constexpr int bytes = 2;
+  __device__ __forceinline__ static float2 ld2(const bf16_t* p) {
+    u2a2 v = *(const u2a2*)p;
+    return make_float2(bf16_to_f32(v.x), bf16_to_f32(v.y));
+  }
+  __device__ __forceinline__ static float ld(const bf16_t* p) { return bf16_to_f32(p->v); }
+  __device__ __forceinline__ static bf16_t from(float f) {
+    bf16_t b;
+    b.v = f32_to_bf16(f);
+    return b;
+  }
+};
+
+// Load element i of an input array of dtype 0 f32, 1 f16, 2 bf16.
+__device__ __forceinline__ float load_any(const void* p, int dtype, size_t i) {
+  if (dtype == 0) return ((const float*)p)[i];
+  if (dtype == 1) return (float)((const _Float16*)p)[i];
+  return bf16_to_f32(((const uint16_t*)p)[i]);
+}
+
+// texture lerp (1-t)*a + t*b evaluated as fma(t, b - a, a)
+__device__ __forceinline__ float lerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// float threshold equivalent to the reference's `fabsf(d) >= 1e-5`
+// (a float compared with the double literal): the least float >= 1e-5.
+__device__ __forceinline__ float parallel_threshold() { return __uint_as_float(0x3727c5adu); }
+
+struct R3 {
+  float x, y, z;
+};
+
+// Flat block index -> logical tile index so that the 8 XCDs (which receive
+// workgroups round-robin) each walk a contiguous range of tiles: rays of one
+// view that run at the same time share one L2.  `nblk` is padded to a
+// multiple of 8 by the launcher; ids >= n_logical exit.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblk_padded) {
+  uint32_t per = nblk_padded >> 3;
+  return (b & 7u) * per + (b >> 3);
+}
+
+}  // namespace mexct

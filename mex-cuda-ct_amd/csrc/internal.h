@@ -1,0 +1,115 @@
+// internal.h -- host-side launch interface between the C ABI (api.hip) and
+// the kernel translation units.  Every launcher enqueues on `st` and
+// returns the hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mexct {
+
+// ---- storage conversion + halo padding (pad.hip)
+// 3D: in [nx,ny,nz] (dtype in_dt) -> out [nx+4, ny+4, nz+4] of storage st_dt.
+hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, int ny, int nz,
+                       int mode, hipStream_t st);
+// 2D stack: in [w,h,n] -> out [w+4, h+4, n].
+hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
+                       int mode, hipStream_t st);
+// 3D interior copy between a padded f32 volume and a plain one.
+hipError_t launch_unpad3(const float* padded, float* out, int nx, int ny, int nz, hipStream_t st);
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
+
+// ---- forward projection (fp.hip)
+struct FPLaunch {
+  const void* vol;  // padded storage volume [NX+4, NY+4, NZ+4]
+  int st_dt;        // storage dtype
+  float* out;       // [W, H, nProj]
+  const float2* sc; // device (sin, cos) per view
+  int geom;         // GEOM_PAR/FAN/CONE/G3D
+  int mode;         // ADDR_CLAMP / ADDR_BORDER
+  int W, H, nviews;
+  int NX, NY, NZ;
+  float sad;
+  float step;
+  const float* pts;          // g3d rays (device)
+  int64_t pts_view_stride;   // elements between per-view pts slices (0 = shared)
+};
+hipError_t launch_fp(const FPLaunch& a, hipStream_t st);
+
+// OSC forward projection + ratio terms over the views first, first+stride, ...
+struct OSCFPLaunch {
+  const float* mu_pad;  // padded f32 volume (border halo of zeros)
+  float* pre;           // [W,H,nProj] updated in place
+  float* post;          // [W,H,nProj] updated in place
+  float* pre_pad;       // [W+4,H+4,count] padded copies for the BP (zero halo)
+  float* post_pad;
+  const float2* sc;
+  int geom, W, H, nProj, first, stride, count;
+  float sad;
+};
+hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st);
+
+// ---- backprojection (bp.hip)
+struct BPLaunch {
+  const void* sino;  // padded storage sinogram [W+4, H+4, nProj] (edge halo)
+  int st_dt;
+  float* out;        // [W, W, H]
+  const float2* sc;
+  int geom, W, H, nviews;
+  float sad;
+  int zr;            // z-voxels per thread (32 or 64)
+};
+hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
+
+struct OSCBPLaunch {
+  const float* pre_pad;   // [W+4, H+4, count]
+  const float* post_pad;
+  float* mu_pad;          // padded f32 volume, interior updated in place
+  const float2* sc;       // all views
+  int geom, W, H, first, stride, count;
+  float sad;
+};
+hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st);
+
+// general-3D ray-driven backprojection (atomics)
+struct G3DBPLaunch {
+  const float* sino;  // [W,H,nProj]
+  const float* pts;
+  int64_t pts_view_stride;
+  const float2* sc;
+  float* out;         // [M,M,N], zeroed by the caller
+  int W, H, nviews, M, N;
+};
+hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st);
+
+// ---- TV (tv.hip)
+// Strided view of a float volume: element (i,j,k) at p[i + sy*j + sz*k].
+struct Vol3 {
+  float* p;
+  int nx, ny, nz;
+  int64_t sy, sz;
+};
+// `steps` TV gradient steps on x in place.  c is read from c_dev when it is
+// not null, else from `c`.  g ([nx,ny,nz]), partials (tv_partials_count
+// doubles) and a_dev (one float) are workspace.
+hipError_t launch_tvmin(Vol3 x, float* g, double* partials, float* a_dev, const float* c_dev,
+                        float c, int steps, hipStream_t st);
+size_t tv_partials_count(int nx, int ny, int nz);
+
+// ---- OSC-TV driver helpers (tv.hip)
+// *sum = sum over voxels of (old - new)^2 in FP64, a NaN difference counted
+// as 0 and old(0,0,0) read as 0 (OSC_TV_recon.m:292,312-314).
+hipError_t launch_diff_norm(Vol3 old_v, Vol3 new_v, double* partials, double* sum,
+                            hipStream_t st);
+// *c_dev = tv * (float)sqrt(*sum)  (OSC_TV_recon.m:314-315)
+hipError_t launch_tv_const(const double* sum, float tv, float* c_dev, hipStream_t st);
+// NaN and +-Inf -> 0 (OSC_TV_recon.m:316-317)
+hipError_t launch_clean(Vol3 x, hipStream_t st);
+
+// ---- FDK prefilter (fdk.hip)
+// out = ((float)(sum_u' x[u'] g[u-u'+W-1]) * dA[view]) / 2 per detector row,
+// x = sino * wgt (float) promoted to double; wgt may be null (no weight).
+hipError_t launch_prefilter(const float* sino, float* out, const float* wgt, const double* g,
+                            const float* dA, int W, int H, int nviews, hipStream_t st);
+
+}  // namespace mexct

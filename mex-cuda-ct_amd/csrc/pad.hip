@@ -1,0 +1,133 @@
+// pad.hip -- storage conversion and halo padding of gathered operands.
+//
+// The reference binds its volume/projection to a texture with clamp or
+// border addressing (CUDAmex_FP.cu:405-409, CUDAmex_BP.cu:270-273,
+// CUDAmex_general3D_FP.cu:217-221, CUDAmex_oscIter.cu:417-462).  Here the
+// operand is copied once per call into a [N+4] haloed layout whose halo
+// holds what those address modes return outside the array, converted to
+// the storage dtype (f32, f16 or bf16) that the gather kernels read.
+#include "common.h"
+#include "internal.h"
+
+namespace mexct {
+
+template <typename S>
+__global__ __launch_bounds__(256) void pad3_kernel(const void* __restrict__ in, int in_dt,
+                                                   S* __restrict__ out, int nx, int ny, int nz,
+                                                   int mode) {
+  const int PX = nx + 2 * HALO, PY = ny + 2 * HALO, PZ = nz + 2 * HALO;
+  const size_t total = (size_t)PX * PY * PZ;
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    int px = (int)(o % PX);
+    size_t r = o / PX;
+    int py = (int)(r % PY);
+    int pz = (int)(r / PY);
+    int i = px - HALO, j = py - HALO, k = pz - HALO;
+    float v;
+    if (mode == ADDR_CLAMP) {
+      i = clampi(i, 0, nx - 1);
+      j = clampi(j, 0, ny - 1);
+      k = clampi(k, 0, nz - 1);
+      v = load_any(in, in_dt, (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k));
+    } else if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) {
+      v = 0.f;
+    } else {
+      v = load_any(in, in_dt, (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k));
+    }
+    out[o] = Store<S>::from(v);
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, int in_dt,
+                                                   S* __restrict__ out, int w, int h, int n,
+                                                   int mode) {
+  const int PX = w + 2 * HALO, PY = h + 2 * HALO;
+  const size_t total = (size_t)PX * PY * n;
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    int px = (int)(o % PX);
+    size_t r = o / PX;
+    int py = (int)(r % PY);
+    int view = (int)(r / PY);
+    int i = px - HALO, j = py - HALO;
+    float v;
+    if (mode == ADDR_CLAMP) {
+      i = clampi(i, 0, w - 1);
+      j = clampi(j, 0, h - 1);
+      v = load_any(in, in_dt, (size_t)i + (size_t)w * j + (size_t)w * h * view);
+    } else if (i < 0 || i >= w || j < 0 || j >= h) {
+      v = 0.f;
+    } else {
+      v = load_any(in, in_dt, (size_t)i + (size_t)w * j + (size_t)w * h * view);
+    }
+    out[o] = Store<S>::from(v);
+  }
+}
+
+__global__ __launch_bounds__(256) void unpad3_kernel(const float* __restrict__ padded,
+                                                     float* __restrict__ out, int nx, int ny,
+                                                     int nz) {
+  const size_t total = (size_t)nx * ny * nz;
+  const size_t PX = nx + 2 * HALO, PY = ny + 2 * HALO;
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(o % nx);
+    size_t r = o / nx;
+    int j = (int)(r % ny);
+    int k = (int)(r / ny);
+    out[o] = padded[(size_t)(i + HALO) + PX * ((size_t)(j + HALO) + PY * (size_t)(k + HALO))];
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* p, float v, size_t n) {
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+       o += (size_t)gridDim.x * blockDim.x)
+    p[o] = v;
+}
+
+static unsigned grid_for(size_t total) {
+  size_t b = (total + 255) / 256;
+  if (b > 65536u) b = 65536u;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, int ny, int nz,
+                       int mode, hipStream_t st) {
+  size_t total = (size_t)(nx + 2 * HALO) * (ny + 2 * HALO) * (nz + 2 * HALO);
+  unsigned g = grid_for(total);
+  if (st_dt == 0)
+    pad3_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, nx, ny, nz, mode);
+  else if (st_dt == 1)
+    pad3_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, nx, ny, nz, mode);
+  else
+    pad3_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, nx, ny, nz, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
+                       int mode, hipStream_t st) {
+  size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO) * n;
+  unsigned g = grid_for(total);
+  if (st_dt == 0)
+    pad2_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode);
+  else if (st_dt == 1)
+    pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode);
+  else
+    pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpad3(const float* padded, float* out, int nx, int ny, int nz, hipStream_t st) {
+  unpad3_kernel<<<grid_for((size_t)nx * ny * nz), 256, 0, st>>>(padded, out, nx, ny, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st) {
+  fill_kernel<<<grid_for(n), 256, 0, st>>>(p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace mexct

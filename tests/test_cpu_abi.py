@@ -1,0 +1,104 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point include/mexct.h declares, and rejects bad arguments before touching
+the device (the reference's argument errors, CUDAmex_FP.cu:352-353,463)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mexct
+from mexct import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "mexct.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(mexct_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ["mexct_fp", "mexct_bp", "mexct_g3d_fp", "mexct_g3d_bp", "mexct_osc_iter",
+                 "mexct_tvmin", "mexct_prefilter", "mexct_osc_tv_recon"]:
+        assert must in names and must + "_dev" in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python signature table covers them all
+    assert set(_declared()) <= set(L.SIGNATURES)
+
+
+def test_version():
+    assert L.lib().mexct_version() >= 10000
+
+
+def _f32(shape):
+    return np.zeros(shape, np.float32, order="F")
+
+
+def test_invalid_geometry():
+    with pytest.raises(mexct.MexctError) as e:
+        mexct.CUDAmex_FP(_f32((8, 8, 4)), 100.0, np.zeros(2, np.float32), 2, 3)
+    assert e.value.code == L.EGEOM and "invalid geometry" in e.value.msg
+    with pytest.raises(mexct.MexctError) as e:
+        mexct.CUDAmex_BP(_f32((8, 4, 2)), 100.0, np.zeros(2, np.float32), -1)
+    assert e.value.code == L.EGEOM
+
+
+def test_nproj_exceeds_angles():
+    with pytest.raises(mexct.MexctError) as e:
+        mexct.CUDAmex_FP(_f32((8, 8, 4)), 100.0, np.zeros(2, np.float32), 3, 0)
+    assert e.value.code == L.EINVAL
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_BP(_f32((8, 4, 5)), 100.0, np.zeros(4, np.float32), 0)
+
+
+def test_non_single_inputs_rejected():
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_FP(np.zeros((8, 8, 4)), 100.0, np.zeros(2, np.float32), 2, 0)
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_FP(_f32((8, 8, 4)), 100.0, np.zeros(2), 2, 0)
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_FP(_f32((8, 6, 4)), 100.0, np.zeros(2, np.float32), 2, 0)
+
+
+def test_osc_bad_subsets_and_picks():
+    mu, pre, post = _f32((8, 8, 4)), _f32((8, 4, 6)), _f32((8, 4, 6))
+    ang = np.zeros(6, np.float32)
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_oscIter(mu, pre, post, 0, np.zeros(1, np.int32), ang, 100.0, 2)
+    with pytest.raises(mexct.MexctError):
+        mexct.CUDAmex_oscIter(mu, pre, post, 3, np.zeros(2, np.int32), ang, 100.0, 2)
+    with pytest.raises(mexct.MexctError):  # in-place args must be Fortran float32
+        mexct.CUDAmex_oscIter(np.zeros((8, 8, 4), np.float32), pre, post, 1,
+                              np.zeros(1, np.int32), ang, 100.0, 2)
+
+
+def test_null_pointer_rejected():
+    lib = L.lib()
+    rc = lib.mexct_fp(None, 8, 8, C.c_float(1.0), None, 0, 0, 0, None, None)
+    assert rc == L.EINVAL
+    assert b"null" in lib.mexct_last_error()
+
+
+def test_prefilter_needs_two_views():
+    with pytest.raises(mexct.MexctError):
+        mexct.preFilterSinogram(_f32((8, 4, 1)), {"type": "cone3d", "SAD": 100.0},
+                                np.zeros(1, np.float32))
+
+
+def test_subset_helpers_match_oracle():
+    import oracle as O
+
+    for (s0, s1, p, K) in [(64, 2, 0.2, 10), (10, 10, 1.0, 20), (32, 4, 0.5, 7)]:
+        for k in range(1, K + 1):
+            assert mexct.calc_num_subsets(s0, s1, p, K, k) == O.calc_num_subsets(s0, s1, p, K, k)
+    for n in range(1, 40):
+        for first in range(1, n + 1):
+            assert np.array_equal(mexct.calc_subset_order(n, first), O.calc_subset_order(n, first))

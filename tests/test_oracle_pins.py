@@ -1,0 +1,419 @@
+"""Pins of the CPU oracle: closed forms, invariants and independent
+computations that a plausible transcription mistake (dropped term, wrong
+sign or index, transposed operand) would fail.  The reference ships no
+tests or golden vectors, so these are the oracle's anchors.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def f32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+# ------------------------------------------------------------------ sampler
+def _numpy_trilinear(T, x, y, z, mode):
+    """Independent float64 trilinear fetch with the public texture rule."""
+    out = 0.0
+    xb, yb, zb = x - 0.5, y - 0.5, z - 0.5
+    i, j, k = int(np.floor(xb)), int(np.floor(yb)), int(np.floor(zb))
+    a, b, c = xb - i, yb - j, zb - k
+    for di, wi in ((0, 1 - a), (1, a)):
+        for dj, wj in ((0, 1 - b), (1, b)):
+            for dk, wk in ((0, 1 - c), (1, c)):
+                ii, jj, kk = i + di, j + dj, k + dk
+                if mode == O.CLAMP:
+                    ii = min(max(ii, 0), T.shape[0] - 1)
+                    jj = min(max(jj, 0), T.shape[1] - 1)
+                    kk = min(max(kk, 0), T.shape[2] - 1)
+                    v = T[ii, jj, kk]
+                elif 0 <= ii < T.shape[0] and 0 <= jj < T.shape[1] and 0 <= kk < T.shape[2]:
+                    v = T[ii, jj, kk]
+                else:
+                    v = 0.0
+                out += wi * wj * wk * float(v)
+    return out
+
+
+def test_tex3_texel_centres_exact():
+    T = f32(rng(1).uniform(size=(5, 4, 3)))
+    for mode in (O.CLAMP, O.BORDER):
+        for i in range(5):
+            for j in range(4):
+                for k in range(3):
+                    assert O.tex3(T, i + 0.5, j + 0.5, k + 0.5, mode) == T[i, j, k]
+
+
+def test_tex3_matches_independent_trilinear():
+    T = f32(rng(2).uniform(-1, 1, size=(6, 5, 4)))
+    r = rng(3)
+    for mode in (O.CLAMP, O.BORDER):
+        for _ in range(400):
+            x, y, z = r.uniform(-2, 8), r.uniform(-2, 7), r.uniform(-2, 6)
+            x, y, z = float(np.float32(x)), float(np.float32(y)), float(np.float32(z))
+            assert O.tex3(T, x, y, z, mode) == pytest.approx(_numpy_trilinear(T, x, y, z, mode),
+                                                              rel=1e-5, abs=1e-6)
+
+
+def test_tex3_address_modes_outside():
+    T = f32(rng(4).uniform(0.5, 1.0, size=(4, 4, 4)))
+    assert O.tex3(T, -7.0, 1.5, 2.5, O.CLAMP) == T[0, 1, 2]
+    assert O.tex3(T, 13.0, 1.5, 2.5, O.CLAMP) == T[3, 1, 2]
+    assert O.tex3(T, -7.0, 1.5, 2.5, O.BORDER) == 0.0
+    # half a texel outside: border blends with 0, clamp does not
+    assert O.tex3(T, 0.0, 1.5, 2.5, O.BORDER) == pytest.approx(0.5 * T[0, 1, 2], rel=1e-7)
+    assert O.tex3(T, 0.0, 1.5, 2.5, O.CLAMP) == T[0, 1, 2]
+
+
+def test_tex2_linear_function_reproduced():
+    a, b, c = 0.75, -1.25, 3.0
+    u, v = np.meshgrid(np.arange(9), np.arange(7), indexing="ij")
+    T = f32(a * u + b * v + c)
+    r = rng(5)
+    for _ in range(200):
+        x, y = float(np.float32(r.uniform(0.5, 8.5))), float(np.float32(r.uniform(0.5, 6.5)))
+        assert O.tex2(T, x, y) == pytest.approx(a * (x - 0.5) + b * (y - 0.5) + c, abs=2e-5)
+
+
+# ------------------------------------------------------------------ FP
+def test_fp_parallel_beta0_is_column_sum():
+    # beta = 0: rays run along -x through texel centres in y and z, 0.5 steps;
+    # with zero x-faces the ray sum is sum_x vol[x, u, v]
+    W, H = 24, 10
+    vol = np.zeros((W, W, H), np.float32, order="F")
+    vol[1:-1] = rng(6).uniform(size=(W - 2, W, H))
+    p = O.fp(vol, 1000.0, np.zeros(1, np.float32), 1, 0)[:, :, 0]
+    np.testing.assert_allclose(p, vol.astype(np.float64).sum(axis=0), rtol=2e-5)
+
+
+def test_fp_parallel_beta0_edge_term():
+    # non-zero faces add (T[W-1] - T[0]) / 4 when 2W samples are taken
+    W, H = 16, 4
+    vol = f32(rng(7).uniform(size=(W, W, H)))
+    p, n = O.fp(vol, 1000.0, np.zeros(1, np.float32), 1, 0, return_samples=True)
+    S = vol.astype(np.float64).sum(axis=0)
+    edge = (vol[-1].astype(np.float64) - vol[0]) / 4.0
+    assert n == 2 * W * W * H
+    np.testing.assert_allclose(p[:, :, 0], S + edge, rtol=1e-4)
+
+
+def test_fp_parallel_cylinder_chord():
+    W, H, R, mu = 64, 4, 20.0, 0.02
+    c = np.arange(W) - (W - 1) / 2.0
+    x, y = np.meshgrid(c, c, indexing="ij")
+    vol = f32(np.repeat(((x * x + y * y) <= R * R)[:, :, None] * mu, H, axis=2))
+    ang = np.array([0.0, 0.7, 1.9], np.float32)
+    p = O.fp(vol, 1000.0, ang, 3, 0)
+    chord = 2 * mu * np.sqrt(np.clip(R * R - c * c, 0, None))
+    for k in range(3):
+        assert np.abs(p[:, 1, k] - chord).max() < 1.5 * mu * 2
+        assert abs(p[:, 1, k].sum() - chord.sum()) < 0.01 * chord.sum()
+
+
+def test_fp_linearity_and_zero():
+    W, H = 20, 12
+    a, b = f32(rng(8).uniform(size=(W, W, H))), f32(rng(9).uniform(size=(W, W, H)))
+    ang = np.array([0.3, 1.1, 2.5], np.float32)
+    for geom in (0, 1, 2):
+        pa = O.fp(a, 200.0, ang, 3, geom, f64=True)
+        pb = O.fp(b, 200.0, ang, 3, geom, f64=True)
+        pab = O.fp(f32(2 * a.astype(np.float64) + 3 * b), 200.0, ang, 3, geom, f64=True)
+        np.testing.assert_allclose(pab, 2 * pa + 3 * pb, rtol=1e-5, atol=1e-6)
+        assert not O.fp(np.zeros((W, W, H), np.float32, order="F"), 200.0, ang, 3, geom).any()
+
+
+def test_fp_parallel_opposite_views_mirror():
+    # beta and beta + pi see the same line integrals with u mirrored; the
+    # sample offsets along the ray differ, so the content is kept smooth
+    W, H = 32, 6
+    c = np.arange(W) - (W - 1) / 2.0
+    x, y = np.meshgrid(c, c, indexing="ij")
+    blob = np.exp(-((x - 3) ** 2 + (y + 2) ** 2) / (2 * 4.0 ** 2))
+    vol = f32(np.repeat(blob[:, :, None], H, axis=2))
+    p = O.fp(vol, 1000.0, np.array([0.4, 0.4 + np.pi], np.float32), 2, 0, f64=True)
+    np.testing.assert_allclose(p[:, :, 1], p[::-1, :, 0], rtol=0, atol=2e-3 * p.max())
+
+
+def test_fp_f32_close_to_f64_twin():
+    W, H = 24, 16
+    vol = f32(rng(11).uniform(size=(W, W, H)))
+    ang = np.array([0.2, 1.0, 2.2, 4.0], np.float32)
+    for geom in (0, 1, 2):
+        a = O.fp(vol, 150.0, ang, 4, geom)
+        b = O.fp(vol, 150.0, ang, 4, geom, f64=True)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
+
+
+# ------------------------------------------------------------------ BP
+def _circle(W):
+    hw = W // 2 - 1
+    ix, iy = np.meshgrid(np.arange(W), np.arange(W), indexing="ij")
+    return (ix - hw) ** 2 + (iy - hw) ** 2 < hw * hw
+
+
+def test_bp_parallel_single_view_closed_form():
+    # beta = 0: voxel (ix,iy,iz) samples texel (u=iy, v=iz) exactly
+    W, H = 18, 7
+    P = f32(rng(12).normal(size=(W, H, 1)))
+    out = O.bp(P, 1000.0, np.zeros(1, np.float32), 0)
+    m = _circle(W)
+    expect = np.where(m[:, :, None], P[None, :, :, 0], 0.0)
+    np.testing.assert_array_equal(out, expect.astype(np.float32))
+
+
+def test_bp_constant_sinogram():
+    W, H, n, c, SR = 20, 9, 7, 0.75, 150.0
+    P = np.full((W, H, n), c, np.float32, order="F")
+    ang = (np.arange(n) * 0.7).astype(np.float32)
+    m = _circle(W)
+    out = O.bp(P, SR, ang, 0)
+    np.testing.assert_allclose(out[m], c * n, rtol=1e-6)
+    assert not out[~m].any()
+    # cone: every tap is c (clamp), so out = c * sum_views U^2
+    out = O.bp(P, SR, ang, 2)
+    hw = W // 2 - 1
+    ix, iy = np.nonzero(m)
+    x = ix - (hw + 0.5)
+    y = iy - (hw + 0.5)
+    s_, c_ = np.sin(ang.astype(np.float32)), np.cos(ang.astype(np.float32))
+    U = SR / (SR - (x[:, None] * c_[None] + y[:, None] * s_[None]))
+    np.testing.assert_allclose(out[ix, iy, 3], c * (U * U).sum(axis=1), rtol=2e-6)
+
+
+def test_bp_f32_close_to_f64_twin():
+    W, H, n = 22, 14, 6
+    P = f32(rng(13).normal(size=(W, H, n)))
+    ang = (np.arange(n) * 1.1).astype(np.float32)
+    for geom in (0, 1, 2):
+        a = O.bp(P, 120.0, ang, geom)
+        b = O.bp(P, 120.0, ang, geom, f64=True)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+
+
+def test_bp_cols_match_full():
+    W, H, n = 20, 10, 5
+    P = f32(rng(14).normal(size=(W, H, n)))
+    ang = (np.arange(n) * 0.9).astype(np.float32)
+    full = O.bp(P, 100.0, ang, 2)
+    ix = np.array([3, 10, 9, 15])
+    iy = np.array([9, 10, 2, 12])
+    np.testing.assert_array_equal(O.bp_cols(P, 100.0, ang, 2, ix, iy), full[ix, iy, :])
+    vol = f32(rng(15).uniform(size=(W, W, H)))
+    fp = O.fp(vol, 100.0, ang, n, 2)
+    r = O.fp_rays(vol, 100.0, ang, 2, ix, iy % H, np.arange(4) % n)
+    np.testing.assert_array_equal(r, fp[ix, iy % H, np.arange(4) % n])
+
+
+# ------------------------------------------------------------------ general 3D
+def test_g3d_fp_parallel_pts_equals_parallel_fp():
+    # general3D_projection_test.m:13-29 builds parallel rays as pts
+    import mexct.phantoms as ph
+
+    W, H = 24, 12
+    vol = np.zeros((W, W, H), np.float32, order="F")
+    vol[3:-3, 3:-3, 3:-3] = rng(16).uniform(size=(W - 6, W - 6, H - 6))
+    ang = (np.arange(4) * np.pi / 2).astype(np.float32)
+    a = O.g3d_fp(vol, ang, ph.parallel_pts(W, H), 4)
+    b = O.fp(vol, 1000.0, ang, 4, 0)
+    np.testing.assert_allclose(a, b, rtol=2e-3, atol=2e-4)
+
+
+def test_g3d_bp_axis_ray_weights():
+    # one ray along +z through voxel-centre (x,y): floor/ceil give iL = i,
+    # iU = i + 1, so interpXX = interpYY = 1 and each in-range sample adds
+    # p * 0.5 * (interpZZ + interpZ) = p * 0.5 to the column
+    M, N, p = 8, 10, 3.0
+    i, j = 3, 5
+    x = i + 0.5 - M / 2.0
+    y = j + 0.5 - M / 2.0
+    pts = np.zeros((1, 1, 6), np.float32, order="F")
+    pts[0, 0] = [x, y, -50.0, x, y, 50.0]
+    out = O.g3d_bp(np.full((1, 1, 1), p, np.float32, order="F"), np.zeros(1, np.float32), pts, M, N)
+    assert np.count_nonzero(out.sum(axis=2)) == 1
+    col = out[i, j].astype(np.float64)
+    # samples at z in [0, N) every 0.5, skipped when ceil(z) >= N: 2N-1 of them
+    assert col.sum() == pytest.approx(p * 0.5 * (2 * N - 1), rel=1e-6)
+
+
+# ------------------------------------------------------------------ OSC
+def _osc_setup(W=16, H=8, n=8, seed=17):
+    r = rng(seed)
+    mu = f32(r.uniform(0.0, 0.02, size=(W, W, H)))
+    pre = f32(np.full((W, H, n), 1e5))
+    return mu, pre, (np.arange(n) * 2 * np.pi / n).astype(np.float32)
+
+
+def test_osc_zero_volume_fixed_point_and_nonnegative():
+    W, H, n = 16, 8, 8
+    mu, pre, ang = _osc_setup(W, H, n)
+    post = f32(pre * 0.5)
+    for geom in (0, 1, 2):
+        z, _, _ = O.osc_iter(np.zeros_like(mu), pre, post, 2, np.array([1, 0], np.int32), ang,
+                             100.0, geom)
+        assert not z.any()
+        m2, _, _ = O.osc_iter(mu, pre, post, 2, np.array([1, 0], np.int32), ang, 100.0, geom)
+        assert (m2 >= 0).all()
+
+
+def test_osc_consistent_data_is_fixed_point():
+    # post = pre*exp(-l) with l the OSC projection of mu makes num = 0
+    W, H, n = 16, 8, 8
+    mu, pre, ang = _osc_setup(W, H, n)
+    for geom in (0, 1, 2):
+        _, pr1, _ = O.osc_iter(mu, pre, np.zeros_like(pre), 1, np.array([0], np.int32), ang, 100.0,
+                               geom)
+        post = f32(pr1)  # = pre*exp(-l) - 0, bit for bit
+        m2, pr2, _ = O.osc_iter(mu, pre, post, 1, np.array([0], np.int32), ang, 100.0, geom)
+        assert not pr2.any()
+        inside = _circle(W)
+        np.testing.assert_array_equal(m2[inside], mu[inside])
+
+
+def test_osc_outside_circle_untouched_and_views_once():
+    W, H, n = 16, 6, 9
+    mu, pre, ang = _osc_setup(W, H, n)
+    post = f32(pre * 0.3)
+    m2, pr2, po2 = O.osc_iter(mu, pre, post, 3, np.array([2, 0, 1], np.int32), ang, 100.0, 2)
+    out = ~_circle(W)
+    np.testing.assert_array_equal(m2[out], mu[out])
+    assert (pr2 != pre).all() and (po2 != post).all()  # every view rewritten once
+
+
+def test_osc_f32_close_to_f64_twin():
+    W, H, n = 16, 8, 8
+    mu, pre, ang = _osc_setup(W, H, n)
+    post = f32(pre * rng(18).uniform(0.3, 0.9, size=pre.shape))
+    a, _, _ = O.osc_iter(mu, pre, post, 2, np.array([0, 1], np.int32), ang, 100.0, 2)
+    b, _, _ = O.osc_iter(mu, pre, post, 2, np.array([0, 1], np.int32), ang, 100.0, 2, f64=True)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
+
+
+# ------------------------------------------------------------------ TV
+def test_tv_gradient_hand_computed_peak():
+    # a single unit peak along x: g = 2/(1+eps) at the peak, -1/(1+eps) beside it
+    x = np.zeros((7, 5, 5), np.float32, order="F")
+    x[3, :, :] = 1.0
+    g = O.tv_gradient(x)
+    eps = np.float32(1e-6)
+    one = np.float32(1.0) / (eps + np.float32(1.0))
+    assert g[3, 2, 2] == pytest.approx(float(2 * one), rel=1e-7)
+    assert g[2, 2, 2] == pytest.approx(float(-one), rel=1e-7)
+    assert g[4, 2, 2] == pytest.approx(float(-one), rel=1e-7)
+    assert g[1, 2, 2] == 0.0 and g[5, 2, 2] == 0.0
+    assert not g[0].any() and not g[:, 0].any() and not g[:, :, 0].any()  # border
+
+
+def test_tv_constant_volume_gives_nan():
+    x = np.full((6, 6, 6), 0.5, np.float32, order="F")
+    assert np.isnan(O.tvmin(x, 0.1, 1)).all()
+
+
+def test_tv_step_length_is_c():
+    x = f32(rng(19).uniform(size=(10, 9, 8)))
+    c = 0.05
+    y = O.tvmin(x, c, 1)
+    assert np.linalg.norm(y.astype(np.float64) - x) == pytest.approx(c, rel=1e-4)
+    y20 = O.tvmin(x, c, 20)
+    assert np.linalg.norm(y20.astype(np.float64) - x) <= 20 * c * (1 + 1e-4)
+
+
+# ------------------------------------------------------------------ FDK filter
+def _kak_slaney_numpy(order):
+    n = np.arange(order // 2 + 1)
+    h = np.zeros(order // 2 + 1)
+    h[0] = 0.25
+    h[1::2] = -1.0 / (np.pi * n[1::2]) ** 2
+    return np.concatenate([h, h[-2:0:-1]])
+
+
+@pytest.mark.parametrize("window", ["ramp", "hamming", "shepp-logan", "cosine", "hann"])
+@pytest.mark.parametrize("length", [20, 96, 384])
+def test_design_filter_vs_numpy_fft(window, length):
+    order = max(64, 1 << int(np.ceil(np.log2(2 * length))))
+    f = 2 * np.real(np.fft.fft(_kak_slaney_numpy(order)))[: order // 2 + 1]
+    w = 2 * np.pi * np.arange(f.size) / order
+    d = 1.0
+    win = {"ramp": np.ones_like(w),
+           "hamming": .54 + .46 * np.cos(w / d),
+           "cosine": np.cos(w / (2 * d)),
+           "hann": (1 + np.cos(w / d)) / 2}
+    if window == "shepp-logan":
+        sl = np.ones_like(w)
+        sl[1:] = np.sin(w[1:] / (2 * d)) / (w[1:] / (2 * d))
+        win[window] = sl
+    g = f.copy()
+    g[1:] *= win[window][1:]
+    full = np.concatenate([g, g[-2:0:-1]])
+    got = O.design_filter(window, length)
+    assert got.size == order
+    np.testing.assert_allclose(got, full, rtol=1e-10, atol=1e-13)
+
+
+def test_design_filter_dc_closed_form():
+    order = 1024
+    odd = np.arange(1, order // 2, 2, dtype=np.float64)
+    dc = 2 * (0.25 - 2 * np.sum(1.0 / (np.pi * odd) ** 2))
+    assert O.design_filter("ramp", 384)[0] == pytest.approx(dc, rel=1e-12)
+    # the ramp's DC tends to 0 as order grows (sum 1/n^2 over odd n = pi^2/8)
+    assert abs(dc) < 1e-3
+
+
+@pytest.mark.parametrize("geom", ["par3d", "fan3d", "cone3d"])
+def test_prefilter_f64_vs_numpy(geom):
+    W, H, n, SAD = 40, 5, 4, 300.0
+    sino = f32(rng(20).uniform(size=(W, H, n)))
+    ang = (np.arange(n) * 0.25 + 0.1).astype(np.float32)
+    got = O.prefilter(sino, geom, SAD, ang, "hamming", 1.0, f64=True)
+    filt = O.design_filter("hamming", W)
+    U = np.arange(1, W + 1)[:, None] - (W - 1) / 2.0
+    V = np.arange(1, H + 1)[None, :] - (H - 1) / 2.0
+    if geom == "cone3d":
+        wgt = (SAD / np.sqrt(SAD ** 2 + U ** 2 + V ** 2)).astype(np.float32)
+    elif geom == "fan3d":
+        wgt = np.broadcast_to(SAD / np.sqrt(SAD ** 2 + U ** 2), (W, H)).astype(np.float32)
+    else:
+        wgt = np.ones((W, H), np.float32)
+    for i in range(n):
+        x = np.zeros((filt.size, H))
+        x[:W] = (sino[:, :, i] * wgt).astype(np.float64)
+        y = np.real(np.fft.ifft(np.fft.fft(x, axis=0) * filt[:, None], axis=0))[:W]
+        dA = np.float32(ang[i] - ang[i - 1] if i > 0 else ang[1] - ang[0])
+        np.testing.assert_allclose(got[:, :, i], y * float(dA) / 2, rtol=1e-9, atol=1e-12)
+
+
+# ------------------------------------------------------------------ subsets
+def test_calc_num_subsets_closed_form():
+    s0, s1, p, K = 64, 2, 0.2, 10
+    for k in range(1, K + 1):
+        v = (s0 - s1) / (K - 1) ** p * (K - k) ** p + s1
+        expect = int(np.floor(v + 0.5))
+        assert O.calc_num_subsets(s0, s1, p, K, k) == expect
+    assert O.calc_num_subsets(64, 2, 0.2, 10, 1) == 64
+    assert O.calc_num_subsets(64, 2, 0.2, 10, 10) == 2
+
+
+def test_calc_subset_order_properties():
+    for n in range(1, 30):
+        for first in range(1, n + 1):
+            picks = O.calc_subset_order(n, first)
+            assert sorted(picks.tolist()) == list(range(n))
+            assert picks[0] == first - 1
+            remaining = [v for v in range(1, n + 1) if v != first]
+            cur = first
+            for nxt in picks[1:] + 1:
+                lo = [v for v in remaining if v < cur]
+                hi = [v for v in remaining if v > cur]
+                side = lo if len(lo) > len(hi) else hi
+                assert nxt == side[(len(side) + 1) // 2 - 1]
+                remaining.remove(nxt)
+                cur = nxt
+
+
+def test_subset_order_known_case():
+    # n = 10 from pick 3: the middle of the larger side each time
+    assert O.calc_subset_order(10, 3).tolist() == [2, 6, 3, 7, 1, 5, 8, 0, 4, 9]
