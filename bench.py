@@ -22,6 +22,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "mex-cuda-ct_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker (cpu_baseline, FP64 sample)
 
 HBM_PEAK_GBS = 8000.0
 
@@ -63,20 +64,23 @@ def main():
     sino = torch.empty((n, H, W), dtype=torch.float32, device=dev)
     filt = torch.empty_like(sino)
     rec = torch.empty((H, W, W), dtype=torch.float32, device=dev)
-    ctx = Context(0)
+    # the operators run on `stream`; the events are recorded on it too
+    stream = torch.cuda.Stream(device=dev)
+    ctx = Context(0, stream=stream, order_with_torch=False)
+    torch.cuda.synchronize()
 
     def run_step(storage, v, events=None):
         if events is not None:
-            events[0].record()
+            events[0].record(stream)
         ctx.fp(v, SAD, ang, n, 2, sino, storage=storage)
         if events is not None:
-            events[1].record()
+            events[1].record(stream)
         ctx.prefilter(sino, 2, SAD, ang, 3, 1.0, filt)
         if events is not None:
-            events[2].record()
+            events[2].record(stream)
         ctx.bp(filt, SAD, ang, 2, rec, storage=storage)
         if events is not None:
-            events[3].record()
+            events[3].record(stream)
 
     def timed(storage, v, steps, warmup):
         for _ in range(warmup):
@@ -119,6 +123,7 @@ def main():
         import oracle as O  # checker only: FP64 reference on a sample
 
         vol16 = vol.half()
+        torch.cuda.synchronize()
         log("[bench] fp16-storage")
         s16, f16_fp_ms, _, f16_bp_ms = timed(mexct.F16, vol16, args.steps, 1)
         rng = np.random.Generator(np.random.PCG64(7))
@@ -161,9 +166,13 @@ def main():
     if not args.no_cpu:
         import oracle as O
 
-        cores = os.cpu_count() or 1
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        cores = max(1, min(cores, 16))  # the host share of one GPU
         O.set_threads(cores)
-        kv = 2  # views of the FP sample
+        kv = 4  # views of the FP sample
         log(f"[bench] cpu baseline: oracle FP on {kv} views, BP on 1024 columns, {cores} threads")
         t0 = time.perf_counter()
         O.fp(vol_np, SAD, ang[:kv], kv, 2)

@@ -66,6 +66,37 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
+MEXDIR = os.path.join(HERE, "mex")
+GATEWAYS = ["CUDAmex_FP", "CUDAmex_BP", "CUDAmex_general3D_FP", "CUDAmex_general3D_BP",
+            "CUDAmex_oscIter", "CUDAmex_TVmin_3D"]
+
+
+def build_mex(force: bool = False) -> list[str]:
+    """Build the MEX gateways against the in-tree mex.h shim (MATLAB-free
+    test build): lib/libmexshim.so and lib/mex/<name>.so, each exporting
+    mexFunction and linked to libmexct.so."""
+    gxx = shutil.which("g++") or "g++"
+    shim_src = os.path.join(MEXDIR, "shim", "mex_shim.cpp")
+    shim = os.path.join(LIBDIR, "libmexshim.so")
+    inc = ["-I", os.path.join(MEXDIR, "shim"), "-I", MEXDIR, "-I", os.path.join(ROOT, "include")]
+    if force or not os.path.exists(shim) or os.path.getmtime(shim_src) > os.path.getmtime(shim):
+        subprocess.run([gxx, "-O2", "-std=c++17", "-fPIC", "-shared", *inc, shim_src, "-o", shim],
+                       check=True)
+    outdir = os.path.join(LIBDIR, "mex")
+    os.makedirs(outdir, exist_ok=True)
+    outs = []
+    for g in GATEWAYS:
+        src = os.path.join(MEXDIR, g + ".cpp")
+        out = os.path.join(outdir, g + ".so")
+        deps = [src, os.path.join(MEXDIR, "gateway.h"), os.path.join(ROOT, "include", "mexct.h")]
+        if force or not os.path.exists(out) or any(os.path.getmtime(d) > os.path.getmtime(out) for d in deps):
+            subprocess.run([gxx, "-O2", "-std=c++17", "-fPIC", "-shared", *inc, src, "-o", out,
+                            "-L", LIBDIR, "-lmexct", "-lmexshim", "-Wl,-rpath,$ORIGIN/.."],
+                           check=True)
+        outs.append(out)
+    return outs
+
+
 def build(force: bool = False, jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -79,6 +110,7 @@ def build(force: bool = False, jobs: int | None = None) -> str:
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
         os.replace(tmp, LIB)
+    build_mex(force)
     return LIB
 
 

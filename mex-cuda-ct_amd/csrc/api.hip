@@ -117,6 +117,7 @@ struct mexct_ctx {
   double f_cut = 0.0;
   int w_W = -1, w_H = -1, w_geom = -1;
   double w_sad = 0.0;
+  std::vector<float> dA_key;  // angle-step table last uploaded to WS_DA
 
   int get(int slot, size_t bytes, void** out) {
     Buf& b = ws[slot];
@@ -798,10 +799,16 @@ int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, i
       c->w_sad = sad;
     }
   }
-  RC(c->get(WS_DA, sizeof(float) * nProj, &dp));
+  // (angles(i) - angles(i-1)) in single, view 1 takes the next step (:80-84)
   std::vector<float> dA((size_t)nProj);
   for (int64_t i = 0; i < nProj; ++i) dA[i] = i > 0 ? angles[i] - angles[i - 1] : angles[1] - angles[0];
-  RC(c->upload(dp, dA.data(), dA.size() * sizeof(float)));
+  const bool have = c->ws[WS_DA].p && c->dA_key.size() == dA.size() &&
+                    memcmp(c->dA_key.data(), dA.data(), dA.size() * sizeof(float)) == 0;
+  RC(c->get(WS_DA, sizeof(float) * nProj, &dp));
+  if (!have) {
+    RC(c->upload(dp, dA.data(), dA.size() * sizeof(float)));
+    c->dA_key.swap(dA);
+  }
   HIPCHK(launch_prefilter(sino, out, (const float*)wp, (const double*)gp, (const float*)dp, (int)W,
                           (int)H, (int)nProj, c->stream));
   return MEXCT_OK;

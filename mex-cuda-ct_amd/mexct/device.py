@@ -32,22 +32,53 @@ def _dptr(t, name, want_f32=False):
     return C.c_void_p(t.data_ptr())
 
 
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
 def _host_f32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
 
 
 class Context:
-    """mexct_ctx on `device`; stream=None uses torch's current stream there."""
+    """mexct_ctx on `device` running on its own HIP stream.
 
-    def __init__(self, device: int = 0, stream=None):
+    stream: None creates a torch.cuda.Stream (default priority); a
+    torch.cuda.Stream or a raw hipStream_t handle (int) is used as given.
+    order_with_torch: each call first waits for torch's current stream and
+    makes torch's current stream wait for it afterwards, so tensors produced
+    or consumed by torch work are ordered with the operators."""
+
+    def __init__(self, device: int = 0, stream=None, order_with_torch: bool = True):
         import torch
 
         self.device = int(device)
         if stream is None:
-            stream = torch.cuda.current_stream(self.device).cuda_stream
+            stream = torch.cuda.Stream(device=self.device)
+        self.stream = stream if isinstance(stream, torch.cuda.Stream) else None
+        handle = stream.cuda_stream if isinstance(stream, torch.cuda.Stream) else int(stream)
+        if handle == 0:
+            raise MexctError(EINVAL, "pass a non-null stream (the null stream cannot be shared)")
+        self.order_with_torch = order_with_torch and self.stream is not None
         h = C.c_void_p()
-        check(lib().mexct_ctx_create(C.byref(h), self.device, C.c_void_p(int(stream))))
+        check(lib().mexct_ctx_create(C.byref(h), self.device, C.c_void_p(handle)))
         self._h = h
+
+    def _before(self):
+        if self.order_with_torch:
+            import torch
+            self.stream.wait_stream(torch.cuda.current_stream(self.device))
+
+    def _after(self):
+        if self.order_with_torch:
+            import torch
+            torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def _call(self, fn, *args):
+        self._before()
+        rc = fn(*args)
+        self._after()
+        check(rc)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -73,9 +104,9 @@ class Context:
         ang = _host_f32(angles)
         if W2 != W or ang.size < nproj or tuple(out.shape) != (nproj, H, W):
             raise MexctError(EINVAL, "fp: shape mismatch")
-        check(lib().mexct_fp_dev(self._h, _dptr(vol, "vol"), _dtype_code(vol), W, H, float(sad),
-                                 ang.ctypes.data_as(C.c_void_p), int(nproj), int(geom),
-                                 _dptr(out, "out", True), opts(storage)))
+        self._call(lib().mexct_fp_dev, self._h, _dptr(vol, "vol"), _dtype_code(vol), W, H,
+                   float(sad), _p(ang), int(nproj), int(geom), _dptr(out, "out", True),
+                   opts(storage))
         return out
 
     def bp(self, sino, sad, angles, geom, out, storage=F32, zrange=0):
@@ -84,9 +115,9 @@ class Context:
         ang = _host_f32(angles)
         if ang.size < n or tuple(out.shape) != (H, W, W):
             raise MexctError(EINVAL, "bp: shape mismatch")
-        check(lib().mexct_bp_dev(self._h, _dptr(sino, "sino"), _dtype_code(sino), W, H, n,
-                                 float(sad), ang.ctypes.data_as(C.c_void_p), int(geom),
-                                 _dptr(out, "out", True), opts(storage, 0, zrange)))
+        self._call(lib().mexct_bp_dev, self._h, _dptr(sino, "sino"), _dtype_code(sino), W, H, n,
+                   float(sad), _p(ang), int(geom), _dptr(out, "out", True),
+                   opts(storage, 0, zrange))
         return out
 
     def g3d_fp(self, vol, angles, pts, nproj, out, storage=F32):
@@ -95,18 +126,17 @@ class Context:
         ppv = 1 if pts.dim() == 4 else 0
         H, W = pts.shape[-2], pts.shape[-1]
         ang = _host_f32(angles)
-        check(lib().mexct_g3d_fp_dev(self._h, _dptr(vol, "vol"), _dtype_code(vol), side, nsl,
-                                     ang.ctypes.data_as(C.c_void_p), _dptr(pts, "pts", True), W,
-                                     H, int(nproj), ppv, _dptr(out, "out", True), opts(storage)))
+        self._call(lib().mexct_g3d_fp_dev, self._h, _dptr(vol, "vol"), _dtype_code(vol), side, nsl,
+                   _p(ang), _dptr(pts, "pts", True), W, H, int(nproj), ppv,
+                   _dptr(out, "out", True), opts(storage))
         return out
 
     def g3d_bp(self, sino, angles, pts, M, N, out):
         n, H, W = sino.shape
         ppv = 1 if pts.dim() == 4 else 0
         ang = _host_f32(angles)
-        check(lib().mexct_g3d_bp_dev(self._h, _dptr(sino, "sino", True), W, H, n,
-                                     ang.ctypes.data_as(C.c_void_p), _dptr(pts, "pts", True), ppv,
-                                     int(M), int(N), _dptr(out, "out", True), None))
+        self._call(lib().mexct_g3d_bp_dev, self._h, _dptr(sino, "sino", True), W, H, n, _p(ang),
+                   _dptr(pts, "pts", True), ppv, int(M), int(N), _dptr(out, "out", True), None)
         return out
 
     def osc_iter(self, mu, pre, post, num_subsets, picks, angles, sad, geom):
@@ -114,23 +144,22 @@ class Context:
         n = pre.shape[0]
         pk = np.ascontiguousarray(np.asarray(picks, dtype=np.int32).ravel())
         ang = _host_f32(angles)
-        check(lib().mexct_osc_iter_dev(self._h, _dptr(mu, "mu", True), W, H,
-                                       _dptr(pre, "pre", True), _dptr(post, "post", True), n,
-                                       int(num_subsets), pk.ctypes.data_as(C.c_void_p),
-                                       ang.ctypes.data_as(C.c_void_p), float(sad), int(geom), None))
+        self._call(lib().mexct_osc_iter_dev, self._h, _dptr(mu, "mu", True), W, H,
+                   _dptr(pre, "pre", True), _dptr(post, "post", True), n, int(num_subsets),
+                   _p(pk), _p(ang), float(sad), int(geom), None)
 
     def tvmin(self, vol, c, steps=20):
         Nz, Ny, Nx = vol.shape
-        check(lib().mexct_tvmin_dev(self._h, _dptr(vol, "vol", True), Nx, Ny, Nz, float(c),
-                                    opts(F32, int(steps))))
+        self._call(lib().mexct_tvmin_dev, self._h, _dptr(vol, "vol", True), Nx, Ny, Nz, float(c),
+                   opts(F32, int(steps)))
         return vol
 
     def prefilter(self, sino, geom, sad, angles, window, cutoff, out):
         n, H, W = sino.shape
         ang = _host_f32(angles)
-        check(lib().mexct_prefilter_dev(self._h, _dptr(sino, "sino", True), W, H, n, int(geom),
-                                        float(sad), ang.ctypes.data_as(C.c_void_p), int(window),
-                                        float(cutoff), _dptr(out, "out", True)))
+        self._call(lib().mexct_prefilter_dev, self._h, _dptr(sino, "sino", True), W, H, n,
+                   int(geom), float(sad), _p(ang), int(window), float(cutoff),
+                   _dptr(out, "out", True))
         return out
 
     def osc_tv_recon(self, pre, post, angles, sad, geom, n_iter, tv, s_init, s_final, p,
@@ -138,10 +167,8 @@ class Context:
         n, H, W = pre.shape
         ang = _host_f32(angles)
         fp_ = np.ascontiguousarray(np.asarray(first_picks, dtype=np.int32).ravel())
-        check(lib().mexct_osc_tv_recon_dev(self._h, _dptr(pre, "pre", True),
-                                           _dptr(post, "post", True), W, H, n,
-                                           ang.ctypes.data_as(C.c_void_p), float(sad), int(geom),
-                                           int(n_iter), float(tv), int(s_init), int(s_final),
-                                           float(p), fp_.ctypes.data_as(C.c_void_p),
-                                           _dptr(out, "out", True), opts(F32, int(tv_steps))))
+        self._call(lib().mexct_osc_tv_recon_dev, self._h, _dptr(pre, "pre", True),
+                   _dptr(post, "post", True), W, H, n, _p(ang), float(sad), int(geom), int(n_iter),
+                   float(tv), int(s_init), int(s_final), float(p), _p(fp_),
+                   _dptr(out, "out", True), opts(F32, int(tv_steps)))
         return out

@@ -201,6 +201,12 @@ def test_osc_tv_recon(gpu):
     _check(got, ref, "OSC-TV", 1e-4, 1e-5)
 
 
+# relative RMSE bound vs the FP64 oracle: FP16 storage (11-bit significand)
+# must stay <= 1e-3; BF16 (8-bit significand) rounds a constant 0.02 region
+# by ~1e-3 by itself, so its bound is 4e-3.
+STORAGE_BOUND = {mexct.F16: 1e-3, mexct.BF16: 4e-3}
+
+
 @pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
 def test_storage_modes(gpu, storage):
     from mexct import phantoms
@@ -211,9 +217,9 @@ def test_storage_modes(gpu, storage):
     fp64 = O.fp(vol, 1000.0, ang, n, 2, f64=True)
     e = np.linalg.norm(fp - fp64) / np.linalg.norm(fp64)
     print(f"FP storage={storage} relRMSE vs FP64 = {e:.3e}")
-    assert e <= 1e-3
+    assert e <= STORAGE_BOUND[storage]
     bp = mexct.CUDAmex_BP(np.asfortranarray(fp64.astype(np.float32)), 1000.0, ang, 2, storage=storage)
     bp64 = O.bp(np.asfortranarray(fp64.astype(np.float32)), 1000.0, ang, 2, f64=True)
     e = np.linalg.norm(bp - bp64) / np.linalg.norm(bp64)
     print(f"BP storage={storage} relRMSE vs FP64 = {e:.3e}")
-    assert e <= 1e-3
+    assert e <= STORAGE_BOUND[storage]
