@@ -79,6 +79,7 @@ enum Slot {
   WS_PREW,
   WS_POSTW,
   WS_PLAIN,
+  WS_MUZ,      // z-fastest copy of the OSC volume
   WS_COUNT
 };
 
@@ -380,9 +381,28 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   if (nProj == 0) return MEXCT_OK;
   DevGuard g(c->device);
   void* pad;
-  RC(c->get(WS_VOL, (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st), &pad));
-  HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
+  const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st);
+  RC(c->get(WS_VOL, pbytes, &pad));
   const float2* sc;
+  if (pbytes < ((size_t)1 << 31)) {
+    // z-fastest haloed volume, one wave per 64 detector rows of a column
+    HIPCHK(launch_pad3z(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
+                        c->stream));
+    RC(c->sincos_table(angles, nProj, &sc));
+    FPZLaunch z;
+    z.vol = pad;
+    z.st_dt = st;
+    z.out = out;
+    z.sc = sc;
+    z.geom = geom;
+    z.W = (int)W;
+    z.H = (int)H;
+    z.nviews = nProj;
+    z.sad = sad;
+    HIPCHK(launch_fpz(z, c->stream));
+    return MEXCT_OK;
+  }
+  HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
   RC(c->sincos_table(angles, nProj, &sc));
   FPLaunch a;
   a.vol = pad;
@@ -615,12 +635,19 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   // halos stay zero: the kernels only ever write the interior
   HIPCHK(launch_fill((float*)pp, 0.f, pv * std::max(cmax, 1), c->stream));
   HIPCHK(launch_fill((float*)qp, 0.f, pv * std::max(cmax, 1), c->stream));
+  void* mz;
+  const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
+  RC(c->get(WS_MUZ, (size_t)MPXY * (H + 4) * 4, &mz));
+  const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
   for (int s = 0; s < numSubsets; ++s) {
     const int sel = picks[s];
     const int count = (sel >= 0 && sel < nProj) ? (nProj - 1 - sel) / numSubsets + 1 : 0;
     if (count > 0) {
+      // the subset projects the volume as left by the previous update
+      HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, MEXCT_F32, W, W, H, ADDR_BORDER,
+                          c->stream));
       OSCFPLaunch f;
-      f.mu_pad = mu_pad;
+      f.mu_z = (const float*)mz;
       f.pre = pre;
       f.post = post;
       f.pre_pad = (float*)pp;
@@ -658,6 +685,9 @@ int osc_check(int64_t W, int64_t H, int64_t nProj, int32_t numSubsets, int32_t g
   if (W < 2 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
   if (numSubsets < 1) return fail(MEXCT_EINVAL, "numSubsets must be >= 1 (got %d)", numSubsets);
   RC(check_padded(W, W, H, "volume"));
+  if ((double)(W + 4) * (W + 4) * (H + 4) * 4.0 >= 2147483648.0)
+    return fail(MEXCT_EINVAL, "volume too large for the OSC state (%lld x %lld x %lld)",
+                (long long)W, (long long)W, (long long)H);
   return MEXCT_OK;
 }
 

@@ -12,6 +12,10 @@ namespace mexct {
 // 3D: in [nx,ny,nz] (dtype in_dt) -> out [nx+4, ny+4, nz+4] of storage st_dt.
 hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, int ny, int nz,
                        int mode, hipStream_t st);
+// z-fastest haloed copy: element (x,y,z) of the x-fastest input at
+// in[x + in_sy*y + in_sz*z] -> out[(x+2)*(ny+4)*(nz+4) + (y+2)*(nz+4) + z+2].
+hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st);
 // 2D stack: in [w,h,n] -> out [w+4, h+4, n].
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
                        int mode, hipStream_t st);
@@ -36,9 +40,20 @@ struct FPLaunch {
 };
 hipError_t launch_fp(const FPLaunch& a, hipStream_t st);
 
+// standard-geometry FP on a z-fastest haloed volume (launch_pad3z layout)
+struct FPZLaunch {
+  const void* vol;
+  int st_dt;
+  float* out;
+  const float2* sc;
+  int geom, W, H, nviews;
+  float sad;
+};
+hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
+
 // OSC forward projection + ratio terms over the views first, first+stride, ...
 struct OSCFPLaunch {
-  const float* mu_pad;  // padded f32 volume (border halo of zeros)
+  const float* mu_z;    // z-fastest f32 volume with a zero (border) halo
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
   float* pre_pad;       // [W+4,H+4,count] padded copies for the BP (zero halo)

@@ -66,6 +66,47 @@ __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, 
   }
 }
 
+// z-fastest haloed copy for the standard-geometry projectors:
+// out[(x+2)*PY*PZ + (y+2)*PZ + (z+2)], PY = ny+4, PZ = nz+4, from an x-fastest
+// input element (x,y,z) at in[x + in_sy*y + in_sz*z].  A 32x32 (z,x) tile is
+// staged in LDS so both the read (along x) and the write (along z) coalesce.
+template <typename S>
+__global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in, int in_dt,
+                                                    int64_t in_sy, int64_t in_sz,
+                                                    S* __restrict__ out, int nx, int ny, int nz,
+                                                    int mode) {
+  __shared__ float tile[32][33];
+  const int PX = nx + 2 * HALO, PY = ny + 2 * HALO, PZ = nz + 2 * HALO;
+  const int pz0 = blockIdx.x * 32, px0 = blockIdx.y * 32, py = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  int j = py - HALO;
+  bool jin = (j >= 0 && j < ny);
+  if (mode == ADDR_CLAMP) {
+    j = clampi(j, 0, ny - 1);
+    jin = true;
+  }
+  for (int r = 0; r < 4; ++r) {
+    const int lz = ty + 8 * r;
+    int i = px0 + tx - HALO, k = pz0 + lz - HALO;
+    bool ok = jin && (i >= 0 && i < nx && k >= 0 && k < nz);
+    if (mode == ADDR_CLAMP) {
+      i = clampi(i, 0, nx - 1);
+      k = clampi(k, 0, nz - 1);
+      ok = true;
+    }
+    float v = 0.f;
+    if (ok && px0 + tx < PX && pz0 + lz < PZ)
+      v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
+    tile[lz][tx] = v;
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int lx = ty + 8 * r;
+    const int px = px0 + lx, pz = pz0 + tx;
+    if (px < PX && pz < PZ) out[((size_t)px * PY + py) * PZ + pz] = Store<S>::from(tile[tx][lx]);
+  }
+}
+
 __global__ __launch_bounds__(256) void unpad3_kernel(const float* __restrict__ padded,
                                                      float* __restrict__ out, int nx, int ny,
                                                      int nz) {
@@ -117,6 +158,21 @@ hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, i
     pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode);
   else
     pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st) {
+  const dim3 g((unsigned)((nz + 2 * HALO + 31) / 32), (unsigned)((nx + 2 * HALO + 31) / 32),
+               (unsigned)(ny + 2 * HALO));
+  if (st_dt == 0)
+    pad3z_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode);
+  else if (st_dt == 1)
+    pad3z_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (_Float16*)out, nx, ny, nz,
+                                              mode);
+  else
+    pad3z_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (bf16_t*)out, nx, ny, nz,
+                                            mode);
   return hipGetLastError();
 }
 
