@@ -458,6 +458,9 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
   RC(check_padded(W, H, nProj, "sinogram"));
+  if ((double)(W + 4) * (H + 4) * nProj * store_bytes(st) >= 2147483648.0)
+    return fail(MEXCT_EINVAL, "sinogram too large for one call (%lld x %lld x %lld)",
+                (long long)W, (long long)H, (long long)nProj);
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
   DevGuard g(c->device);
   void* pad;

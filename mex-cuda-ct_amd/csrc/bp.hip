@@ -20,8 +20,32 @@
 
 namespace mexct {
 
+template <typename S>
+__device__ __forceinline__ float2 bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float2 bld2<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                              uint32_t soff) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  return make_float2(v.x, v.y);
+}
+template <>
+__device__ __forceinline__ float2 bld2<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff) {
+  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+  h2v v = __builtin_bit_cast(h2v, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  return make_float2((float)v.x, (float)v.y);
+}
+template <>
+__device__ __forceinline__ float2 bld2<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                               uint32_t soff) {
+  const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+
 struct BPKArgs {
   const void* sino;
+  uint32_t sino_bytes;  // buffer range (the launcher guarantees < 2^31)
   float* out;
   const float2* sc;
   int W, H, nviews, PW;
@@ -64,7 +88,9 @@ __device__ __forceinline__ float advance_rows(float pz, float dz, int z0) {
   return pz + (float)z0;  // k + 0.5 held exactly by the running sum
 }
 
-template <int GEOM, int ZR, typename S>
+// SMALL: a view's haloed slice is below 2^24 bytes, so lane offsets are
+// exact in float and cost one FMA.
+template <int GEOM, int ZR, typename S, bool SMALL>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
@@ -85,7 +111,10 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   }
   const float x = (float)ix - ((float)hw + 0.5f);
   const float y = (float)iy - ((float)hw + 0.5f);
-  const S* __restrict__ sino = (const S*)a.sino;
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.sino, 0, (int)a.sino_bytes, 0x00020000);
+  const uint32_t es = (uint32_t)sizeof(S);
+  const int row_bytes = a.PW * (int)es;  // one detector row, also the (j+1) soffset
   float acc[ZR];
 #pragma unroll
   for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
@@ -98,22 +127,41 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
     const float xb = py - 0.5f;
     const float fx = floorf(xb);
     const float ax = xb - fx;
-    const int i0 = (int)fminf(fmaxf(fx, -2.f), (float)a.W);
-    const S* col = sino + (size_t)v * a.PV + (i0 + HALO);
+    const float fi0 = __builtin_amdgcn_fmed3f(fx, -2.f, (float)a.W);
+    // view v's slice starts at a wave-uniform scalar offset; the lane's
+    // byte offset of (u = i0, v = j0) is (i0+2)*es + (j0+2)*row_bytes, an
+    // integer below 2^24 and therefore exact in float
+    const uint32_t soff0 = (uint32_t)v * (uint32_t)a.PV * es;
+    const uint32_t soff1 = soff0 + (uint32_t)row_bytes;
+    const float colf = (fi0 + (float)HALO) * (float)es + (float)(HALO * row_bytes);
+    const float rbf = (float)row_bytes;
     pz = advance_rows<GEOM>(pz, dz, z0);
+    // groups of 8 z-voxels: offsets and loads of the whole group are issued
+    // before its interpolation, so 16 loads per lane are in flight
+    constexpr int G = 8;
 #pragma unroll
-    for (int m = 0; m < ZR; ++m) {
-      const float zb = pz - 0.5f;
-      const float fz = floorf(zb);
-      const float az = zb - fz;
-      const int j0 = (int)fminf(fmaxf(fz, -2.f), (float)a.H);
-      const S* row = col + (j0 + HALO) * a.PW;
-      const float2 q0 = Store<S>::ld2(row);
-      const float2 q1 = Store<S>::ld2(row + a.PW);
-      const float c0 = lerp(q0.x, q0.y, ax);
-      const float c1 = lerp(q1.x, q1.y, ax);
-      acc[m] = acc[m] + UU * lerp(c0, c1, az);
-      pz = pz + dz;
+    for (int m0 = 0; m0 < ZR; m0 += G) {
+      float az[G];
+      float2 q0[G], q1[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const float zb = pz - 0.5f;
+        const float fz = floorf(zb);
+        az[g] = zb - fz;
+        const float fj0 = __builtin_amdgcn_fmed3f(fz, -2.f, (float)a.H);
+        const uint32_t voff =
+            SMALL ? (uint32_t)__builtin_fmaf(fj0, rbf, colf)
+                  : (uint32_t)((int)fj0 + HALO) * (uint32_t)row_bytes + (uint32_t)colf;
+        q0[g] = bld2<S>(r, voff, soff0);
+        q1[g] = bld2<S>(r, voff, soff1);
+        pz = pz + dz;
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const float c0 = lerp(q0[g].x, q0[g].y, ax);
+        const float c1 = lerp(q1[g].x, q1[g].y, ax);
+        acc[m0 + g] = acc[m0 + g] + UU * lerp(c0, c1, az[g]);
+      }
     }
   }
 #pragma unroll
@@ -123,14 +171,22 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 
 static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
 
-template <int ZR, typename S>
-static void bp_dispatch(const BPKArgs& k, int geom, hipStream_t st) {
+template <int ZR, typename S, bool SMALL>
+static void bp_dispatch_g(const BPKArgs& k, int geom, hipStream_t st) {
   const unsigned g = padded_grid(k.n_logical);
   switch (geom) {
-    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S><<<g, 64, 0, st>>>(k); break;
-    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S><<<g, 64, 0, st>>>(k); break;
-    default: bp_kernel<GEOM_CONE, ZR, S><<<g, 64, 0, st>>>(k); break;
+    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
+    default: bp_kernel<GEOM_CONE, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
   }
+}
+
+template <int ZR, typename S>
+static void bp_dispatch(const BPKArgs& k, int geom, hipStream_t st) {
+  if ((uint64_t)k.PV * sizeof(S) < (1u << 24))
+    bp_dispatch_g<ZR, S, true>(k, geom, st);
+  else
+    bp_dispatch_g<ZR, S, false>(k, geom, st);
 }
 
 template <typename S>
@@ -145,6 +201,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   const int zr = a.zr == 64 ? 64 : 32;
   BPKArgs k;
   k.sino = a.sino;
+  k.sino_bytes = (uint32_t)((size_t)(a.W + 2 * HALO) * (a.H + 2 * HALO) * a.nviews *
+                            (a.st_dt == 0 ? 4u : 2u));
   k.out = a.out;
   k.sc = a.sc;
   k.W = a.W;

@@ -62,7 +62,49 @@ __device__ __forceinline__ float2 buf_ld2<bf16_t>(__amdgpu_buffer_rsrc_t r, uint
   return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
 }
 
-// One ray through the z-fastest volume.
+// Corner pairs and weights of one sample.
+struct Tap8 {
+  float2 p00, p01, p10, p11;
+  float ax, ay, az;
+};
+
+template <typename S>
+__device__ __forceinline__ Tap8 fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, const R3& cur,
+                                        float fnx, float fny, float fnz, int base) {
+  Tap8 t;
+  const float xb = (cur.x - a.bx) - 0.5f;
+  const float yb = (cur.y - a.by) - 0.5f;
+  const float zb = (cur.z - a.bz) - 0.5f;
+  const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+  t.ax = xb - fx;
+  t.ay = yb - fy;
+  t.az = zb - fz;
+  const int i = (int)__builtin_amdgcn_fmed3f(fx, -2.f, fnx);
+  const int j = (int)__builtin_amdgcn_fmed3f(fy, -2.f, fny);
+  const int kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
+  // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
+  const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
+  const uint32_t voff = e * (uint32_t)sizeof(S);
+  t.p00 = buf_ld2<S>(r, voff, 0);         // (i,   j  ): z k, k+1
+  t.p01 = buf_ld2<S>(r, voff, a.off_j);   // (i,   j+1)
+  t.p10 = buf_ld2<S>(r, voff, a.off_i);   // (i+1, j  )
+  t.p11 = buf_ld2<S>(r, voff, a.off_ij);  // (i+1, j+1)
+  return t;
+}
+
+__device__ __forceinline__ float interp_z(const Tap8& t) {
+  const float c00 = lerp(t.p00.x, t.p10.x, t.ax);
+  const float c10 = lerp(t.p01.x, t.p11.x, t.ax);
+  const float c01 = lerp(t.p00.y, t.p10.y, t.ax);
+  const float c11 = lerp(t.p01.y, t.p11.y, t.ax);
+  const float c0 = lerp(c00, c10, t.ay);
+  const float c1 = lerp(c01, c11, t.ay);
+  return lerp(c0, c1, t.az);
+}
+
+// One ray through the z-fastest volume, software-pipelined: the loads of
+// sample k+1 are issued before sample k is interpolated.  A fetch past the
+// last sample reads a clamped, in-range corner and is discarded.
 template <typename S, int STEP2>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
@@ -70,33 +112,24 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
   const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
   const int base = 2 * a.PY * a.PZ + 2 * a.PZ + 2;
   float dval = 0.f;
-  for (int k = 0; k < n; ++k) {
-    const float xb = (cur.x - a.bx) - 0.5f;
-    const float yb = (cur.y - a.by) - 0.5f;
-    const float zb = (cur.z - a.bz) - 0.5f;
-    const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
-    const float ax = xb - fx, ay = yb - fy, az = zb - fz;
-    const int i = (int)__builtin_amdgcn_fmed3f(fx, -2.f, fnx);
-    const int j = (int)__builtin_amdgcn_fmed3f(fy, -2.f, fny);
-    const int kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
-    // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
-    const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
-    const uint32_t voff = e * (uint32_t)sizeof(S);
-    const float2 p00 = buf_ld2<S>(r, voff, 0);         // (i,   j  ): z k, k+1
-    const float2 p01 = buf_ld2<S>(r, voff, a.off_j);   // (i,   j+1)
-    const float2 p10 = buf_ld2<S>(r, voff, a.off_i);   // (i+1, j  )
-    const float2 p11 = buf_ld2<S>(r, voff, a.off_ij);  // (i+1, j+1)
-    const float c00 = lerp(p00.x, p10.x, ax);
-    const float c10 = lerp(p01.x, p11.x, ax);
-    const float c01 = lerp(p00.y, p10.y, ax);
-    const float c11 = lerp(p01.y, p11.y, ax);
-    const float c0 = lerp(c00, c10, ay);
-    const float c1 = lerp(c01, c11, ay);
-    dval = dval + lerp(c0, c1, az) * step;
+  if (n <= 0) return dval;
+  // two register sets in ping-pong: while one sample is interpolated the
+  // next sample's four loads are in flight
+  Tap8 t0 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
+  int k = 0;
+  for (; k + 1 < n; k += 2) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
+    const Tap8 t1 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
+    dval = dval + interp_z(t0) * step;
+    cur.x = cur.x + inc.x;
+    cur.y = cur.y + inc.y;
+    cur.z = cur.z + inc.z;
+    t0 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
+    dval = dval + interp_z(t1) * step;
   }
+  if (k < n) dval = dval + interp_z(t0) * step;
   return dval;
 }
 
