@@ -109,6 +109,43 @@ def main():
     filt32 = filt.clone()
     rec32 = rec.clone()
 
+    # Measured gather roofline: the probe issues the same pair-load pattern
+    # with ideal coherence (4 loads/sample for FP, 2 per update for BP).
+    import oracle as O  # checker: sample-count estimate for the FP gather bytes
+
+    def probe_ms(nloads, storage, reps=3):
+        ctx.probe_gather(storage, W, H, n, nloads)  # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ns = ctx.probe_gather(storage, W, H, n, nloads)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, ns
+
+    views8 = np.linspace(0, n - 1, 8).astype(int)
+    S_est = float(np.mean([O.fp(vol_np, SAD, ang[v:v + 1], 1, 2, return_samples=True)[1]
+                           for v in views8])) * n
+    hw = W // 2 - 1
+    gx, gy = np.meshgrid(np.arange(W), np.arange(W), indexing="ij")
+    U_bp = float(((gx - hw) ** 2 + (gy - hw) ** 2 < hw * hw).sum()) * H * n
+    pf_ms, pf_n = probe_ms(4, mexct.F32)
+    pb_ms, pb_n = probe_ms(2, mexct.F32)
+    fp_gather = S_est * 8 * 4 / (fp_ms * 1e-3) / 1e9
+    bp_gather = U_bp * 4 * 4 / (bp_ms * 1e-3) / 1e9
+    pf_gbs = pf_n * 8 * 4 / (pf_ms * 1e-3) / 1e9
+    pb_gbs = pb_n * 4 * 4 / (pb_ms * 1e-3) / 1e9
+    gather = {
+        "fp": {"achieved_GBs": round(fp_gather, 1), "measured_peak_GBs": round(pf_gbs, 1),
+               "frac": round(fp_gather / pf_gbs, 4), "samples": S_est,
+               "samples_note": "ray-march samples, mean of 8 views x views (oracle count)"},
+        "bp": {"achieved_GBs": round(bp_gather, 1), "measured_peak_GBs": round(pb_gbs, 1),
+               "frac": round(bp_gather / pb_gbs, 4), "updates_in_circle": U_bp},
+        "unit": "GB/s of gathered texels (8 taps x 4 B per FP sample, 4 x 4 B per BP update)",
+    }
+    log(f"[bench] gather roofline: FP {fp_gather:.0f}/{pf_gbs:.0f} GB/s ({fp_gather / pf_gbs:.2f}), "
+        f"BP {bp_gather:.0f}/{pb_gbs:.0f} GB/s ({bp_gather / pb_gbs:.2f})")
+
     # FP roofline: compulsory HBM bytes of a view-by-view projector, the
     # volume read once per view plus the projections written once.
     fp_bytes = n * (float(W) * W * H * 4) + n * float(W) * H * 4
@@ -120,7 +157,6 @@ def main():
 
     f16 = None
     if not args.no_f16:
-        import oracle as O  # checker only: FP64 reference on a sample
 
         vol16 = vol.half()
         torch.cuda.synchronize()
@@ -213,6 +249,7 @@ def main():
         "fp_ms": round(fp_ms, 3), "filter_ms": round(fl_ms, 3), "bp_ms": round(bp_ms, 3),
         "fp_gups": round(fp_gups, 1), "bp_gups": round(bp_gups, 1),
         "roofline": roofline,
+        "gather_roofline": gather,
         "fp16_storage": f16,
         "cpu_baseline": cpu,
     }

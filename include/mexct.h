@@ -173,6 +173,18 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
                            int32_t sFinal, double p, const int32_t* first_picks, float* mu_out,
                            const mexct_opts* opts);
 
+/* ------------------------------------------------------------------ measurement
+ * No reference counterpart: the gather-throughput microbenchmark behind the
+ * projectors' "measured gather roofline".  Launches, on the context's stream,
+ * a kernel that issues the forward projector's load pattern (nloads = 4:
+ * four z-pair loads per sample) or the backprojector's (nloads = 2) with
+ * ideal coherence over a z-fastest haloed [W,W,H] volume of the given
+ * storage dtype held in the context's workspace (its contents are whatever
+ * the last projection left there).  One sample per lane per step, W steps,
+ * W*H*nviews lanes; *samples receives the total sample count. */
+int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H, int32_t nviews,
+                           int32_t nloads, int64_t* samples);
+
 /* OSC_TV_recon.m:323-349 helpers (host only). */
 int mexct_calc_num_subsets(int32_t sInit, int32_t sFinal, double p, int32_t K, int32_t k);
 int mexct_calc_subset_order(int32_t n, int32_t first_pick, int32_t* picks);

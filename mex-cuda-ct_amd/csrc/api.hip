@@ -868,6 +868,27 @@ int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int3
   return MEXCT_OK;
 }
 
+// ------------------------------------------------------------------ measurement
+int mexct_probe_gather_dev(mexct_ctx* c, int32_t storage, int64_t W, int64_t H, int32_t nviews,
+                           int32_t nloads, int64_t* samples) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_dtype(storage, "storage"));
+  if (W < 2 || H < 1 || nviews < 1 || (nloads != 2 && nloads != 4))
+    return fail(MEXCT_EINVAL, "bad probe arguments");
+  const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(storage);
+  if (pbytes >= ((size_t)1 << 31)) return fail(MEXCT_EINVAL, "probe volume too large");
+  DevGuard g(c->device);
+  void *vol, *sink;
+  RC(c->get(WS_VOL, pbytes, &vol));
+  RC(c->get(WS_SCALARS, 4096, &sink));
+  const int bx = (int)((W + 3) / 4);
+  const int by = (int)((H + 63) / 64) * nviews;
+  HIPCHK(launch_gather_probe(vol, storage, (int)W, (int)W, (int)H, bx, by, (int)W, nloads,
+                             (float*)sink, c->stream));
+  if (samples) *samples = (int64_t)bx * 4 * (int64_t)by * 64 * W;
+  return MEXCT_OK;
+}
+
 // ------------------------------------------------------------------ OSC-TV driver
 int mexct_calc_num_subsets(int32_t sInit, int32_t sFinal, double p, int32_t K, int32_t k) {
   // round((sInit - sFinal)/(K-1)^p * (K-k)^p + sFinal)  (OSC_TV_recon.m:323-325)

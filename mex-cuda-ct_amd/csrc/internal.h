@@ -127,4 +127,8 @@ hipError_t launch_clean(Vol3 x, hipStream_t st);
 hipError_t launch_prefilter(const float* sino, float* out, const float* wgt, const double* g,
                             const float* dA, int W, int H, int nviews, hipStream_t st);
 
+// ---- gather microbenchmark (probe.hip)
+hipError_t launch_gather_probe(const void* vol_z, int st_dt, int NX, int NY, int NZ, int nblocks_x,
+                               int nblocks_y, int nsteps, int nloads, float* sink, hipStream_t st);
+
 }  // namespace mexct

@@ -58,6 +58,7 @@ SIGNATURES = {
                            _d, _vp, _vp, _op],
     "mexct_osc_tv_recon_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _f, _i32, _i32, _f, _i32,
                                _i32, _d, _vp, _vp, _op],
+    "mexct_probe_gather_dev": [_vp, _i32, _i64, _i64, _i32, _i32, C.POINTER(_i64)],
     "mexct_calc_num_subsets": [_i32, _i32, _d, _i32, _i32],
     "mexct_calc_subset_order": [_i32, _i32, _vp],
 }

@@ -97,6 +97,14 @@ class Context:
     def workspace_bytes(self) -> int:
         return int(lib().mexct_ctx_workspace_bytes(self._h))
 
+    def probe_gather(self, storage, W, H, nviews, nloads=4):
+        """Gather microbenchmark launch (see mexct_probe_gather_dev); returns
+        the sample count of the launch."""
+        n = C.c_int64()
+        self._call(lib().mexct_probe_gather_dev, self._h, int(storage), int(W), int(H),
+                   int(nviews), int(nloads), C.byref(n))
+        return n.value
+
     # ------------------------------------------------------------ operators
     def fp(self, vol, sad, angles, nproj, geom, out, storage=F32):
         """vol (H, W, W) any of f32/f16/bf16 -> out (nProj, H, W) f32."""
