@@ -185,6 +185,59 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
 int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H, int32_t nviews,
                            int32_t nloads, int64_t* samples);
 
+/* ------------------------------------------------------------------ request server
+ * No reference counterpart (each MEX call owns the GPU while it runs).  A
+ * continuous-batching server for many callers sharing one GPU:
+ * `slots` workers, each with its own stream and workspace, take queued
+ * requests in arrival order as long as the sum of their device footprints
+ * stays within hbm_budget bytes; a worker that takes a request also takes up
+ * to max_group-1 queued requests with the same operator, geometry, sizes,
+ * angles, SAD and storage, and runs them back to back.  Requests run the
+ * same operators as the entry points above, so a result does not depend on
+ * what else is in flight.  Arrays are host memory in the MATLAB layout; the
+ * caller keeps input/output alive and unchanged until mexct_server_wait
+ * returns for the ticket.
+ *   FP:    input vol [W,W,H]          -> output sino [W,H,nProj]
+ *   BP:    input sino [W,H,nProj]     -> output vol [W,W,H]
+ *   FDK:   input sino, prefilter(window, cutoff) then BP -> output vol
+ *   OSCTV: input pre, input2 post     -> output mu [W,W,H] (mexct_osc_tv_recon) */
+#define MEXCT_OP_FP 0
+#define MEXCT_OP_BP 1
+#define MEXCT_OP_FDK 2
+#define MEXCT_OP_OSCTV 3
+
+typedef struct mexct_request {
+  int32_t op, geom, storage, window; /* window as mexct_prefilter (FDK only) */
+  int64_t W, H, nProj;               /* angles holds nProj entries          */
+  float sad;
+  float osc_tv;                      /* OSCTV: TV, nIter, sInit, sFinal, p  */
+  double cutoff;
+  double osc_p;
+  int32_t osc_iters, osc_s_init, osc_s_final, tv_steps; /* tv_steps as mexct_opts */
+  const float* angles;
+  const float* input;
+  const float* input2;
+  float* output;
+  const int32_t* first_picks; /* OSCTV: osc_iters entries */
+} mexct_request;
+
+typedef struct mexct_server mexct_server;
+
+int mexct_server_create(mexct_server** sv, int device, int64_t hbm_budget, int32_t slots,
+                        int32_t max_group);
+/* Queues a request; MEXCT_ENOMEM if its footprint alone exceeds the budget. */
+int mexct_server_submit(mexct_server* sv, const mexct_request* req, uint64_t* ticket);
+/* Blocks until the request has finished; *status is its return code (on
+ * error mexct_last_error() holds its message), latency_s = submit -> output
+ * in host memory, queue_s = submit -> start.  Any pointer may be NULL. */
+int mexct_server_wait(mexct_server* sv, uint64_t ticket, int32_t* status, double* latency_s,
+                      double* queue_s);
+/* stats[0..n): done, groups, requests run in groups, queued, bytes reserved,
+ * peak bytes reserved, workspace releases. */
+int mexct_server_stats(mexct_server* sv, int64_t* stats, int32_t n);
+/* Finishes running requests, drops queued ones, frees everything. */
+int mexct_server_destroy(mexct_server* sv);
+
 /* OSC_TV_recon.m:323-349 helpers (host only). */
 int mexct_calc_num_subsets(int32_t sInit, int32_t sFinal, double p, int32_t K, int32_t k);
 int mexct_calc_subset_order(int32_t n, int32_t first_pick, int32_t* picks);

@@ -330,6 +330,11 @@ void conv_taps(const std::vector<double>& full, int W, std::vector<double>& g) {
 
 }  // namespace
 
+namespace mexct {
+hipStream_t ctx_stream(const mexct_ctx* c) { return c->stream; }
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace mexct
+
 // =============================================================== C ABI
 extern "C" {
 

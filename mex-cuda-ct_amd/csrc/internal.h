@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct mexct_ctx;
+
 namespace mexct {
 
 // ---- storage conversion + halo padding (pad.hip)
@@ -130,5 +132,10 @@ hipError_t launch_prefilter(const float* sino, float* out, const float* wgt, con
 // ---- gather microbenchmark (probe.hip)
 hipError_t launch_gather_probe(const void* vol_z, int st_dt, int NX, int NY, int NZ, int nblocks_x,
                                int nblocks_y, int nsteps, int nloads, float* sink, hipStream_t st);
+
+// ---- context internals shared with the request server (api.hip)
+hipStream_t ctx_stream(const mexct_ctx* c);
+// sets the calling thread's mexct_last_error() text and returns code
+int set_error(int code, const char* msg);
 
 }  // namespace mexct

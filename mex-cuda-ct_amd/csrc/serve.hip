@@ -1,0 +1,419 @@
+// serve.hip -- continuous-batching request server on one GPU.
+//
+// There is no counterpart in the reference, where each MEX call owns the
+// device for its whole duration.  A server owns `slots` workers.  Each
+// worker has its own context (stream at default priority, grow-only
+// workspace) and its own host thread.  Requests (FP, BP, FDK = prefilter +
+// BP, OSC-TV) queue in arrival order.  A free worker admits the oldest
+// request whose device footprint fits the HBM budget, so requests that
+// arrive while others run start as soon as a worker and memory are free,
+// and their kernels overlap on the device.  A worker that admits a request
+// also takes up to max_group-1 queued requests with the same key (operator,
+// geometry, sizes, angles, SAD, window, storage) and runs them back to back
+// with the geometry tables and buffers already resident.
+//
+// Memory accounting: every worker reserves max(bytes it holds, footprint
+// of the request it runs).  The sum of reservations never exceeds the
+// budget.  When the oldest request cannot be admitted and releasing the
+// memory held by idle workers would let it fit, the idle workers release
+// it.  Later requests may overtake a blocked oldest request at most
+// kMaxOvertake times; after that nothing is admitted before it.
+//
+// The operators are deterministic, so a request's result is the same
+// whether it runs alone or among other requests.
+#include "../../include/mexct.h"
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+constexpr int kMaxOvertake = 8;
+constexpr size_t kSlack = (size_t)24 << 20;  // tables, filter taps, 2 MiB rounding
+
+double now_s() {
+  return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+struct Req {
+  uint64_t id = 0;
+  mexct_request r{};
+  std::vector<float> angles;
+  std::vector<int32_t> picks;
+  int64_t footprint = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  int status = 1;  // 1 queued, 2 running, 0 done, < 0 error code
+  int overtaken = 0;
+  std::string err;
+};
+
+// Upper estimate of the device bytes a request needs on a worker: the
+// worker's staging arrays plus the C ABI's workspace for that operator
+// (haloed copies at 4 bytes per element, the widest storage).
+int64_t footprint_of(const mexct_request& r) {
+  const size_t W = (size_t)r.W, H = (size_t)r.H, P = (size_t)r.nProj;
+  const size_t v = W * W * H * 4, s = W * H * P * 4;
+  const size_t vh = (W + 4) * (W + 4) * (H + 4) * 4;
+  const size_t sh = (W + 4) * (H + 4) * P * 4;
+  size_t b = 0;
+  switch (r.op) {
+    case MEXCT_OP_FP: b = v + s + vh; break;
+    case MEXCT_OP_BP: b = s + v + sh; break;
+    case MEXCT_OP_FDK: b = 2 * s + v + sh; break;
+    default: b = 8 * s + 2 * sh + 3 * vh + 3 * v; break;  // OSC-TV state
+  }
+  return (int64_t)(b + kSlack);
+}
+
+bool same_key(const Req& a, const Req& b) {
+  const mexct_request &x = a.r, &y = b.r;
+  return x.op == y.op && x.geom == y.geom && x.storage == y.storage && x.W == y.W &&
+         x.H == y.H && x.nProj == y.nProj && x.sad == y.sad && x.window == y.window &&
+         x.cutoff == y.cutoff && x.tv_steps == y.tv_steps && a.angles == b.angles;
+}
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct mexct_server {
+  struct Worker {
+    mexct_ctx* ctx = nullptr;
+    std::thread th;
+    Buf a, b, o, x;  // staged input, second input, output, filtered sinogram
+    int64_t reserved = 0;  // this worker's share of in_use
+    bool running = false;
+  };
+  int device = 0;
+  int64_t budget = 0;
+  int64_t in_use = 0;
+  int max_group = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::vector<std::unique_ptr<Worker>> workers;
+  int64_t n_done = 0, n_groups = 0, n_grouped = 0, n_releases = 0, peak_in_use = 0;
+};
+
+namespace {
+
+using Worker = mexct_server::Worker;
+
+int grow(Buf& b, size_t bytes) {
+  if (b.cap >= bytes) return MEXCT_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return mexct::set_error(MEXCT_ENOMEM, "server: device allocation failed");
+  }
+  b.cap = bytes;
+  return MEXCT_OK;
+}
+
+int64_t held_bytes(const Worker& w) {
+  return (int64_t)(w.a.cap + w.b.cap + w.o.cap + w.x.cap) + mexct_ctx_workspace_bytes(w.ctx);
+}
+
+// Frees a worker's staging arrays and context workspace.  The worker's own
+// thread calls this while the worker is idle; its stream is drained first.
+int release(mexct_server* sv, Worker& w) {
+  (void)mexct_ctx_sync(w.ctx);
+  for (Buf* b : {&w.a, &w.b, &w.o, &w.x}) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->cap = 0;
+  }
+  mexct_ctx_destroy(w.ctx);
+  w.ctx = nullptr;
+  return mexct_ctx_create(&w.ctx, sv->device, nullptr);
+}
+
+int copy_in(Worker& w, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, mexct::ctx_stream(w.ctx));
+  return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
+}
+
+int copy_out(Worker& w, void* dst, const void* src, size_t bytes) {
+  hipStream_t st = mexct::ctx_stream(w.ctx);
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
+}
+
+#define TRY(x)          \
+  do {                  \
+    int r_ = (x);       \
+    if (r_) return r_;  \
+  } while (0)
+
+// Runs one request on a worker: host -> device, operator(s) on the worker's
+// stream, device -> host.  Returns when the output is in host memory.
+int run_one(Worker& w, Req& q) {
+  const mexct_request& r = q.r;
+  const mexct_opts o{r.storage, r.tv_steps, 0, 0};
+  const size_t vb = (size_t)r.W * r.W * r.H * 4, sb = (size_t)r.W * r.H * r.nProj * 4;
+  const float* ang = q.angles.data();
+  switch (r.op) {
+    case MEXCT_OP_FP:
+      TRY(grow(w.a, vb));
+      TRY(grow(w.o, sb));
+      TRY(copy_in(w, w.a.p, r.input, vb));
+      TRY(mexct_fp_dev(w.ctx, w.a.p, MEXCT_F32, r.W, r.H, r.sad, ang, (int32_t)r.nProj, r.geom,
+                       (float*)w.o.p, &o));
+      return copy_out(w, r.output, w.o.p, sb);
+    case MEXCT_OP_BP:
+    case MEXCT_OP_FDK: {
+      TRY(grow(w.a, sb));
+      TRY(grow(w.o, vb));
+      TRY(copy_in(w, w.a.p, r.input, sb));
+      const void* bp_in = w.a.p;
+      if (r.op == MEXCT_OP_FDK) {
+        TRY(grow(w.x, sb));
+        TRY(mexct_prefilter_dev(w.ctx, (const float*)w.a.p, r.W, r.H, r.nProj, r.geom, r.sad,
+                                ang, r.window, r.cutoff, (float*)w.x.p));
+        bp_in = w.x.p;
+      }
+      TRY(mexct_bp_dev(w.ctx, bp_in, MEXCT_F32, r.W, r.H, r.nProj, r.sad, ang, r.geom,
+                       (float*)w.o.p, &o));
+      return copy_out(w, r.output, w.o.p, vb);
+    }
+    case MEXCT_OP_OSCTV:
+      TRY(grow(w.a, sb));
+      TRY(grow(w.b, sb));
+      TRY(grow(w.o, vb));
+      TRY(copy_in(w, w.a.p, r.input, sb));
+      TRY(copy_in(w, w.b.p, r.input2, sb));
+      TRY(mexct_osc_tv_recon_dev(w.ctx, (const float*)w.a.p, (const float*)w.b.p, r.W, r.H,
+                                 r.nProj, ang, r.sad, r.geom, r.osc_iters, r.osc_tv,
+                                 r.osc_s_init, r.osc_s_final, r.osc_p, q.picks.data(),
+                                 (float*)w.o.p, &o));
+      return copy_out(w, r.output, w.o.p, vb);
+    default: return mexct::set_error(MEXCT_EINVAL, "server: unknown operator");
+  }
+}
+
+enum class Pick { kWait, kRun, kRelease };
+
+// Chooses an idle worker's next action (server lock held).
+Pick choose(mexct_server* sv, Worker& w, std::vector<Req*>& group) {
+  if (sv->queue.empty()) return Pick::kWait;
+  Req* head = sv->queue.front();
+  int64_t idle_reserved = 0;
+  for (auto& o : sv->workers)
+    if (!o->running) idle_reserved += o->reserved;
+  const int64_t others = sv->in_use - w.reserved;
+  for (auto it = sv->queue.begin(); it != sv->queue.end(); ++it) {
+    Req* q = *it;
+    const int64_t need = std::max(w.reserved, q->footprint);
+    if (others + need > sv->budget) {
+      if (q != head) continue;
+      // The oldest request does not fit.  Idle workers give their memory
+      // back when that alone makes room for it.
+      if (w.reserved > 0 && sv->in_use - idle_reserved + head->footprint <= sv->budget)
+        return Pick::kRelease;
+      if (head->overtaken >= kMaxOvertake) return Pick::kWait;
+      continue;
+    }
+    if (q != head) ++head->overtaken;  // head was blocked to get here
+    group.push_back(q);
+    w.reserved = need;
+    sv->in_use = others + need;
+    sv->peak_in_use = std::max(sv->peak_in_use, sv->in_use);
+    it = sv->queue.erase(it);
+    for (auto jt = it; jt != sv->queue.end() && (int)group.size() < sv->max_group;) {
+      if (same_key(**jt, *q)) {
+        group.push_back(*jt);
+        jt = sv->queue.erase(jt);
+      } else {
+        ++jt;
+      }
+    }
+    return Pick::kRun;
+  }
+  return Pick::kWait;
+}
+
+void worker_loop(mexct_server* sv, Worker* w) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    std::vector<Req*> group;
+    Pick pick = Pick::kWait;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      for (;;) {
+        if (sv->stop) return;
+        pick = choose(sv, *w, group);
+        if (pick != Pick::kWait) break;
+        sv->cv_work.wait(lk);
+      }
+      if (pick == Pick::kRun) {
+        w->running = true;
+        const double t = now_s();
+        for (Req* q : group) {
+          q->status = 2;
+          q->t_start = t;
+        }
+        ++sv->n_groups;
+        sv->n_grouped += (int64_t)group.size();
+      }
+    }
+    if (pick == Pick::kRelease) {
+      const int rc = release(sv, *w);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->in_use -= w->reserved;
+      w->reserved = 0;
+      ++sv->n_releases;
+      if (rc) sv->stop = true;  // the context could not be recreated: take no more work
+      sv->cv_work.notify_all();
+      sv->cv_done.notify_all();
+      continue;
+    }
+    for (Req* q : group) {
+      const int rc = run_one(*w, *q);
+      const std::string msg = rc ? std::string(mexct_last_error()) : std::string();
+      std::lock_guard<std::mutex> lk(sv->mu);
+      q->status = rc ? rc : 0;
+      q->err = msg;
+      q->t_done = now_s();
+      ++sv->n_done;
+      sv->cv_done.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      w->running = false;
+      const int64_t held = held_bytes(*w);
+      sv->in_use += held - w->reserved;
+      w->reserved = held;
+      sv->peak_in_use = std::max(sv->peak_in_use, sv->in_use);
+    }
+    sv->cv_work.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mexct_server_create(mexct_server** out, int device, int64_t hbm_budget, int32_t slots,
+                        int32_t max_group) {
+  if (!out) return mexct::set_error(MEXCT_EINVAL, "server is null");
+  if (slots < 1 || slots > 16 || hbm_budget <= 0)
+    return mexct::set_error(MEXCT_EINVAL, "server: need 1 <= slots <= 16 and a positive budget");
+  std::unique_ptr<mexct_server> sv(new mexct_server());
+  sv->device = device;
+  sv->budget = hbm_budget;
+  sv->max_group = max_group > 0 ? max_group : 1;
+  for (int i = 0; i < slots; ++i) {
+    std::unique_ptr<Worker> w(new Worker());
+    const int rc = mexct_ctx_create(&w->ctx, device, nullptr);
+    if (rc) {
+      for (auto& p : sv->workers) mexct_ctx_destroy(p->ctx);
+      return rc;
+    }
+    sv->workers.push_back(std::move(w));
+  }
+  for (auto& w : sv->workers) w->th = std::thread(worker_loop, sv.get(), w.get());
+  *out = sv.release();
+  return MEXCT_OK;
+}
+
+int mexct_server_submit(mexct_server* sv, const mexct_request* r, uint64_t* ticket) {
+  if (!sv || !r || !ticket) return mexct::set_error(MEXCT_EINVAL, "server: null argument");
+  if (r->op < MEXCT_OP_FP || r->op > MEXCT_OP_OSCTV)
+    return mexct::set_error(MEXCT_EINVAL, "server: unknown operator");
+  if (r->geom < 0 || r->geom > 2) return mexct::set_error(MEXCT_EGEOM, "invalid geometry");
+  if (r->W < 1 || r->H < 1 || r->nProj < 1 || r->nProj > INT32_MAX || !r->angles || !r->input ||
+      !r->output ||
+      (r->op == MEXCT_OP_OSCTV && (!r->input2 || !r->first_picks || r->osc_iters < 1)))
+    return mexct::set_error(MEXCT_EINVAL, "server: bad request dims or null array");
+  std::unique_ptr<Req> q(new Req());
+  q->r = *r;
+  q->angles.assign(r->angles, r->angles + r->nProj);
+  if (r->op == MEXCT_OP_OSCTV) q->picks.assign(r->first_picks, r->first_picks + r->osc_iters);
+  q->footprint = footprint_of(*r);
+  if (q->footprint > sv->budget)
+    return mexct::set_error(MEXCT_ENOMEM, "server: request needs more than the HBM budget");
+  q->t_submit = now_s();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sv->stop) return mexct::set_error(MEXCT_EHIP, "server: stopped");
+  q->id = sv->next_id++;
+  *ticket = q->id;
+  sv->queue.push_back(q.get());
+  sv->reqs[q->id] = std::move(q);
+  sv->cv_work.notify_all();
+  return MEXCT_OK;
+}
+
+int mexct_server_wait(mexct_server* sv, uint64_t ticket, int32_t* status, double* latency_s,
+                      double* queue_s) {
+  if (!sv) return mexct::set_error(MEXCT_EINVAL, "server is null");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return mexct::set_error(MEXCT_EINVAL, "server: unknown ticket");
+  Req* q = it->second.get();
+  sv->cv_done.wait(lk, [&] { return q->status <= 0 || (sv->stop && q->status == 1); });
+  if (q->status == 1) return mexct::set_error(MEXCT_EHIP, "server: stopped before the request ran");
+  if (status) *status = q->status;
+  if (latency_s) *latency_s = q->t_done - q->t_submit;
+  if (queue_s) *queue_s = q->t_start - q->t_submit;
+  if (q->status) mexct::set_error(q->status, q->err.c_str());
+  sv->reqs.erase(it);
+  return MEXCT_OK;
+}
+
+int mexct_server_stats(mexct_server* sv, int64_t* stats, int32_t n) {
+  if (!sv || !stats || n < 1) return mexct::set_error(MEXCT_EINVAL, "server: null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  const int64_t v[7] = {sv->n_done, sv->n_groups,      sv->n_grouped, (int64_t)sv->queue.size(),
+                        sv->in_use, sv->peak_in_use, sv->n_releases};
+  for (int i = 0; i < n && i < 7; ++i) stats[i] = v[i];
+  return MEXCT_OK;
+}
+
+int mexct_server_destroy(mexct_server* sv) {
+  if (!sv) return MEXCT_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  sv->cv_done.notify_all();
+  for (auto& w : sv->workers)
+    if (w->th.joinable()) w->th.join();
+  (void)hipSetDevice(sv->device);
+  for (auto& w : sv->workers) {
+    if (w->ctx) (void)mexct_ctx_sync(w->ctx);
+    for (Buf* b : {&w->a, &w->b, &w->o, &w->x})
+      if (b->p) (void)hipFree(b->p);
+    mexct_ctx_destroy(w->ctx);
+  }
+  delete sv;
+  return MEXCT_OK;
+}
+
+}  // extern "C"

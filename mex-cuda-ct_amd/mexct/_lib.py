@@ -29,6 +29,19 @@ class Opts(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class Request(C.Structure):
+    """mexct_request: one request of the continuous-batching server."""
+    _fields_ = [("op", C.c_int32), ("geom", C.c_int32), ("storage", C.c_int32),
+                ("window", C.c_int32), ("W", C.c_int64), ("H", C.c_int64), ("nProj", C.c_int64),
+                ("sad", C.c_float), ("osc_tv", C.c_float), ("cutoff", C.c_double),
+                ("osc_p", C.c_double), ("osc_iters", C.c_int32), ("osc_s_init", C.c_int32),
+                ("osc_s_final", C.c_int32), ("tv_steps", C.c_int32), ("angles", C.c_void_p),
+                ("input", C.c_void_p), ("input2", C.c_void_p), ("output", C.c_void_p),
+                ("first_picks", C.c_void_p)]
+
+
+OP_FP, OP_BP, OP_FDK, OP_OSCTV = 0, 1, 2, 3
+
 _vp, _i32, _i64, _f, _d = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double
 _op = C.POINTER(Opts)
 
@@ -59,6 +72,11 @@ SIGNATURES = {
     "mexct_osc_tv_recon_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _f, _i32, _i32, _f, _i32,
                                _i32, _d, _vp, _vp, _op],
     "mexct_probe_gather_dev": [_vp, _i32, _i64, _i64, _i32, _i32, C.POINTER(_i64)],
+    "mexct_server_create": [C.POINTER(_vp), C.c_int, _i64, _i32, _i32],
+    "mexct_server_submit": [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)],
+    "mexct_server_wait": [_vp, C.c_uint64, C.POINTER(_i32), C.POINTER(_d), C.POINTER(_d)],
+    "mexct_server_stats": [_vp, C.POINTER(_i64), _i32],
+    "mexct_server_destroy": [_vp],
     "mexct_calc_num_subsets": [_i32, _i32, _d, _i32, _i32],
     "mexct_calc_subset_order": [_i32, _i32, _vp],
 }

@@ -102,3 +102,35 @@ def test_subset_helpers_match_oracle():
     for n in range(1, 40):
         for first in range(1, n + 1):
             assert np.array_equal(mexct.calc_subset_order(n, first), O.calc_subset_order(n, first))
+
+
+def test_request_struct_layout_matches_header(tmp_path):
+    """mexct_request as ctypes sees it must match the C compiler's layout."""
+    import shutil
+    import subprocess
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    fields = [f for f, _ in L.Request._fields_]
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "mexct.h"\nint main(void){'
+                   'printf("%zu", sizeof(mexct_request));'
+                   + "".join(f'printf(" %zu", offsetof(mexct_request, {f}));' for f in fields)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got[0] == C.sizeof(L.Request)
+    assert got[1:] == [getattr(L.Request, f).offset for f in fields]
+
+
+def test_server_create_rejects_bad_arguments():
+    h = C.c_void_p()
+    assert L.lib().mexct_server_create(C.byref(h), 0, 1 << 30, 0, 1) == L.EINVAL
+    assert L.lib().mexct_server_create(C.byref(h), 0, 0, 2, 1) == L.EINVAL
+    assert L.lib().mexct_server_create(None, 0, 1 << 30, 2, 1) == L.EINVAL
+    assert L.lib().mexct_server_submit(None, None, None) == L.EINVAL
+    assert L.lib().mexct_server_destroy(None) == L.OK

@@ -1,0 +1,148 @@
+"""The continuous-batching request server (mexct_server_*).
+
+A request's result must be bit-identical whether it runs alone (through the
+direct entry points) or in a mix of concurrent requests of other operators
+and sizes; the HBM budget bounds the reserved device memory; same-shaped
+requests are grouped; a failing request does not disturb the others.
+"""
+import numpy as np
+import pytest
+
+import mexct
+import oracle as O
+from mexct.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _angles(n, off=0.05):
+    return (np.arange(n) * (2 * np.pi / n) + off).astype(np.float32)
+
+
+def _vol(W, H, seed):
+    return np.asfortranarray(_rng(seed).uniform(0, 1, (W, W, H)).astype(np.float32))
+
+
+def _sino(W, H, n, seed):
+    return np.asfortranarray(_rng(seed).uniform(0, 1, (W, H, n)).astype(np.float32))
+
+
+def _mix():
+    """[(kind, args, alone_fn)] covering every operator and geometry."""
+    cases = []
+    for k, (W, H, n, geom, sad) in enumerate([(48, 40, 24, 2, 200.0), (64, 20, 16, 0, 500.0),
+                                              (40, 44, 30, 1, 150.0), (96, 64, 48, 2, 400.0)]):
+        vol, ang = _vol(W, H, 100 + k), _angles(n)
+        cases.append(("fp", (vol, sad, ang, n, geom),
+                      lambda a=(vol, sad, ang, n, geom): mexct.CUDAmex_FP(*a)))
+        P = _sino(W, H, n, 200 + k)
+        cases.append(("bp", (P, sad, ang, geom),
+                      lambda a=(P, sad, ang, geom): mexct.CUDAmex_BP(*a)))
+        g = {"type": ["par3d", "fan3d", "cone3d"][geom], "SAD": sad}
+        cases.append(("fdk", (P, g, ang, "hamming", 0.8),
+                      lambda P=P, g=g, ang=ang, sad=sad, geom=geom: mexct.CUDAmex_BP(
+                          mexct.preFilterSinogram(P, g, ang, "hamming", 0.8), sad, ang, geom)))
+    W, H, n = 32, 12, 24
+    ang = _angles(n)
+    vol = _vol(W, H, 7) * 0.02
+    post = np.asfortranarray(np.exp(-mexct.CUDAmex_FP(vol, 300.0, ang, n, 2)) * 1e4)
+    pre = np.asfortranarray(np.full((W, H, n), 1e4, np.float32))
+    g = {"type": "cone3d", "SAD": 300.0}
+    osc = (pre, post, ang, g, 4, 0.1, 6, 2, 1.0)
+    cases.append(("osc", osc, lambda: mexct.OSC_TV_recon(*osc, tv_steps=5, seed=11)))
+    return cases
+
+
+def _submit(s, kind, a):
+    if kind == "fp":
+        return s.submit_fp(*a)
+    if kind == "bp":
+        return s.submit_bp(*a)
+    if kind == "fdk":
+        return s.submit_fdk(*a)
+    return s.submit_osc_tv(*a, tv_steps=5, seed=11)
+
+
+def test_alone_equals_in_mix(gpu):
+    cases = _mix()
+    alone = [fn() for _, _, fn in cases]
+    order = _rng(3).permutation(len(cases) * 2) % len(cases)  # every case twice, shuffled
+    with Server(hbm_budget=8 << 30, slots=4, max_group=3) as s:
+        tickets = [(_submit(s, cases[i][0], cases[i][1]), i) for i in order]
+        lat = []
+        for t, i in tickets:
+            out, l, q = s.wait(t)
+            lat.append(l)
+            assert q >= 0 and l >= q
+            assert np.array_equal(out, alone[i]), f"{cases[i][0]} case {i} differs in the mix"
+        st = s.stats()
+    print(f"mix: {len(tickets)} requests p50={np.percentile(lat, 50) * 1e3:.2f} ms "
+          f"p99={np.percentile(lat, 99) * 1e3:.2f} ms stats={st}")
+    assert st["done"] == len(tickets) and st["queued"] == 0
+    assert st["peak_reserved_bytes"] <= 8 << 30
+
+
+def test_server_fp_matches_oracle(gpu):
+    W, H, n = 36, 30, 12
+    vol, ang = _vol(W, H, 41), _angles(n)
+    with Server(hbm_budget=1 << 30, slots=2) as s:
+        got = s.result(s.submit_fp(vol, 250.0, ang, n, 2))
+    assert np.array_equal(got, O.fp(vol, 250.0, ang, n, 2))
+
+
+def test_budget_bounds_reservations(gpu):
+    # footprints of ~2/3 of the budget: at most one request runs at a time
+    W, H, n = 128, 96, 64
+    P = [_sino(W, H, n, 300 + k) for k in range(4)]
+    ang = _angles(n)
+    one = Server(hbm_budget=1 << 40, slots=1)
+    probe = one.submit_bp(P[0], 500.0, ang, 2)
+    ref0 = one.result(probe)
+    need = one.stats()["peak_reserved_bytes"]
+    one.close()
+    budget = int(need * 1.5)
+    with Server(hbm_budget=budget, slots=4, max_group=1) as s:
+        ts = [s.submit_bp(p, 500.0, ang, 2) for p in P]
+        outs = [s.result(t) for t in ts]
+        st = s.stats()
+    assert np.array_equal(outs[0], ref0)
+    assert st["peak_reserved_bytes"] <= budget
+    assert st["done"] == 4
+
+
+def test_same_shape_requests_are_grouped(gpu):
+    W, H, n = 32, 16, 8
+    ang = _angles(n)
+    vols = [_vol(W, H, 500 + k) for k in range(8)]
+    with Server(hbm_budget=1 << 30, slots=1, max_group=4) as s:
+        ts = [s.submit_fp(v, 300.0, ang, n, 2) for v in vols]
+        outs = [s.result(t) for t in ts]
+        st = s.stats()
+    for v, o in zip(vols, outs):
+        assert np.array_equal(o, mexct.CUDAmex_FP(v, 300.0, ang, n, 2))
+    assert st["grouped"] == 8 and st["groups"] < 8
+
+
+def test_failing_request_is_isolated(gpu):
+    W, H = 32, 16
+    ang = _angles(8)
+    bad = _sino(W, H, 1, 9)  # the FDK filter needs two views
+    good = _vol(W, H, 10)
+    with Server(hbm_budget=1 << 30, slots=2) as s:
+        tb = s.submit_fdk(bad, {"type": "cone3d", "SAD": 300.0}, ang[:1])
+        tg = s.submit_fp(good, 300.0, ang, 8, 2)
+        with pytest.raises(mexct.MexctError):
+            s.wait(tb)
+        out = s.result(tg)
+    assert np.array_equal(out, mexct.CUDAmex_FP(good, 300.0, ang, 8, 2))
+
+
+def test_submit_rejects_over_budget(gpu):
+    with Server(hbm_budget=1 << 20, slots=1) as s:
+        with pytest.raises(mexct.MexctError) as e:
+            s.submit_bp(_sino(64, 64, 64, 1), 300.0, _angles(64), 2)
+        assert e.value.code == -4
