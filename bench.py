@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--views", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -229,6 +232,10 @@ def main():
                          f"columns x {H} z x {n} views ({tb:.1f} s); FP32 oracle"}
         log(f"[bench] cpu: FP {cfp:.3f} GUPS, BP {cbp:.3f} GUPS")
 
+    serve = None
+    if not args.no_serve:
+        serve = serving(args.serve_requests, args.serve_rate)
+
     line = {
         "metric": "cone-beam FP/BP GUPS 384^3x512",
         "value": round(value, 2),
@@ -252,8 +259,165 @@ def main():
         "gather_roofline": gather,
         "fp16_storage": f16,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
+
+
+SERVE_SIDES = (128, 192, 256, 384, 512)
+SERVE_VIEWS = (90, 180, 360, 720)
+
+
+def serving_stream(n_req, rate, seed=5):
+    """The mixed request stream: 40% FP, 30% BP, 20% FDK, 10% OSC-TV (5
+    iterations), cubic sides from SERVE_SIDES, views from SERVE_VIEWS,
+    par/fan/cone, SAD ~ U[800,1200], FP32 or FP16 storage, Poisson arrivals
+    at `rate` per second."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kinds = rng.choice(["fp", "bp", "fdk", "osc"], size=n_req, p=[0.4, 0.3, 0.2, 0.1])
+    reqs = []
+    t = 0.0
+    for k in kinds:
+        t += float(rng.exponential(1.0 / rate))
+        reqs.append({"kind": str(k), "side": int(rng.choice(SERVE_SIDES)),
+                     "views": int(rng.choice(SERVE_VIEWS)), "geom": int(rng.integers(0, 3)),
+                     "sad": float(np.float32(rng.uniform(800.0, 1200.0))),
+                     "storage": int(rng.integers(0, 2)), "arrival": t})
+    return reqs
+
+
+def serving(n_req, rate):
+    """Open-loop serving run of serving_stream through mexct.serve.Server,
+    a saturated run (all requests queued at once) and the one-at-a-time
+    plain-FP32 loop over the same requests through the direct entry points.
+    Latency = arrival -> output in host memory (host arrays in and out)."""
+    import threading
+
+    import mexct
+    from mexct import phantoms
+    from mexct.serve import Server
+
+    reqs = serving_stream(n_req, rate)
+    log(f"[serve] {n_req} requests, rate {rate}/s; building inputs")
+    vols, sinos, angs = {}, {}, {}
+    for r in reqs:
+        W, v = r["side"], r["views"]
+        if W not in vols:
+            vols[W] = phantoms.cylinder(W, W, 0.36 * W)
+        if (W, v) not in sinos:
+            sinos[(W, v)] = np.asfortranarray(np.full((W, W, v), 0.5, np.float32))
+        if v not in angs:
+            angs[v] = phantoms.angles_uniform(v, 2 * np.pi)
+    pre_c = {}
+
+    def osc_args(r):
+        W, v = r["side"], r["views"]
+        if (W, v) not in pre_c:
+            pre_c[(W, v)] = (np.asfortranarray(np.full((W, W, v), 1e5, np.float32)),
+                             np.asfortranarray(np.full((W, W, v), 6e4, np.float32)))
+        pre, post = pre_c[(W, v)]
+        g = {"type": ["par3d", "fan3d", "cone3d"][r["geom"]], "SAD": r["sad"]}
+        return pre, post, angs[v], g, 5, 0.1, 10, 10, 1.0
+
+    def submit(s, r):
+        W, v, g, sad, st = r["side"], r["views"], r["geom"], r["sad"], r["storage"]
+        if r["kind"] == "fp":
+            return s.submit_fp(vols[W], sad, angs[v], v, g, storage=st)
+        if r["kind"] == "bp":
+            return s.submit_bp(sinos[(W, v)], sad, angs[v], g, storage=st)
+        if r["kind"] == "fdk":
+            return s.submit_fdk(sinos[(W, v)], {"type": ["par3d", "fan3d", "cone3d"][g],
+                                                "SAD": sad}, angs[v], "hamming", 1.0,
+                                storage=st)
+        return s.submit_osc_tv(*osc_args(r), tv_steps=10, picks=np.ones(5, np.int32), storage=st)
+
+    def run_alone(r):
+        W, v, g, sad = r["side"], r["views"], r["geom"], r["sad"]
+        if r["kind"] == "fp":
+            return mexct.CUDAmex_FP(vols[W], sad, angs[v], v, g)
+        if r["kind"] == "bp":
+            return mexct.CUDAmex_BP(sinos[(W, v)], sad, angs[v], g)
+        if r["kind"] == "fdk":
+            geo = {"type": ["par3d", "fan3d", "cone3d"][g], "SAD": sad}
+            return mexct.CUDAmex_BP(mexct.preFilterSinogram(sinos[(W, v)], geo, angs[v],
+                                                            "hamming", 1.0), sad, angs[v], g)
+        return mexct.OSC_TV_recon(*osc_args(r), tv_steps=10, picks=np.ones(5, np.int32))
+
+    budget = 96 << 30
+
+    def served(open_loop):
+        lats, queues = [None] * n_req, [None] * n_req
+        with Server(hbm_budget=budget, slots=4, max_group=4) as s:
+            tickets = [None] * n_req
+            ready = threading.Semaphore(0)
+
+            def waiter():
+                for i in range(n_req):
+                    ready.acquire()
+                    _, lat, q = s.wait(tickets[i][0])
+                    lats[i] = tickets[i][1] + lat
+                    queues[i] = q
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            for i, r in enumerate(reqs):
+                if open_loop:
+                    d = t0 + r["arrival"] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    lag = time.perf_counter() - (t0 + r["arrival"])
+                else:
+                    lag = 0.0
+                tickets[i] = (submit(s, r), lag)
+                ready.release()
+            th.join()
+            wall = time.perf_counter() - t0
+            st = s.stats()
+        return np.array(lats), wall, st
+
+    log("[serve] saturated run")
+    lat_sat, wall_sat, st_sat = served(False)
+    log(f"[serve] saturated: {n_req / wall_sat:.2f} req/s, stats {st_sat}")
+    log("[serve] open-loop run")
+    lat_open, wall_open, st_open = served(True)
+    log(f"[serve] open loop: p50 {np.percentile(lat_open, 50) * 1e3:.1f} ms "
+        f"p99 {np.percentile(lat_open, 99) * 1e3:.1f} ms")
+    log("[serve] one-at-a-time FP32 loop")
+    svc = []
+    for r in reqs:
+        t = time.perf_counter()
+        run_alone(r)
+        svc.append(time.perf_counter() - t)
+    svc = np.array(svc)
+    # the same arrivals through a FIFO single server with these service times
+    arr = np.array([r["arrival"] for r in reqs])
+    done = np.zeros(n_req)
+    prev = 0.0
+    for i in range(n_req):
+        prev = max(arr[i], prev) + svc[i]
+        done[i] = prev
+    lat_seq = done - arr
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 2)  # noqa: E731
+    return {
+        "workload": f"{n_req} requests (seed 5): 40% FP, 30% BP, 20% FDK, 10% OSC-TV "
+                    "(5 iterations, 10 subsets, 10 TV steps); cubic sides "
+                    f"{list(SERVE_SIDES)}, views {list(SERVE_VIEWS)}, par/fan/cone, "
+                    "SAD U[800,1200], f32 or f16 storage; host arrays in and out",
+        "arrival_rate_per_s": rate,
+        "hbm_budget_bytes": budget, "slots": 4, "max_group": 4,
+        "open_loop": {"p50_ms": pct(lat_open, 50), "p99_ms": pct(lat_open, 99),
+                      "mean_ms": round(float(lat_open.mean()) * 1e3, 2),
+                      "throughput_req_s": round(n_req / wall_open, 3),
+                      "peak_reserved_gb": round(st_open["peak_reserved_bytes"] / 1e9, 2)},
+        "saturated": {"throughput_req_s": round(n_req / wall_sat, 3),
+                      "p50_ms": pct(lat_sat, 50), "p99_ms": pct(lat_sat, 99),
+                      "groups": st_sat["groups"], "grouped": st_sat["grouped"]},
+        "one_at_a_time_fp32": {"throughput_req_s": round(n_req / float(svc.sum()), 3),
+                               "p50_ms": pct(lat_seq, 50), "p99_ms": pct(lat_seq, 99),
+                               "note": "measured service times replayed through a FIFO "
+                                       "single server with the same arrivals"},
+    }
 
 
 if __name__ == "__main__":

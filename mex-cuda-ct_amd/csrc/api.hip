@@ -80,6 +80,7 @@ enum Slot {
   WS_POSTW,
   WS_PLAIN,
   WS_MUZ,      // z-fastest copy of the OSC volume
+  WS_SINOMAX,  // max(|pre|, |post|) bits for the FP16 OSC copies
   WS_COUNT
 };
 
@@ -633,33 +634,52 @@ namespace {
 
 // One CUDAmex_oscIter pass over the subsets on the device state: mu_pad is
 // the zero-haloed volume, pre/post [W,H,nProj] are rewritten in place.
+// One CUDAmex_oscIter call on the padded f32 state mu_pad.  The subset
+// projections read a z-fastest copy of mu_pad in storage st, and the dual
+// backprojection reads the ratio terms in storage st (FP16: scaled by a
+// power of two from max(|pre|, |post|), which the ratio N/D cancels).
 int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* post, int nProj,
-                int numSubsets, const int32_t* picks, const float2* sc, float sad, int geom) {
+                int numSubsets, const int32_t* picks, const float2* sc, float sad, int geom,
+                int st) {
   const int cmax = (nProj + numSubsets - 1) / numSubsets;
   const size_t pv = (size_t)(W + 4) * (H + 4);
+  const size_t es = (size_t)store_bytes(st);
+  // whole 32-bit words so the zero fill covers every 2-byte element
+  const size_t pwords = (pv * std::max(cmax, 1) * es + 3) / 4;
   void *pp, *qp;
-  RC(c->get(WS_PREPAD, pv * std::max(cmax, 1) * 4, &pp));
-  RC(c->get(WS_POSTPAD, pv * std::max(cmax, 1) * 4, &qp));
+  RC(c->get(WS_PREPAD, pwords * 4, &pp));
+  RC(c->get(WS_POSTPAD, pwords * 4, &qp));
   // halos stay zero: the kernels only ever write the interior
-  HIPCHK(launch_fill((float*)pp, 0.f, pv * std::max(cmax, 1), c->stream));
-  HIPCHK(launch_fill((float*)qp, 0.f, pv * std::max(cmax, 1), c->stream));
+  HIPCHK(launch_fill((float*)pp, 0.f, pwords, c->stream));
+  HIPCHK(launch_fill((float*)qp, 0.f, pwords, c->stream));
   void* mz;
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
-  RC(c->get(WS_MUZ, (size_t)MPXY * (H + 4) * 4, &mz));
+  RC(c->get(WS_MUZ, (size_t)MPXY * (H + 4) * es, &mz));
+  uint32_t* smax = nullptr;
+  if (st == MEXCT_F16) {
+    void* m;
+    RC(c->get(WS_SINOMAX, 256, &m));
+    smax = (uint32_t*)m;
+    HIPCHK(launch_fill((float*)smax, 0.f, 1, c->stream));
+    HIPCHK(launch_absmax(pre, (size_t)W * H * nProj, smax, c->stream));
+    HIPCHK(launch_absmax(post, (size_t)W * H * nProj, smax, c->stream));
+  }
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
   for (int s = 0; s < numSubsets; ++s) {
     const int sel = picks[s];
     const int count = (sel >= 0 && sel < nProj) ? (nProj - 1 - sel) / numSubsets + 1 : 0;
     if (count > 0) {
       // the subset projects the volume as left by the previous update
-      HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, MEXCT_F32, W, W, H, ADDR_BORDER,
+      HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
                           c->stream));
       OSCFPLaunch f;
-      f.mu_z = (const float*)mz;
+      f.mu_z = mz;
+      f.st_dt = st;
       f.pre = pre;
       f.post = post;
-      f.pre_pad = (float*)pp;
-      f.post_pad = (float*)qp;
+      f.pre_pad = pp;
+      f.post_pad = qp;
+      f.sino_max = smax;
       f.sc = sc;
       f.geom = geom;
       f.W = W;
@@ -672,8 +692,9 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
       HIPCHK(launch_osc_fp(f, c->stream));
     }
     OSCBPLaunch b;
-    b.pre_pad = (const float*)pp;
-    b.post_pad = (const float*)qp;
+    b.pre_pad = pp;
+    b.post_pad = qp;
+    b.st_dt = st;
     b.mu_pad = mu_pad;
     b.sc = sc;
     b.geom = geom;
@@ -704,9 +725,10 @@ int osc_check(int64_t W, int64_t H, int64_t nProj, int32_t numSubsets, int32_t g
 int mexct_osc_iter_dev(mexct_ctx* c, float* mu, int64_t W, int64_t H, float* pre, float* post,
                        int64_t nProj, int32_t numSubsets, const int32_t* picks,
                        const float* angles, float sad, int32_t geom, const mexct_opts* opts) {
-  (void)opts;
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(osc_check(W, H, nProj, numSubsets, geom));
+  const int st = storage_of(opts);
+  RC(check_dtype(st, "storage"));
   if (!mu || !picks || (nProj > 0 && (!pre || !post || !angles)))
     return fail(MEXCT_EINVAL, "null array argument");
   DevGuard g(c->device);
@@ -716,7 +738,7 @@ int mexct_osc_iter_dev(mexct_ctx* c, float* mu, int64_t W, int64_t H, float* pre
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   RC(osc_subsets(c, (float*)mp, (int)W, (int)H, pre, post, (int)nProj, numSubsets, picks, sc, sad,
-                 geom));
+                 geom, st));
   HIPCHK(launch_unpad3((const float*)mp, mu, (int)W, (int)W, (int)H, c->stream));
   return MEXCT_OK;
 }
@@ -953,6 +975,8 @@ int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, in
   if (nIter < 2) return fail(MEXCT_EINVAL, "nIter must be >= 2 (calcNumSubsets divides by (nIter-1)^p)");
   if (!pre || !post || !angles || !first_picks || !mu_out) return fail(MEXCT_EINVAL, "null array argument");
   RC(check_padded(W, W, H, "volume"));
+  const int st_dt = storage_of(opts);
+  RC(check_dtype(st_dt, "storage"));
   DevGuard g(c->device);
   hipStream_t st = c->stream;
   const size_t psz = (size_t)W * H * nProj, vsz = (size_t)W * W * H;
@@ -1001,7 +1025,7 @@ int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, in
       HIPCHK(hipMemcpyAsync(postw, post2, psz * 4, hipMemcpyDeviceToDevice, st));
     }
     RC(osc_subsets(c, (float*)mp, (int)W, (int)H, wpre, wpost, (int)nProj, n, picks.data(), sc,
-                   sad, geom));
+                   sad, geom, st_dt));
     if (k > 1) {
       HIPCHK(launch_diff_norm(old_v, mu_v, (double*)pp, sum, st));
       HIPCHK(launch_tv_const(sum, tv, cdev, st));

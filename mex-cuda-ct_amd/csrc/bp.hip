@@ -230,8 +230,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
 // and :113-136) for one subset: num/den sums stay in registers over the
 // subset's views and the voxel is updated in place, inside the circle only.
 struct OSCBPKArgs {
-  const float* pre_pad;
-  const float* post_pad;
+  const void* pre_pad;   // storage S
+  const void* post_pad;
   float* mu_pad;
   const float2* sc;
   int W, H, PW, first, stride, count;
@@ -242,7 +242,7 @@ struct OSCBPKArgs {
 
 constexpr int OSC_ZR = 32;
 
-template <int GEOM>
+template <int GEOM, typename S>
 __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
@@ -292,8 +292,8 @@ __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
     const float ax = xb - fx;
     const int i0 = (int)fminf(fmaxf(fx, -2.f), (float)a.W);
     const size_t coff = (size_t)j * a.PV + (i0 + HALO);
-    const float* colN = a.pre_pad + coff;
-    const float* colD = a.post_pad + coff;
+    const S* colN = (const S*)a.pre_pad + coff;
+    const S* colD = (const S*)a.post_pad + coff;
     if (GEOM == GEOM_CONE) {
       for (int m = 0; m < z0; ++m) pz = pz + U;
     } else {
@@ -306,10 +306,10 @@ __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
       const float az = zb - fz;
       const int j0 = (int)fminf(fmaxf(fz, -2.f), (float)a.H);
       const size_t r = (size_t)(j0 + HALO) * a.PW;
-      const float2 n0 = Store<float>::ld2(colN + r);
-      const float2 n1 = Store<float>::ld2(colN + r + a.PW);
-      const float2 d0 = Store<float>::ld2(colD + r);
-      const float2 d1 = Store<float>::ld2(colD + r + a.PW);
+      const float2 n0 = Store<S>::ld2(colN + r);
+      const float2 n1 = Store<S>::ld2(colN + r + a.PW);
+      const float2 d0 = Store<S>::ld2(colD + r);
+      const float2 d1 = Store<S>::ld2(colD + r + a.PW);
       const float tn = lerp(lerp(n0.x, n0.y, ax), lerp(n1.x, n1.y, ax), az);
       const float td = lerp(lerp(d0.x, d0.y, ax), lerp(d1.x, d1.y, ax), az);
       accN[m] = accN[m] + W2 * tn;
@@ -325,6 +325,15 @@ __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
       const float fval = mu[(size_t)m * a.MPXY];
       mu[(size_t)m * a.MPXY] = fmaxf(fval + (fval * accN[m] / accD[m]), 0.f);
     }
+  }
+}
+
+template <typename S>
+static void osc_bp_dispatch(const OSCBPKArgs& k, int geom, unsigned g, hipStream_t st) {
+  switch (geom) {
+    case GEOM_PAR: osc_bp_update_kernel<GEOM_PAR, S><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: osc_bp_update_kernel<GEOM_FAN, S><<<g, 64, 0, st>>>(k); break;
+    default: osc_bp_update_kernel<GEOM_CONE, S><<<g, 64, 0, st>>>(k); break;
   }
 }
 
@@ -350,11 +359,36 @@ hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st) {
   k.n_logical = k.tiles_x * k.tiles_x * k.nz;
   if (k.n_logical == 0) return hipSuccess;
   const unsigned g = padded_grid(k.n_logical);
-  switch (a.geom) {
-    case GEOM_PAR: osc_bp_update_kernel<GEOM_PAR><<<g, 64, 0, st>>>(k); break;
-    case GEOM_FAN: osc_bp_update_kernel<GEOM_FAN><<<g, 64, 0, st>>>(k); break;
-    default: osc_bp_update_kernel<GEOM_CONE><<<g, 64, 0, st>>>(k); break;
+  if (a.st_dt == 0)
+    osc_bp_dispatch<float>(k, a.geom, g, st);
+  else if (a.st_dt == 1)
+    osc_bp_dispatch<_Float16>(k, a.geom, g, st);
+  else
+    osc_bp_dispatch<bf16_t>(k, a.geom, g, st);
+  return hipGetLastError();
+}
+
+namespace {
+// Block maxima of |x| combined with one atomicMax per block; non-negative
+// floats order like their bit patterns.
+__global__ __launch_bounds__(256) void absmax_kernel(const float* x, size_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const float v = fabsf(x[i]);
+    if (v == v) m = max(m, __float_as_uint(v));
   }
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  __shared__ uint32_t wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
+}
+}  // namespace
+
+hipError_t launch_absmax(const float* x, size_t n, uint32_t* out_bits, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+  absmax_kernel<<<g, 256, 0, st>>>(x, n, out_bits);
   return hipGetLastError();
 }
 

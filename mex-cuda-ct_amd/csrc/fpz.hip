@@ -27,8 +27,9 @@ struct FPZArgs {
   float* out;
   float* pre;          // OSC epilogue
   float* post;
-  float* pre_pad;
-  float* post_pad;
+  void* pre_pad;       // storage type S, FP16 scaled by 2^-e (e: exponent of *sino_max)
+  void* post_pad;
+  const uint32_t* sino_max;  // bits of max(|pre|, |post|), read when S is FP16
   const float2* sc;
   int W, H, rows, NX, NY, NZ;
   int PY, PZ;
@@ -133,6 +134,15 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
   return dval;
 }
 
+// The FP16 copies of the OSC ratio terms are scaled by an exact power of two
+// (counts reach 1e5, beyond FP16's 65504); the backprojection only forms the
+// ratio of two sums of them, which the common scale leaves unchanged.
+template <typename S>
+__device__ __forceinline__ float scale_pow2(float v, int e) {
+  if (__is_same(S, _Float16)) return ldexpf(v, -e);
+  return v;
+}
+
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
 // written in place, plus zero-haloed copies for the dual backprojection.
 template <int GEOM, typename S, int EPI>
@@ -150,6 +160,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const size_t p = (size_t)view * a.W * a.H + (size_t)iy * a.W + ix;
   float preval = 0.f, postval = 0.f;
   size_t q = 0;
+  int sexp = 0;  // FP16 copies hold value * 2^-sexp, so that |value| <= 1
   if (EPI == 0) {
     if (iy >= a.rows) {  // the reference's grid never writes these rows
       a.out[p] = 0.f;
@@ -159,9 +170,10 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     preval = a.pre[p];
     postval = a.post[p];
     q = (size_t)jv * a.PV + (size_t)(iy + HALO) * a.PW + (ix + HALO);
+    if (__is_same(S, _Float16)) (void)frexpf(__uint_as_float(*a.sino_max), &sexp);
     if (iy >= a.rows) {  // unprocessed rows keep their values
-      a.pre_pad[q] = preval;
-      a.post_pad[q] = postval;
+      ((S*)a.pre_pad)[q] = Store<S>::from(scale_pow2<S>(preval, sexp));
+      ((S*)a.post_pad)[q] = Store<S>::from(scale_pow2<S>(postval, sexp));
       return;
     }
   }
@@ -183,8 +195,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     const float npo = preval * expf(-dval) * dval;
     a.pre[p] = np;
     a.post[p] = npo;
-    a.pre_pad[q] = np;
-    a.post_pad[q] = npo;
+    ((S*)a.pre_pad)[q] = Store<S>::from(scale_pow2<S>(np, sexp));
+    ((S*)a.post_pad)[q] = Store<S>::from(scale_pow2<S>(npo, sexp));
   }
 }
 
@@ -246,17 +258,23 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
 }
 
 hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.mu_z, 0, a.sc, a.W, a.H, a.count, a.sad);
+  FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.sc, a.W, a.H, a.count, a.sad);
   k.pre = a.pre;
   k.post = a.post;
   k.pre_pad = a.pre_pad;
   k.post_pad = a.post_pad;
+  k.sino_max = a.sino_max;
   k.first = a.first;
   k.stride = a.stride;
   k.PW = a.W + 2 * HALO;
   k.PV = (a.W + 2 * HALO) * (a.H + 2 * HALO);
   if (k.n_logical == 0) return hipSuccess;
-  fpz_dispatch<1, float>(k, a.geom, st);
+  if (a.st_dt == 0)
+    fpz_dispatch<1, float>(k, a.geom, st);
+  else if (a.st_dt == 1)
+    fpz_dispatch<1, _Float16>(k, a.geom, st);
+  else
+    fpz_dispatch<1, bf16_t>(k, a.geom, st);
   return hipGetLastError();
 }
 

@@ -55,11 +55,13 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 
 // OSC forward projection + ratio terms over the views first, first+stride, ...
 struct OSCFPLaunch {
-  const float* mu_z;    // z-fastest f32 volume with a zero (border) halo
+  const void* mu_z;     // z-fastest volume of storage st_dt with a zero (border) halo
+  int st_dt;            // storage of mu_z and of pre_pad / post_pad
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
-  float* pre_pad;       // [W+4,H+4,count] padded copies for the BP (zero halo)
-  float* post_pad;
+  void* pre_pad;        // [W+4,H+4,count] padded copies for the BP (zero halo)
+  void* post_pad;
+  const uint32_t* sino_max;  // FP16: bits of max(|pre|,|post|); copies are scaled by 2^-e
   const float2* sc;
   int geom, W, H, nProj, first, stride, count;
   float sad;
@@ -79,14 +81,18 @@ struct BPLaunch {
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
 
 struct OSCBPLaunch {
-  const float* pre_pad;   // [W+4, H+4, count]
-  const float* post_pad;
+  const void* pre_pad;    // [W+4, H+4, count] of storage st_dt
+  const void* post_pad;
+  int st_dt;
   float* mu_pad;          // padded f32 volume, interior updated in place
   const float2* sc;       // all views
   int geom, W, H, first, stride, count;
   float sad;
 };
 hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st);
+// *out_bits = max(*out_bits, bits of max |x[i]|) over n floats (NaN ignored);
+// the caller zero-fills *out_bits first.
+hipError_t launch_absmax(const float* x, size_t n, uint32_t* out_bits, hipStream_t st);
 
 // general-3D ray-driven backprojection (atomics)
 struct G3DBPLaunch {

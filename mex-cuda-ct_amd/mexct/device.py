@@ -147,14 +147,14 @@ class Context:
                    _dptr(pts, "pts", True), ppv, int(M), int(N), _dptr(out, "out", True), None)
         return out
 
-    def osc_iter(self, mu, pre, post, num_subsets, picks, angles, sad, geom):
+    def osc_iter(self, mu, pre, post, num_subsets, picks, angles, sad, geom, storage=F32):
         H, W, _ = mu.shape
         n = pre.shape[0]
         pk = np.ascontiguousarray(np.asarray(picks, dtype=np.int32).ravel())
         ang = _host_f32(angles)
         self._call(lib().mexct_osc_iter_dev, self._h, _dptr(mu, "mu", True), W, H,
                    _dptr(pre, "pre", True), _dptr(post, "post", True), n, int(num_subsets),
-                   _p(pk), _p(ang), float(sad), int(geom), None)
+                   _p(pk), _p(ang), float(sad), int(geom), opts(int(storage)))
 
     def tvmin(self, vol, c, steps=20):
         Nz, Ny, Nx = vol.shape
@@ -171,12 +171,12 @@ class Context:
         return out
 
     def osc_tv_recon(self, pre, post, angles, sad, geom, n_iter, tv, s_init, s_final, p,
-                     first_picks, out, tv_steps=20):
+                     first_picks, out, tv_steps=20, storage=F32):
         n, H, W = pre.shape
         ang = _host_f32(angles)
         fp_ = np.ascontiguousarray(np.asarray(first_picks, dtype=np.int32).ravel())
         self._call(lib().mexct_osc_tv_recon_dev, self._h, _dptr(pre, "pre", True),
                    _dptr(post, "post", True), W, H, n, _p(ang), float(sad), int(geom), int(n_iter),
                    float(tv), int(s_init), int(s_final), float(p), _p(fp_),
-                   _dptr(out, "out", True), opts(F32, int(tv_steps)))
+                   _dptr(out, "out", True), opts(int(storage), int(tv_steps)))
         return out

@@ -114,11 +114,13 @@ def CUDAmex_general3D_BP(P, angles, pts, M, N):
     return out
 
 
-def CUDAmex_oscIter(mu, pre, post, numSubsets, subsetPicks, angles, SAD, geomFlag):
+def CUDAmex_oscIter(mu, pre, post, numSubsets, subsetPicks, angles, SAD, geomFlag, storage=F32):
     """CUDAmex_oscIter(mu, pre, post, numSubsets, picks, angles, SAD, geomFlag).
 
     Updates mu, pre and post IN PLACE, as the reference mutates its MATLAB
-    inputs (CUDAmex_oscIter.cu:527-528, :564-570).  picks are int32, 0-based."""
+    inputs (CUDAmex_oscIter.cu:527-528, :564-570).  picks are int32, 0-based.
+    storage F16/BF16 keeps the projected volume copy and the backprojected
+    ratio terms in that type; mu, pre and post stay float32."""
     mu = _single(mu, "mu", writable=True)
     pre = _single(pre, "pre", writable=True)
     post = _single(post, "post", writable=True)
@@ -130,7 +132,7 @@ def CUDAmex_oscIter(mu, pre, post, numSubsets, subsetPicks, angles, SAD, geomFla
     ang = _vec(angles, np.float32, "angles")
     check(lib().mexct_osc_iter(_ptr(mu), W, H, _ptr(pre), _ptr(post), n, int(numSubsets),
                                _ptr(picks), picks.size, _ptr(ang), ang.size,
-                               float(np.float32(SAD)), int(geomFlag), None))
+                               float(np.float32(SAD)), int(geomFlag), opts(int(storage))))
 
 
 def CUDAmex_TVmin_3D(vol, c, steps=20):
@@ -179,7 +181,7 @@ def first_picks(seed, s_init, s_final, p, n_iter):
 
 
 def OSC_TV_recon(pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit, numSubsFinal,
-                 pVal, tv_steps=20, seed=0, picks=None):
+                 pVal, tv_steps=20, seed=0, picks=None, storage=F32):
     """recon = OSC_TV_recon(pre, post, angles, geom, nIter, TV, sInit, sFinal, p)
     (OSC_TV_recon.m:1-102) with all iterations on the device."""
     pre = _single(pre, "pre")
@@ -195,7 +197,7 @@ def OSC_TV_recon(pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit,
                                    float(np.float32(geom["SAD"])), g, int(numIter),
                                    float(np.float32(TV_constant)), int(numSubsInit),
                                    int(numSubsFinal), float(pVal), _ptr(picks), _ptr(mu),
-                                   opts(F32, int(tv_steps))))
+                                   opts(int(storage), int(tv_steps))))
     return mu
 
 

@@ -86,7 +86,7 @@ class Server:
         return self._submit(r, (sino, ang), out)
 
     def submit_osc_tv(self, pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit,
-                      numSubsFinal, pVal, tv_steps=20, seed=0, picks=None) -> int:
+                      numSubsFinal, pVal, tv_steps=20, seed=0, picks=None, storage=F32) -> int:
         """OSC_TV_recon(pre, post, angles, geom, nIter, TV, sInit, sFinal, p)
         as a request."""
         pre = _single(pre, "pre")
@@ -95,7 +95,7 @@ class Server:
         if _dims3(post) != (W, H, n):
             raise MexctError(EINVAL, "pre and post differ in shape")
         g = GEOM[str(geom["type"]).lower()]
-        r, ang = self._base(OP_OSCTV, g, geom["SAD"], scanAngles, W, H, n, F32)
+        r, ang = self._base(OP_OSCTV, g, geom["SAD"], scanAngles, W, H, n, storage)
         if picks is None:
             picks = first_picks(seed, numSubsInit, numSubsFinal, pVal, int(numIter))
         picks = _vec(picks, np.int32, "picks")

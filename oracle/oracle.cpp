@@ -752,7 +752,52 @@ int prefilter(const float* sino, int W, int H, int nProj, int geom, double sad, 
 
 }  // namespace
 
-// ================================================================ C API
+// OSC_TV_recon.m:1-102 in R arithmetic (float: the reference's; double: the
+// FP64 twin the storage forms are measured against).
+template <typename R>
+int osc_tv_recon(const float* pre_in, const float* post_in, int W, int H, int nProj,
+                 const float* angles, float sad, int geom, int nIter, float tv, int sInit,
+                 int sFinal, double p, int tv_steps, const int32_t* first_picks, R* mu_out) {
+  if (nIter < 2) return -1;  // calcNumSubsets divides by (nIter-1)^p
+  const size_t vsz = (size_t)W * W * H, psz = (size_t)W * H * nProj;
+  std::vector<R> mu(vsz, (R)(1.0 * 1e-10));
+  std::vector<R> pre(pre_in, pre_in + psz), post(post_in, post_in + psz);
+  const R epsf = (R)1e-6;
+  for (auto& v : pre) if ((double)v < 1e-6) v = epsf;
+  for (auto& v : post) if ((double)v < 1e-6) v = epsf;
+  std::vector<R> pre2 = pre, post2 = post;
+  pre2[0] = (R)0;
+  post2[0] = (R)0;
+  for (int k = 1; k <= nIter; ++k) {
+    std::vector<R> mu_old = mu;
+    mu_old[0] = (R)0;
+    int n = orc_calc_num_subsets(sInit, sFinal, p, nIter, k);
+    if (n < 1) return -2;
+    std::vector<int32_t> picks(n);
+    orc_calc_subset_order(n, first_picks[k - 1], picks.data());
+    int rc = osc_iter<R>(mu.data(), W, H, pre.data(), post.data(), nProj, n, picks.data(),
+                         angles, sad, geom);
+    if (rc) return rc;
+    pre = pre2;
+    post = post2;
+    if (k > 1) {
+      double sum = 0.0;
+      for (size_t i = 0; i < vsz; ++i) {
+        R dlt = mu_old[i] - mu[i];
+        if (std::isnan(dlt)) dlt = (R)0;
+        sum += (double)dlt * (double)dlt;
+      }
+      R d_a = (R)std::sqrt(sum);
+      tvmin<R>(mu.data(), W, W, H, (R)tv * d_a, tv_steps);
+      for (auto& v : mu)
+        if (std::isnan(v) || std::isinf(v)) v = (R)0;
+    }
+  }
+  std::memcpy(mu_out, mu.data(), sizeof(R) * vsz);
+  return 0;
+}
+
+
 extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
@@ -955,43 +1000,16 @@ int orc_osc_tv_recon(const float* pre_in, const float* post_in, int W, int H, in
                      const float* angles, float sad, int geom, int nIter, float tv, int sInit,
                      int sFinal, double p, int tv_steps, const int32_t* first_picks,
                      float* mu_out) {
-  if (nIter < 2) return -1;  // calcNumSubsets divides by (nIter-1)^p
-  const size_t vsz = (size_t)W * W * H, psz = (size_t)W * H * nProj;
-  std::vector<float> mu(vsz, (float)(1.0 * 1e-10));
-  std::vector<float> pre(pre_in, pre_in + psz), post(post_in, post_in + psz);
-  const float epsf = (float)1e-6;
-  for (auto& v : pre) if ((double)v < 1e-6) v = epsf;
-  for (auto& v : post) if ((double)v < 1e-6) v = epsf;
-  std::vector<float> pre2 = pre, post2 = post;
-  pre2[0] = 0.0f;
-  post2[0] = 0.0f;
-  for (int k = 1; k <= nIter; ++k) {
-    std::vector<float> mu_old = mu;
-    mu_old[0] = 0.0f;
-    int n = orc_calc_num_subsets(sInit, sFinal, p, nIter, k);
-    if (n < 1) return -2;
-    std::vector<int32_t> picks(n);
-    orc_calc_subset_order(n, first_picks[k - 1], picks.data());
-    int rc = osc_iter<float>(mu.data(), W, H, pre.data(), post.data(), nProj, n, picks.data(),
-                             angles, sad, geom);
-    if (rc) return rc;
-    pre = pre2;
-    post = post2;
-    if (k > 1) {
-      double sum = 0.0;
-      for (size_t i = 0; i < vsz; ++i) {
-        float dlt = mu_old[i] - mu[i];
-        if (std::isnan(dlt)) dlt = 0.0f;
-        sum += (double)dlt * (double)dlt;
-      }
-      float d_a = (float)std::sqrt(sum);
-      tvmin<float>(mu.data(), W, W, H, tv * d_a, tv_steps);
-      for (auto& v : mu)
-        if (std::isnan(v) || std::isinf(v)) v = 0.0f;
-    }
-  }
-  std::memcpy(mu_out, mu.data(), sizeof(float) * vsz);
-  return 0;
+  return osc_tv_recon<float>(pre_in, post_in, W, H, nProj, angles, sad, geom, nIter, tv, sInit,
+                             sFinal, p, tv_steps, first_picks, mu_out);
+}
+
+int orc_osc_tv_recon_f64(const float* pre_in, const float* post_in, int W, int H, int nProj,
+                         const float* angles, float sad, int geom, int nIter, float tv,
+                         int sInit, int sFinal, double p, int tv_steps,
+                         const int32_t* first_picks, double* mu_out) {
+  return osc_tv_recon<double>(pre_in, post_in, W, H, nProj, angles, sad, geom, nIter, tv, sInit,
+                              sFinal, p, tv_steps, first_picks, mu_out);
 }
 
 }  // extern "C"

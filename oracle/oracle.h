@@ -120,6 +120,11 @@ int orc_osc_tv_recon(const float* pre, const float* post, int W, int H, int nPro
                      const float* angles, float sad, int geom, int nIter, float tv,
                      int sInit, int sFinal, double p, int tv_steps,
                      const int32_t* first_picks, float* mu_out);
+/* The same driver in FP64 throughout (inputs promoted from float). */
+int orc_osc_tv_recon_f64(const float* pre, const float* post, int W, int H, int nProj,
+                         const float* angles, float sad, int geom, int nIter, float tv,
+                         int sInit, int sFinal, double p, int tv_steps,
+                         const int32_t* first_picks, double* mu_out);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,8 @@ def lib():
             "orc_calc_subset_order": (None, [i, i, _i32p]),
             "orc_osc_tv_recon": (i, [_f32p, _f32p, i, i, i, _f32p, f, i, i, f, i, i, d, i, _i32p,
                                      _f32p]),
+            "orc_osc_tv_recon_f64": (i, [_f32p, _f32p, i, i, i, _f32p, f, i, i, f, i, i, d, i,
+                                         _i32p, _f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -281,15 +283,17 @@ def calc_subset_order(n, first_pick):
 
 
 def osc_tv_recon(pre, post, angles, sad, geom, n_iter, tv, s_init, s_final, p, tv_steps,
-                 first_picks):
+                 first_picks, f64=False):
+    """OSC_TV_recon in float (the reference arithmetic) or, with f64=True, in
+    double throughout."""
     pre = _f32(pre)
     post = _f32(post)
     W, H, n = pre.shape
-    mu = np.zeros((W, W, H), dtype=np.float32, order="F")
+    mu = np.zeros((W, W, H), dtype=np.float64 if f64 else np.float32, order="F")
     fp_ = np.ascontiguousarray(first_picks, dtype=np.int32)
-    rc = lib().orc_osc_tv_recon(pre, post, W, H, n, _angles(angles), float(sad), int(geom),
-                                int(n_iter), float(tv), int(s_init), int(s_final), float(p),
-                                int(tv_steps), fp_, mu)
+    fn = lib().orc_osc_tv_recon_f64 if f64 else lib().orc_osc_tv_recon
+    rc = fn(pre, post, W, H, n, _angles(angles), float(sad), int(geom), int(n_iter), float(tv),
+            int(s_init), int(s_final), float(p), int(tv_steps), fp_, mu)
     if rc:
         raise ValueError(f"orc_osc_tv_recon failed ({rc})")
     return mu

@@ -223,3 +223,54 @@ def test_storage_modes(gpu, storage):
     e = np.linalg.norm(bp - bp64) / np.linalg.norm(bp64)
     print(f"BP storage={storage} relRMSE vs FP64 = {e:.3e}")
     assert e <= STORAGE_BOUND[storage]
+
+
+# OSC-TV of the few-view cone workload (12 ellipsoids in a cylinder, Poisson
+# transmission at I0=1e5, 10 subsets, 20 iterations, 10 TV steps, first-pick
+# seed 303) at 64^3 with 60 views, against the FP64 oracle.  FP16/BF16
+# storage keeps the projected volume copy and the backprojected ratio terms
+# in that type; the bounds are those of the single FP and BP.
+_OSC64 = {}
+
+
+def _osc_case_64():
+    if not _OSC64:
+        from mexct import phantoms
+        W, n = 64, 60
+        vol = phantoms.osc_phantom(W, seed=3)
+        ang = phantoms.angles_uniform(n, 2 * np.pi)
+        fp = O.fp(vol, 1000.0, ang, n, 2)
+        pre, post = phantoms.poisson_transmission(fp, 1e5, seed=33)
+        picks = mexct.first_picks(303, 10, 10, 1.0, 20)
+        ref = O.osc_tv_recon(pre, post, ang, 1000.0, 2, 20, 0.1, 10, 10, 1.0, 10, picks, f64=True)
+        _OSC64.update(pre=pre, post=post, ang=ang, picks=picks, ref=ref)
+    return _OSC64
+
+
+@pytest.mark.parametrize("storage", [mexct.F32, mexct.F16, mexct.BF16])
+def test_osc_tv_storage_vs_fp64(gpu, storage):
+    c = _osc_case_64()
+    got = mexct.OSC_TV_recon(c["pre"], c["post"], c["ang"], {"type": "cone3d", "SAD": 1000.0}, 20,
+                             0.1, 10, 10, 1.0, tv_steps=10, picks=c["picks"], storage=storage)
+    ref = c["ref"]
+    e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"OSC-TV storage={storage} relRMSE vs FP64 = {e:.3e}")
+    assert e <= {mexct.F32: 1e-4}.get(storage, STORAGE_BOUND.get(storage))
+
+
+@pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
+def test_osc_iter_storage_vs_fp64(gpu, storage):
+    from mexct import phantoms
+    W, H, n = 48, 32, 24
+    vol = phantoms.osc_phantom(W, seed=5)[:, :, :H].copy(order="F")
+    ang = _angles(n)
+    fp = O.fp(vol, 500.0, ang, n, 2)
+    pre, post = phantoms.poisson_transmission(fp, 1e5, seed=7)
+    mu0 = np.asfortranarray(np.full((W, W, H), 0.015, np.float32))
+    picks = np.array([2, 0, 3, 1], np.int32)
+    mu, pr, po = mu0.copy(order="F"), pre.copy(order="F"), post.copy(order="F")
+    mexct.CUDAmex_oscIter(mu, pr, po, 4, picks, ang, 500.0, 2, storage=storage)
+    mu64, _, _ = O.osc_iter(mu0, pre, post, 4, picks, ang, 500.0, 2, f64=True)
+    e = np.linalg.norm(mu - mu64) / np.linalg.norm(mu64)
+    print(f"oscIter storage={storage} relRMSE vs FP64 = {e:.3e}")
+    assert e <= STORAGE_BOUND[storage]

@@ -293,6 +293,20 @@ def test_osc_f32_close_to_f64_twin():
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
 
 
+def test_osc_tv_recon_f32_close_to_f64_twin():
+    # the FP64 driver is the same template as the float one: a term dropped
+    # or mistyped in either instantiation separates them far beyond rounding
+    W, H, n = 16, 8, 12
+    mu, pre, ang = _osc_setup(W, H, n)
+    post = f32(pre * rng(21).uniform(0.3, 0.9, size=pre.shape))
+    picks = np.array([1, 2, 1], np.int32)
+    a = O.osc_tv_recon(pre, post, ang, 100.0, 2, 3, 0.1, 4, 2, 1.0, 5, picks)
+    b = O.osc_tv_recon(pre, post, ang, 100.0, 2, 3, 0.1, 4, 2, 1.0, 5, picks, f64=True)
+    assert b.dtype == np.float64
+    e = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert 0 < e < 1e-4
+
+
 # ------------------------------------------------------------------ TV
 def test_tv_gradient_hand_computed_peak():
     # a single unit peak along x: g = 2/(1+eps) at the peak, -1/(1+eps) beside it
