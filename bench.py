@@ -112,16 +112,18 @@ def main():
     filt32 = filt.clone()
     rec32 = rec.clone()
 
-    # Measured gather roofline: the probe issues the same pair-load pattern
-    # with ideal coherence (4 loads/sample for FP, 2 per update for BP).
-    import oracle as O  # checker: sample-count estimate for the FP gather bytes
+    # Measured gather roofline: the probe issues the projectors' own load
+    # instructions with ideal coherence (FP: quad-volume loads per sample,
+    # BP: one row-paired load per update).  Rates are compared as texels
+    # gathered per second: 8 per FP sample, 4 per BP update.
+    import oracle as O  # checker: sample-count estimate for the FP gather rate
 
-    def probe_ms(nloads, storage, reps=3):
-        ctx.probe_gather(storage, W, H, n, nloads)  # warm
+    def probe_ms(pattern, storage, reps=3):
+        ctx.probe_gather(storage, W, H, n, pattern)  # warm
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            ns = ctx.probe_gather(storage, W, H, n, nloads)
+            ns = ctx.probe_gather(storage, W, H, n, pattern)
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps, ns
@@ -132,22 +134,32 @@ def main():
     hw = W // 2 - 1
     gx, gy = np.meshgrid(np.arange(W), np.arange(W), indexing="ij")
     U_bp = float(((gx - hw) ** 2 + (gy - hw) ** 2 < hw * hw).sum()) * H * n
-    pf_ms, pf_n = probe_ms(4, mexct.F32)
-    pb_ms, pb_n = probe_ms(2, mexct.F32)
-    fp_gather = S_est * 8 * 4 / (fp_ms * 1e-3) / 1e9
-    bp_gather = U_bp * 4 * 4 / (bp_ms * 1e-3) / 1e9
-    pf_gbs = pf_n * 8 * 4 / (pf_ms * 1e-3) / 1e9
-    pb_gbs = pb_n * 4 * 4 / (pb_ms * 1e-3) / 1e9
-    gather = {
-        "fp": {"achieved_GBs": round(fp_gather, 1), "measured_peak_GBs": round(pf_gbs, 1),
-               "frac": round(fp_gather / pf_gbs, 4), "samples": S_est,
-               "samples_note": "ray-march samples, mean of 8 views x views (oracle count)"},
-        "bp": {"achieved_GBs": round(bp_gather, 1), "measured_peak_GBs": round(pb_gbs, 1),
-               "frac": round(bp_gather / pb_gbs, 4), "updates_in_circle": U_bp},
-        "unit": "GB/s of gathered texels (8 taps x 4 B per FP sample, 4 x 4 B per BP update)",
-    }
-    log(f"[bench] gather roofline: FP {fp_gather:.0f}/{pf_gbs:.0f} GB/s ({fp_gather / pf_gbs:.2f}), "
-        f"BP {bp_gather:.0f}/{pb_gbs:.0f} GB/s ({bp_gather / pb_gbs:.2f})")
+
+    def gather_fracs(storage, t_fp, t_bp):
+        es = 4 if storage == mexct.F32 else 2
+        pf_ms, pf_n = probe_ms(0, storage)
+        pb_ms, pb_n = probe_ms(1, storage)
+        fp_a = S_est * 8 * es / (t_fp * 1e-3) / 1e9
+        bp_a = U_bp * 4 * es / (t_bp * 1e-3) / 1e9
+        fp_p = pf_n * 8 * es / (pf_ms * 1e-3) / 1e9
+        bp_p = pb_n * 4 * es / (pb_ms * 1e-3) / 1e9
+        return {
+            "fp": {"achieved_GBs": round(fp_a, 1), "measured_peak_GBs": round(fp_p, 1),
+                   "frac": round(fp_a / fp_p, 4), "probe_ms": round(pf_ms, 3)},
+            "bp": {"achieved_GBs": round(bp_a, 1), "measured_peak_GBs": round(bp_p, 1),
+                   "frac": round(bp_a / bp_p, 4), "probe_ms": round(pb_ms, 3)},
+        }
+
+    gather = gather_fracs(mexct.F32, fp_ms, bp_ms)
+    gather.update({"fp_samples": S_est,
+                   "fp_samples_note": "ray-march samples, mean of 8 views x views (oracle count)",
+                   "bp_updates_in_circle": U_bp,
+                   "unit": "GB/s of gathered texels (8 per FP sample, 4 per BP update, "
+                           "storage bytes each)"})
+    log(f"[bench] gather roofline: FP {gather['fp']['achieved_GBs']:.0f}/"
+        f"{gather['fp']['measured_peak_GBs']:.0f} GB/s ({gather['fp']['frac']:.2f}), "
+        f"BP {gather['bp']['achieved_GBs']:.0f}/{gather['bp']['measured_peak_GBs']:.0f} GB/s "
+        f"({gather['bp']['frac']:.2f})")
 
     # FP roofline: compulsory HBM bytes of a view-by-view projector, the
     # volume read once per view plus the projections written once.
@@ -190,7 +202,9 @@ def main():
         rec32n = rec32.cpu().numpy()[:, cy, cx].T
         bb16 = float(np.linalg.norm(rec16 - refb) / np.linalg.norm(refb))
         bb32 = float(np.linalg.norm(rec32n - refb) / np.linalg.norm(refb))
-        f16 = {"fp_ms": round(f16_fp_ms, 3), "bp_ms": round(f16_bp_ms, 3),
+        gr16 = gather_fracs(mexct.F16, f16_fp_ms, f16_bp_ms)
+        log(f"[bench] fp16 gather roofline: FP {gr16['fp']['frac']:.2f}, BP {gr16['bp']['frac']:.2f}")
+        f16 = {"fp_ms": round(f16_fp_ms, 3), "bp_ms": round(f16_bp_ms, 3), "gather_roofline": gr16,
                "fp_gups": round(U / (f16_fp_ms * 1e-3) / 1e9, 1),
                "bp_gups": round(U / (f16_bp_ms * 1e-3) / 1e9, 1),
                "fp_speedup_vs_f32": round(fp_ms / f16_fp_ms, 3),

@@ -176,14 +176,18 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
 /* ------------------------------------------------------------------ measurement
  * No reference counterpart: the gather-throughput microbenchmark behind the
  * projectors' "measured gather roofline".  Launches, on the context's stream,
- * a kernel that issues the forward projector's load pattern (nloads = 4:
- * four z-pair loads per sample) or the backprojector's (nloads = 2) with
- * ideal coherence over a z-fastest haloed [W,W,H] volume of the given
- * storage dtype held in the context's workspace (its contents are whatever
- * the last projection left there).  One sample per lane per step, W steps,
- * W*H*nviews lanes; *samples receives the total sample count. */
+ * a kernel that issues the load instructions of the forward projector
+ * (MEXCT_PROBE_FP: per sample two 16-byte loads in f32, one in f16/bf16, of
+ * the corner-quad volume) or of the backprojector (MEXCT_PROBE_BP: per
+ * update one 16-byte / 8-byte load of the row-paired sinogram) with ideal
+ * coherence, over a buffer of the projector's size for W, H, nviews held in
+ * the context's workspace (its contents are whatever the last projection
+ * left there).  *samples receives the number of samples (FP) or updates
+ * (BP) the launch performs. */
+#define MEXCT_PROBE_FP 0
+#define MEXCT_PROBE_BP 1
 int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H, int32_t nviews,
-                           int32_t nloads, int64_t* samples);
+                           int32_t pattern, int64_t* samples);
 
 /* ------------------------------------------------------------------ request server
  * No reference counterpart (each MEX call owns the GPU while it runs).  A

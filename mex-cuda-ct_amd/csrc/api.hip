@@ -388,16 +388,24 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   DevGuard g(c->device);
   void* pad;
   const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st);
-  RC(c->get(WS_VOL, pbytes, &pad));
+  const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * store_bytes(st);
+  const int quad = qbytes < ((size_t)1 << 32);
   const float2* sc;
-  if (pbytes < ((size_t)1 << 31)) {
-    // z-fastest haloed volume, one wave per 64 detector rows of a column
-    HIPCHK(launch_pad3z(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
-                        c->stream));
+  if (quad || pbytes < ((size_t)1 << 31)) {
+    // z-fastest haloed volume (corner quads when they fit 32-bit offsets),
+    // one wave per 64 detector rows of a column
+    RC(c->get(WS_VOL, quad ? qbytes : pbytes, &pad));
+    if (quad)
+      HIPCHK(launch_pad3q(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
+                          c->stream));
+    else
+      HIPCHK(launch_pad3z(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
+                          c->stream));
     RC(c->sincos_table(angles, nProj, &sc));
     FPZLaunch z;
     z.vol = pad;
     z.st_dt = st;
+    z.quad = quad;
     z.out = out;
     z.sc = sc;
     z.geom = geom;
@@ -408,6 +416,7 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
     HIPCHK(launch_fpz(z, c->stream));
     return MEXCT_OK;
   }
+  RC(c->get(WS_VOL, pbytes, &pad));
   HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
   RC(c->sincos_table(angles, nProj, &sc));
   FPLaunch a;
@@ -464,13 +473,15 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
   RC(check_padded(W, H, nProj, "sinogram"));
-  if ((double)(W + 4) * (H + 4) * nProj * store_bytes(st) >= 2147483648.0)
+  // row-paired haloed copy: (W+4) x (H+3) elements of two values per view
+  const size_t rp_bytes = (size_t)(W + 4) * (H + 3) * 2 * store_bytes(st);
+  if ((double)rp_bytes * nProj >= 2147483648.0)
     return fail(MEXCT_EINVAL, "sinogram too large for one call (%lld x %lld x %lld)",
                 (long long)W, (long long)H, (long long)nProj);
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
   DevGuard g(c->device);
   void* pad;
-  RC(c->get(WS_SINO, (size_t)(W + 4) * (H + 4) * std::max<int64_t>(nProj, 1) * store_bytes(st), &pad));
+  RC(c->get(WS_SINO, rp_bytes * std::max<int64_t>(nProj, 1), &pad));
   if (nProj > 0)
     HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream));
   const float2* sc = nullptr;
@@ -654,7 +665,9 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   HIPCHK(launch_fill((float*)qp, 0.f, pwords, c->stream));
   void* mz;
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
-  RC(c->get(WS_MUZ, (size_t)MPXY * (H + 4) * es, &mz));
+  const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
+  const int quad = qbytes < ((size_t)1 << 32);
+  RC(c->get(WS_MUZ, quad ? qbytes : (size_t)MPXY * (H + 4) * es, &mz));
   uint32_t* smax = nullptr;
   if (st == MEXCT_F16) {
     void* m;
@@ -670,11 +683,16 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
     const int count = (sel >= 0 && sel < nProj) ? (nProj - 1 - sel) / numSubsets + 1 : 0;
     if (count > 0) {
       // the subset projects the volume as left by the previous update
-      HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
-                          c->stream));
+      if (quad)
+        HIPCHK(launch_pad3q(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
+                            c->stream));
+      else
+        HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
+                            c->stream));
       OSCFPLaunch f;
       f.mu_z = mz;
       f.st_dt = st;
+      f.quad = quad;
       f.pre = pre;
       f.post = post;
       f.pre_pad = pp;
@@ -897,22 +915,23 @@ int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int3
 
 // ------------------------------------------------------------------ measurement
 int mexct_probe_gather_dev(mexct_ctx* c, int32_t storage, int64_t W, int64_t H, int32_t nviews,
-                           int32_t nloads, int64_t* samples) {
+                           int32_t pattern, int64_t* samples) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_dtype(storage, "storage"));
-  if (W < 2 || H < 1 || nviews < 1 || (nloads != 2 && nloads != 4))
+  if (W < 2 || H < 1 || nviews < 1 || (pattern != MEXCT_PROBE_FP && pattern != MEXCT_PROBE_BP) ||
+      !samples)
     return fail(MEXCT_EINVAL, "bad probe arguments");
-  const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(storage);
-  if (pbytes >= ((size_t)1 << 31)) return fail(MEXCT_EINVAL, "probe volume too large");
+  const size_t es = (size_t)store_bytes(storage);
+  const size_t bytes = pattern == MEXCT_PROBE_FP
+                           ? (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es
+                           : (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews;
+  if (bytes >= ((size_t)1 << 32)) return fail(MEXCT_EINVAL, "probe buffer too large");
   DevGuard g(c->device);
-  void *vol, *sink;
-  RC(c->get(WS_VOL, pbytes, &vol));
+  void *buf, *sink;
+  RC(c->get(pattern == MEXCT_PROBE_FP ? WS_VOL : WS_SINO, bytes, &buf));
   RC(c->get(WS_SCALARS, 4096, &sink));
-  const int bx = (int)((W + 3) / 4);
-  const int by = (int)((H + 63) / 64) * nviews;
-  HIPCHK(launch_gather_probe(vol, storage, (int)W, (int)W, (int)H, bx, by, (int)W, nloads,
-                             (float*)sink, c->stream));
-  if (samples) *samples = (int64_t)bx * 4 * (int64_t)by * 64 * W;
+  HIPCHK(launch_gather_probe(buf, storage, pattern, (int)W, (int)H, nviews, (float*)sink, samples,
+                             c->stream));
   return MEXCT_OK;
 }
 

@@ -20,36 +20,40 @@
 
 namespace mexct {
 
+// Four taps of a row-paired sinogram element pair: {(j0,i0), (j0+1,i0),
+// (j0,i0+1), (j0+1,i0+1)} in one load.
 template <typename S>
-__device__ __forceinline__ float2 bld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+__device__ __forceinline__ float4 bld4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
 template <>
-__device__ __forceinline__ float2 bld2<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+__device__ __forceinline__ float4 bld4<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                               uint32_t soff) {
-  typedef float f2v __attribute__((ext_vector_type(2)));
-  f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-  return make_float2(v.x, v.y);
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  f4v v = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  return make_float4(v.x, v.y, v.z, v.w);
 }
 template <>
-__device__ __forceinline__ float2 bld2<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+__device__ __forceinline__ float4 bld4<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                  uint32_t soff) {
-  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-  h2v v = __builtin_bit_cast(h2v, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-  return make_float2((float)v.x, (float)v.y);
+  typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+  h4v v = __builtin_bit_cast(h4v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
 }
 template <>
-__device__ __forceinline__ float2 bld2<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+__device__ __forceinline__ float4 bld4<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                uint32_t soff) {
-  const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+  typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+  const u2v u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                     __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
 }
 
 struct BPKArgs {
-  const void* sino;
+  const void* sino;     // row-paired haloed sinogram
   uint32_t sino_bytes;  // buffer range (the launcher guarantees < 2^31)
   float* out;
   const float2* sc;
-  int W, H, nviews, PW;
-  int64_t PV;
+  int W, H, nviews, PW;  // PW: elements (pairs) per row
+  int64_t PV;            // pairs per view
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
 };
@@ -113,8 +117,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const float y = (float)iy - ((float)hw + 0.5f);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.sino, 0, (int)a.sino_bytes, 0x00020000);
-  const uint32_t es = (uint32_t)sizeof(S);
-  const int row_bytes = a.PW * (int)es;  // one detector row, also the (j+1) soffset
+  const uint32_t es = 2u * (uint32_t)sizeof(S);  // bytes of one row-paired element
+  const int row_bytes = a.PW * (int)es;          // one detector row of pairs
   float acc[ZR];
 #pragma unroll
   for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
@@ -131,18 +135,18 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
     // view v's slice starts at a wave-uniform scalar offset; the lane's
     // byte offset of (u = i0, v = j0) is (i0+2)*es + (j0+2)*row_bytes, an
     // integer below 2^24 and therefore exact in float
-    const uint32_t soff0 = (uint32_t)v * (uint32_t)a.PV * es;
-    const uint32_t soff1 = soff0 + (uint32_t)row_bytes;
+    const uint32_t soff = (uint32_t)v * (uint32_t)a.PV * es;
     const float colf = (fi0 + (float)HALO) * (float)es + (float)(HALO * row_bytes);
     const float rbf = (float)row_bytes;
+    const uint32_t colb = (uint32_t)((int)fi0 + HALO) * es;  // integer form (large slices)
     pz = advance_rows<GEOM>(pz, dz, z0);
     // groups of 8 z-voxels: offsets and loads of the whole group are issued
-    // before its interpolation, so 16 loads per lane are in flight
+    // before its interpolation
     constexpr int G = 8;
 #pragma unroll
     for (int m0 = 0; m0 < ZR; m0 += G) {
       float az[G];
-      float2 q0[G], q1[G];
+      float4 q[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const float zb = pz - 0.5f;
@@ -151,15 +155,14 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
         const float fj0 = __builtin_amdgcn_fmed3f(fz, -2.f, (float)a.H);
         const uint32_t voff =
             SMALL ? (uint32_t)__builtin_fmaf(fj0, rbf, colf)
-                  : (uint32_t)((int)fj0 + HALO) * (uint32_t)row_bytes + (uint32_t)colf;
-        q0[g] = bld2<S>(r, voff, soff0);
-        q1[g] = bld2<S>(r, voff, soff1);
+                  : (uint32_t)((int)fj0 + HALO) * (uint32_t)row_bytes + colb;
+        q[g] = bld4<S>(r, voff, soff);
         pz = pz + dz;
       }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const float c0 = lerp(q0[g].x, q0[g].y, ax);
-        const float c1 = lerp(q1[g].x, q1[g].y, ax);
+        const float c0 = lerp(q[g].x, q[g].z, ax);  // row j0: columns i0, i0+1
+        const float c1 = lerp(q[g].y, q[g].w, ax);  // row j0+1
         acc[m0 + g] = acc[m0 + g] + UU * lerp(c0, c1, az[g]);
       }
     }
@@ -183,7 +186,7 @@ static void bp_dispatch_g(const BPKArgs& k, int geom, hipStream_t st) {
 
 template <int ZR, typename S>
 static void bp_dispatch(const BPKArgs& k, int geom, hipStream_t st) {
-  if ((uint64_t)k.PV * sizeof(S) < (1u << 24))
+  if ((uint64_t)k.PV * 2u * sizeof(S) < (1u << 24))
     bp_dispatch_g<ZR, S, true>(k, geom, st);
   else
     bp_dispatch_g<ZR, S, false>(k, geom, st);
@@ -201,7 +204,7 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   const int zr = a.zr == 64 ? 64 : 32;
   BPKArgs k;
   k.sino = a.sino;
-  k.sino_bytes = (uint32_t)((size_t)(a.W + 2 * HALO) * (a.H + 2 * HALO) * a.nviews *
+  k.sino_bytes = (uint32_t)((size_t)(a.W + 2 * HALO) * (a.H + 2 * HALO - 1) * a.nviews * 2u *
                             (a.st_dt == 0 ? 4u : 2u));
   k.out = a.out;
   k.sc = a.sc;
@@ -209,7 +212,7 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.H = a.H;
   k.nviews = a.nviews;
   k.PW = a.W + 2 * HALO;
-  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO);
+  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO - 1);
   k.v0 = float(a.H - 1) / 2;
   k.SR = a.sad;
   k.tiles_x = (uint32_t)((a.W + 7) / 8);

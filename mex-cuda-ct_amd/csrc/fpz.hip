@@ -9,8 +9,15 @@
 // at each step the 64 lanes read the (k, k+1) pairs of nearly the same xy
 // corners, one or two cache lines per load, where an x-fastest volume costs
 // about one line per lane.  A 256-thread workgroup holds 4 adjacent
-// columns.  Loads are 32-bit buffer offsets with the (j+1), (i+1) and
-// (i+1, j+1) corner strides in scalar offsets.
+// columns.  Loads are 32-bit buffer offsets.
+//
+// Two z-fastest layouts.  QUAD (the default): element (x,y,z) holds the four
+// xy corners {V(x,y,z), V(x,y+1,z), V(x+1,y,z), V(x+1,y+1,z)}, so the eight
+// taps of a sample are two consecutive elements: two 16-byte loads in f32,
+// one in f16/bf16, instead of four pair loads.  It costs 4x the memory of
+// the plain layout, which MI355X's HBM holds easily; the plain layout
+// (pair loads at the (j+1), (i+1), (i+1,j+1) strides in scalar offsets)
+// serves volumes whose quad copy would pass 4 GiB.
 //
 // Arithmetic follows the oracle operation by operation (ray setup and clip
 // in ray.h; sample coordinate p - b, then - 0.5; lerps x, then y, then z),
@@ -50,29 +57,87 @@ __device__ __forceinline__ float2 buf_ld2<float>(__amdgpu_buffer_rsrc_t r, uint3
   return make_float2(v.x, v.y);
 }
 template <>
-__device__ __forceinline__ float2 buf_ld2<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                                    uint32_t soff) {
-  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-  h2v v = __builtin_bit_cast(h2v, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-  return make_float2((float)v.x, (float)v.y);
-}
-template <>
 __device__ __forceinline__ float2 buf_ld2<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                   uint32_t soff) {
   const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
   return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
 }
 
-// Corner pairs and weights of one sample.
+// Corner pairs and weights of one sample (f32 and bf16 taps, widened).
+template <typename S>
 struct Tap8 {
   float2 p00, p01, p10, p11;
   float ax, ay, az;
 };
+// f16 taps stay packed, two halves per dword: the first lerp level reads
+// them through mixed-precision FMAs.
+//   plain layout: w0..w3 = (z, z+1) pairs of (i,j), (i,j+1), (i+1,j), (i+1,j+1)
+//   quad layout:  w0 = {(i,j), (i,j+1)} and w1 = {(i+1,j), (i+1,j+1)} at z,
+//                 w2, w3 the same at z+1
+template <>
+struct Tap8<_Float16> {
+  uint32_t w0, w1, w2, w3;
+  float ax, ay, az;
+};
 
+// lerp(a, b, t) of the f16 halves HI of A (a) and B (b), in float:
+// fma(b, 1, -a) is exactly the difference of the widened halves and
+// fma(t, d, a) the lerp, so the result equals lerp((float)a, (float)b, t).
+template <int HI>
+__device__ __forceinline__ float lerp_h(uint32_t A, uint32_t B, float t) {
+  float d, c;
+  if (HI) {
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
+  } else {
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
+  }
+  return c;
+}
+
+// Eight taps of one quad-layout sample: element e holds the xy corners at
+// z = k, element e+1 at z = k+1.
 template <typename S>
-__device__ __forceinline__ Tap8 fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, const R3& cur,
-                                        float fnx, float fny, float fnz, int base) {
-  Tap8 t;
+__device__ __forceinline__ void quad_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, Tap8<S>& t);
+template <>
+__device__ __forceinline__ void quad_ld<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                               Tap8<float>& t) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v q0 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+  const f4v q1 = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16u, 0, 0));
+  t.p00 = make_float2(q0.x, q1.x);
+  t.p01 = make_float2(q0.y, q1.y);
+  t.p10 = make_float2(q0.z, q1.z);
+  t.p11 = make_float2(q0.w, q1.w);
+}
+template <>
+__device__ __forceinline__ void quad_ld<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                  Tap8<_Float16>& t) {
+  typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+  const u4v q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+  t.w0 = q.x;
+  t.w1 = q.y;
+  t.w2 = q.z;
+  t.w3 = q.w;
+}
+template <>
+__device__ __forceinline__ void quad_ld<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                Tap8<bf16_t>& t) {
+  typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+  const u4v q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+  // q.x = {(i,j), (i,j+1)}, q.y = {(i+1,j), (i+1,j+1)} at k; q.z, q.w at k+1
+  t.p00 = make_float2(__uint_as_float(q.x << 16), __uint_as_float(q.z << 16));
+  t.p01 = make_float2(__uint_as_float(q.x & 0xffff0000u), __uint_as_float(q.z & 0xffff0000u));
+  t.p10 = make_float2(__uint_as_float(q.y << 16), __uint_as_float(q.w << 16));
+  t.p11 = make_float2(__uint_as_float(q.y & 0xffff0000u), __uint_as_float(q.w & 0xffff0000u));
+}
+
+template <typename S, int QUAD>
+__device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a,
+                                           const R3& cur, float fnx, float fny, float fnz,
+                                           int base) {
+  Tap8<S> t;
   const float xb = (cur.x - a.bx) - 0.5f;
   const float yb = (cur.y - a.by) - 0.5f;
   const float zb = (cur.z - a.bz) - 0.5f;
@@ -85,19 +150,46 @@ __device__ __forceinline__ Tap8 fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs&
   const int kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
   // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
   const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
-  const uint32_t voff = e * (uint32_t)sizeof(S);
-  t.p00 = buf_ld2<S>(r, voff, 0);         // (i,   j  ): z k, k+1
-  t.p01 = buf_ld2<S>(r, voff, a.off_j);   // (i,   j+1)
-  t.p10 = buf_ld2<S>(r, voff, a.off_i);   // (i+1, j  )
-  t.p11 = buf_ld2<S>(r, voff, a.off_ij);  // (i+1, j+1)
+  if (QUAD) {
+    quad_ld<S>(r, e * (uint32_t)(4 * sizeof(S)), t);
+  } else {
+    const uint32_t voff = e * (uint32_t)sizeof(S);
+    if constexpr (__is_same(S, _Float16)) {
+      t.w0 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
+      t.w1 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, a.off_j, 0);
+      t.w2 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, a.off_i, 0);
+      t.w3 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, a.off_ij, 0);
+    } else {
+      t.p00 = buf_ld2<S>(r, voff, 0);         // (i,   j  ): z k, k+1
+      t.p01 = buf_ld2<S>(r, voff, a.off_j);   // (i,   j+1)
+      t.p10 = buf_ld2<S>(r, voff, a.off_i);   // (i+1, j  )
+      t.p11 = buf_ld2<S>(r, voff, a.off_ij);  // (i+1, j+1)
+    }
+  }
   return t;
 }
 
-__device__ __forceinline__ float interp_z(const Tap8& t) {
-  const float c00 = lerp(t.p00.x, t.p10.x, t.ax);
-  const float c10 = lerp(t.p01.x, t.p11.x, t.ax);
-  const float c01 = lerp(t.p00.y, t.p10.y, t.ax);
-  const float c11 = lerp(t.p01.y, t.p11.y, t.ax);
+template <typename S, int QUAD>
+__device__ __forceinline__ float interp_z(const Tap8<S>& t) {
+  float c00, c10, c01, c11;  // c_{y,z}: lerps along x
+  if constexpr (__is_same(S, _Float16)) {
+    if (QUAD) {
+      c00 = lerp_h<0>(t.w0, t.w1, t.ax);
+      c10 = lerp_h<1>(t.w0, t.w1, t.ax);
+      c01 = lerp_h<0>(t.w2, t.w3, t.ax);
+      c11 = lerp_h<1>(t.w2, t.w3, t.ax);
+    } else {
+      c00 = lerp_h<0>(t.w0, t.w2, t.ax);
+      c10 = lerp_h<0>(t.w1, t.w3, t.ax);
+      c01 = lerp_h<1>(t.w0, t.w2, t.ax);
+      c11 = lerp_h<1>(t.w1, t.w3, t.ax);
+    }
+  } else {
+    c00 = lerp(t.p00.x, t.p10.x, t.ax);
+    c10 = lerp(t.p01.x, t.p11.x, t.ax);
+    c01 = lerp(t.p00.y, t.p10.y, t.ax);
+    c11 = lerp(t.p01.y, t.p11.y, t.ax);
+  }
   const float c0 = lerp(c00, c10, t.ay);
   const float c1 = lerp(c01, c11, t.ay);
   return lerp(c0, c1, t.az);
@@ -106,7 +198,7 @@ __device__ __forceinline__ float interp_z(const Tap8& t) {
 // One ray through the z-fastest volume, software-pipelined: the loads of
 // sample k+1 are issued before sample k is interpolated.  A fetch past the
 // last sample reads a clamped, in-range corner and is discarded.
-template <typename S, int STEP2>
+template <typename S, int STEP2, int QUAD>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
@@ -116,21 +208,21 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
   if (n <= 0) return dval;
   // two register sets in ping-pong: while one sample is interpolated the
   // next sample's four loads are in flight
-  Tap8 t0 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
+  Tap8<S> t0 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
   int k = 0;
   for (; k + 1 < n; k += 2) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    const Tap8 t1 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
-    dval = dval + interp_z(t0) * step;
+    const Tap8<S> t1 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    dval = dval + interp_z<S, QUAD>(t0) * step;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    t0 = fetch_z<S>(r, a, cur, fnx, fny, fnz, base);
-    dval = dval + interp_z(t1) * step;
+    t0 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    dval = dval + interp_z<S, QUAD>(t1) * step;
   }
-  if (k < n) dval = dval + interp_z(t0) * step;
+  if (k < n) dval = dval + interp_z<S, QUAD>(t0) * step;
   return dval;
 }
 
@@ -145,7 +237,7 @@ __device__ __forceinline__ float scale_pow2(float v, int e) {
 
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
 // written in place, plus zero-haloed copies for the dual backprojection.
-template <int GEOM, typename S, int EPI>
+template <int GEOM, typename S, int EPI, int QUAD>
 __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
@@ -186,8 +278,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   R3 cur, inc;
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
   const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, 0, (int)a.vol_bytes, 0x00020000);
-  const float dval = march_z<S, STEP2>(r, a, cur, inc, n);
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
+  const float dval = march_z<S, STEP2, QUAD>(r, a, cur, inc, n);
   if (EPI == 0) {
     a.out[p] = dval;
   } else {
@@ -200,8 +292,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   }
 }
 
-static FPZArgs make_zargs(const void* vol, int st_dt, const float2* sc, int W, int H, int nviews,
-                          float sad) {
+static FPZArgs make_zargs(const void* vol, int st_dt, int quad, const float2* sc, int W, int H,
+                          int nviews, float sad) {
   FPZArgs k{};
   k.vol = vol;
   k.sc = sc;
@@ -212,10 +304,11 @@ static FPZArgs make_zargs(const void* vol, int st_dt, const float2* sc, int W, i
   k.NX = W;
   k.NY = W;
   k.NZ = H;
-  k.PY = W + 2 * HALO;
+  // quad: corner elements for x, y in [-2, N] (N+3 of them); plain: [-2, N+1]
+  k.PY = quad ? W + 2 * HALO - 1 : W + 2 * HALO;
   k.PZ = H + 2 * HALO;
-  const uint32_t es = st_dt == 0 ? 4u : 2u;
-  k.vol_bytes = (uint32_t)((size_t)(W + 2 * HALO) * k.PY * k.PZ * es);
+  const uint32_t es = (st_dt == 0 ? 4u : 2u) * (quad ? 4u : 1u);
+  k.vol_bytes = (uint32_t)((size_t)k.PY * k.PY * k.PZ * es);
   k.off_j = (uint32_t)k.PZ * es;
   k.off_i = (uint32_t)k.PY * (uint32_t)k.PZ * es;
   k.off_ij = k.off_i + k.off_j;
@@ -234,31 +327,39 @@ static FPZArgs make_zargs(const void* vol, int st_dt, const float2* sc, int W, i
   return k;
 }
 
-template <int EPI, typename S>
-static void fpz_dispatch(const FPZArgs& k, int geom, hipStream_t st) {
+template <int EPI, typename S, int QUAD>
+static void fpz_dispatch_q(const FPZArgs& k, int geom, hipStream_t st) {
   const unsigned g = (k.n_logical + 7u) & ~7u;
   switch (geom) {
-    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI><<<g, 256, 0, st>>>(k); break;
-    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI><<<g, 256, 0, st>>>(k); break;
-    default: fpz_kernel<GEOM_CONE, S, EPI><<<g, 256, 0, st>>>(k); break;
+    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
+    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
+    default: fpz_kernel<GEOM_CONE, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
   }
 }
 
+template <int EPI, typename S>
+static void fpz_dispatch(const FPZArgs& k, int geom, int quad, hipStream_t st) {
+  if (quad)
+    fpz_dispatch_q<EPI, S, 1>(k, geom, st);
+  else
+    fpz_dispatch_q<EPI, S, 0>(k, geom, st);
+}
+
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.vol, a.st_dt, a.sc, a.W, a.H, a.nviews, a.sad);
+  FPZArgs k = make_zargs(a.vol, a.st_dt, a.quad, a.sc, a.W, a.H, a.nviews, a.sad);
   k.out = a.out;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
-    fpz_dispatch<0, float>(k, a.geom, st);
+    fpz_dispatch<0, float>(k, a.geom, a.quad, st);
   else if (a.st_dt == 1)
-    fpz_dispatch<0, _Float16>(k, a.geom, st);
+    fpz_dispatch<0, _Float16>(k, a.geom, a.quad, st);
   else
-    fpz_dispatch<0, bf16_t>(k, a.geom, st);
+    fpz_dispatch<0, bf16_t>(k, a.geom, a.quad, st);
   return hipGetLastError();
 }
 
 hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.sc, a.W, a.H, a.count, a.sad);
+  FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.quad, a.sc, a.W, a.H, a.count, a.sad);
   k.pre = a.pre;
   k.post = a.post;
   k.pre_pad = a.pre_pad;
@@ -270,11 +371,11 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.PV = (a.W + 2 * HALO) * (a.H + 2 * HALO);
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
-    fpz_dispatch<1, float>(k, a.geom, st);
+    fpz_dispatch<1, float>(k, a.geom, a.quad, st);
   else if (a.st_dt == 1)
-    fpz_dispatch<1, _Float16>(k, a.geom, st);
+    fpz_dispatch<1, _Float16>(k, a.geom, a.quad, st);
   else
-    fpz_dispatch<1, bf16_t>(k, a.geom, st);
+    fpz_dispatch<1, bf16_t>(k, a.geom, a.quad, st);
   return hipGetLastError();
 }
 

@@ -18,7 +18,13 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 // in[x + in_sy*y + in_sz*z] -> out[(x+2)*(ny+4)*(nz+4) + (y+2)*(nz+4) + z+2].
 hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
                         int st_dt, int nx, int ny, int nz, int mode, hipStream_t st);
-// 2D stack: in [w,h,n] -> out [w+4, h+4, n].
+// z-fastest corner-quad copy: element (x,y,z), x,y in [-2,n], z in [-2,nz+1], at
+// ((x+2)*(ny+3) + y+2)*(nz+4) + z+2 holds {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
+// V(x+1,y+1,z)} of the haloed volume V (same input addressing as launch_pad3z).
+hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st);
+// 2D stack, row-paired: in [w,h,n] -> out [2, w+4, h+3, n] where element
+// (px, py) holds the haloed values at rows py and py+1 of column px.
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
                        int mode, hipStream_t st);
 // 3D interior copy between a padded f32 volume and a plain one.
@@ -44,8 +50,9 @@ hipError_t launch_fp(const FPLaunch& a, hipStream_t st);
 
 // standard-geometry FP on a z-fastest haloed volume (launch_pad3z layout)
 struct FPZLaunch {
-  const void* vol;
+  const void* vol;  // launch_pad3q output (quad = 1) or launch_pad3z output (quad = 0)
   int st_dt;
+  int quad;
   float* out;
   const float2* sc;
   int geom, W, H, nviews;
@@ -57,6 +64,7 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 struct OSCFPLaunch {
   const void* mu_z;     // z-fastest volume of storage st_dt with a zero (border) halo
   int st_dt;            // storage of mu_z and of pre_pad / post_pad
+  int quad;             // mu_z layout: launch_pad3q (1) or launch_pad3z (0)
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
   void* pre_pad;        // [W+4,H+4,count] padded copies for the BP (zero halo)
@@ -70,7 +78,7 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st);
 
 // ---- backprojection (bp.hip)
 struct BPLaunch {
-  const void* sino;  // padded storage sinogram [W+4, H+4, nProj] (edge halo)
+  const void* sino;  // row-paired haloed sinogram [2, W+4, H+3, nProj] (launch_pad2, edge halo)
   int st_dt;
   float* out;        // [W, W, H]
   const float2* sc;
@@ -136,8 +144,10 @@ hipError_t launch_prefilter(const float* sino, float* out, const float* wgt, con
                             const float* dA, int W, int H, int nviews, hipStream_t st);
 
 // ---- gather microbenchmark (probe.hip)
-hipError_t launch_gather_probe(const void* vol_z, int st_dt, int NX, int NY, int NZ, int nblocks_x,
-                               int nblocks_y, int nsteps, int nloads, float* sink, hipStream_t st);
+// pattern 0: the FP's quad-volume loads; 1: the BP's row-paired loads.
+// buf holds at least the quad volume / row-paired sinogram of W, H, nviews.
+hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, int H, int nviews,
+                               float* sink, int64_t* samples, hipStream_t st);
 
 // ---- context internals shared with the request server (api.hip)
 hipStream_t ctx_stream(const mexct_ctx* c);

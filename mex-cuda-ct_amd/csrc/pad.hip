@@ -39,30 +39,41 @@ __global__ __launch_bounds__(256) void pad3_kernel(const void* __restrict__ in, 
   }
 }
 
+// Row-paired haloed sinogram for the backprojector:
+// out[((view*(PY-1) + py)*PX + px)*2 + r] = P(px-2, py-2+r), r in {0,1},
+// PX = w+4, PY = h+4, with clamp (edge copies) or border (zeros) outside.
+// Element (py, px) and its neighbour (py, px+1) hold the four taps of a
+// bilinear sample at (px-2, py-2), so one 16-byte (f32) or 8-byte
+// (f16/bf16) load fetches them all.
 template <typename S>
 __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, int in_dt,
                                                    S* __restrict__ out, int w, int h, int n,
                                                    int mode) {
-  const int PX = w + 2 * HALO, PY = h + 2 * HALO;
-  const size_t total = (size_t)PX * PY * n;
+  const int PX = w + 2 * HALO, PR = h + 2 * HALO - 1;
+  const size_t total = (size_t)PX * PR * n;
   for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
        o += (size_t)gridDim.x * blockDim.x) {
-    int px = (int)(o % PX);
-    size_t r = o / PX;
-    int py = (int)(r % PY);
-    int view = (int)(r / PY);
-    int i = px - HALO, j = py - HALO;
-    float v;
-    if (mode == ADDR_CLAMP) {
-      i = clampi(i, 0, w - 1);
-      j = clampi(j, 0, h - 1);
-      v = load_any(in, in_dt, (size_t)i + (size_t)w * j + (size_t)w * h * view);
-    } else if (i < 0 || i >= w || j < 0 || j >= h) {
-      v = 0.f;
-    } else {
-      v = load_any(in, in_dt, (size_t)i + (size_t)w * j + (size_t)w * h * view);
+    const int px = (int)(o % PX);
+    const size_t r = o / PX;
+    const int py = (int)(r % PR);
+    const int view = (int)(r / PR);
+    const size_t vbase = (size_t)w * h * view;
+    float v[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int i = px - HALO, j = py - HALO + q;
+      if (mode == ADDR_CLAMP) {
+        i = clampi(i, 0, w - 1);
+        j = clampi(j, 0, h - 1);
+        v[q] = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
+      } else if (i < 0 || i >= w || j < 0 || j >= h) {
+        v[q] = 0.f;
+      } else {
+        v[q] = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
+      }
     }
-    out[o] = Store<S>::from(v);
+    out[2 * o] = Store<S>::from(v[0]);
+    out[2 * o + 1] = Store<S>::from(v[1]);
   }
 }
 
@@ -104,6 +115,71 @@ __global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in,
     const int lx = ty + 8 * r;
     const int px = px0 + lx, pz = pz0 + tx;
     if (px < PX && pz < PZ) out[((size_t)px * PY + py) * PZ + pz] = Store<S>::from(tile[tx][lx]);
+  }
+}
+
+// Corner-quad copy (see launch_pad3q).  A block covers 32 z x 32 x of one
+// y: the (z, x..x+32) tiles of input rows y and y+1 are staged in LDS (read
+// along x), then each lane writes one quad element (along z).
+template <typename S>
+struct Quad;
+template <>
+struct Quad<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static void put(float* out, size_t e, float a, float b, float c, float d) {
+    v4 q = {a, b, c, d};
+    *(v4*)(out + 4 * e) = q;
+  }
+};
+template <>
+struct Quad<_Float16> {
+  typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+  __device__ static void put(_Float16* out, size_t e, float a, float b, float c, float d) {
+    v4 q = {(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+    *(v4*)(out + 4 * e) = q;
+  }
+};
+template <>
+struct Quad<bf16_t> {
+  typedef uint16_t v4 __attribute__((ext_vector_type(4)));
+  __device__ static void put(bf16_t* out, size_t e, float a, float b, float c, float d) {
+    v4 q = {f32_to_bf16(a), f32_to_bf16(b), f32_to_bf16(c), f32_to_bf16(d)};
+    *(v4*)(out + 4 * e) = q;
+  }
+};
+
+template <typename S>
+__global__ __launch_bounds__(256) void pad3q_kernel(const void* __restrict__ in, int in_dt,
+                                                    int64_t in_sy, int64_t in_sz,
+                                                    S* __restrict__ out, int nx, int ny, int nz,
+                                                    int mode) {
+  __shared__ float tile[2][32][33];
+  const int QX = nx + 2 * HALO - 1, QY = ny + 2 * HALO - 1, QZ = nz + 2 * HALO;
+  const int qz0 = blockIdx.x * 32, qx0 = blockIdx.y * 32, qy = blockIdx.z;
+  for (int e = threadIdx.x; e < 2 * 32 * 33; e += 256) {
+    const int r = e / (32 * 33);
+    const int rem = e - r * (32 * 33);
+    const int lz = rem / 33, lx = rem - lz * 33;
+    int i = qx0 + lx - HALO, j = qy + r - HALO, k = qz0 + lz - HALO;
+    float v = 0.f;
+    if (mode == ADDR_CLAMP) {
+      i = clampi(i, 0, nx - 1);
+      j = clampi(j, 0, ny - 1);
+      k = clampi(k, 0, nz - 1);
+      v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
+    } else if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+      v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
+    }
+    tile[r][lz][lx] = v;
+  }
+  __syncthreads();
+  const int tz = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int p = 0; p < 4; ++p) {
+    const int lx = ty + 8 * p;
+    const int qx = qx0 + lx, qz = qz0 + tz;
+    if (qx < QX && qz < QZ)
+      Quad<S>::put(out, ((size_t)qx * QY + qy) * QZ + qz, tile[0][tz][lx], tile[1][tz][lx],
+                   tile[0][tz][lx + 1], tile[1][tz][lx + 1]);
   }
 }
 
@@ -150,7 +226,7 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
                        int mode, hipStream_t st) {
-  size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO) * n;
+  size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO - 1) * n;
   unsigned g = grid_for(total);
   if (st_dt == 0)
     pad2_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode);
@@ -158,6 +234,21 @@ hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, i
     pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode);
   else
     pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st) {
+  const dim3 g((unsigned)((nz + 2 * HALO + 31) / 32), (unsigned)((nx + 2 * HALO - 1 + 31) / 32),
+               (unsigned)(ny + 2 * HALO - 1));
+  if (st_dt == 0)
+    pad3q_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode);
+  else if (st_dt == 1)
+    pad3q_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (_Float16*)out, nx, ny, nz,
+                                              mode);
+  else
+    pad3q_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (bf16_t*)out, nx, ny, nz,
+                                            mode);
   return hipGetLastError();
 }
 

@@ -1,87 +1,102 @@
 // probe.hip -- gather-throughput microbenchmark (measurement only).
 //
-// Issues the forward projector's load pattern -- per sample four (k, k+1)
-// pair loads at the (i,j), (i,j+1), (i+1,j), (i+1,j+1) corners of a
-// z-fastest haloed volume, 32-bit buffer offsets, corner strides in scalar
-// offsets -- with ideal coherence: the 64 lanes of a wave hold 64
-// consecutive z, each wave one (j) column, and every lane steps along x.
-// The only VALU work is the offset step and a sum that keeps the loads
-// alive, so the rate it reaches is the memory pipeline's ceiling for the
-// projector's gathers (the "measured gather roofline").  The backprojector
-// pattern uses two pair loads per update ((j0, j0+1) rows of one column).
+// Issues the projectors' own load instructions with ideal coherence, with
+// only the VALU work that keeps the loads alive, so the rate it reaches is
+// the memory pipeline's ceiling for that pattern (the "measured gather
+// roofline"):
+//
+//   FP: the corner-quad z-fastest volume; per sample two 16-byte loads
+//       (f32) or one (f16/bf16) of consecutive quad elements.  The 64 lanes
+//       of a wave hold 64 consecutive z, each wave one y column, and every
+//       lane steps along x.
+//   BP: the row-paired haloed sinogram; per update one 16-byte (f32) or
+//       8-byte (f16/bf16) load of two adjacent pair elements.  The 64 lanes
+//       of a wave hold 64 consecutive detector columns; every lane walks
+//       down the rows of one view, wrapping, so each view's slice is re-read
+//       from cache as the backprojector re-reads it.
 #include "common.h"
 #include "internal.h"
 
 namespace mexct {
 
-template <typename S>
-__device__ __forceinline__ float2 pld2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  if constexpr (sizeof(S) == 4) {
-    typedef float f2v __attribute__((ext_vector_type(2)));
-    f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-    return make_float2(v.x, v.y);
+template <int LB>
+__device__ __forceinline__ float probe_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  if constexpr (LB == 16) {
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+    return __uint_as_float(v.x ^ v.y ^ v.z ^ v.w);
   } else {
-    typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-    h2v v = __builtin_bit_cast(h2v, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-    return make_float2((float)v.x, (float)v.y);
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+    const u2v v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+    return __uint_as_float(v.x ^ v.y);
   }
 }
 
-template <typename S, int NL>
-__global__ __launch_bounds__(256) void gather_probe_kernel(const void* vol, uint32_t bytes,
-                                                           uint32_t off_j, uint32_t off_i,
-                                                           uint32_t step_bytes, int PY, int PZ,
-                                                           int ncols, int nz, int nsteps,
-                                                           float* sink) {
-  const int col = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) % ncols;  // j
+// FP pattern.  Quad element (x, y, z) at ((x+2)*QY + y+2)*QZ + z+2, E bytes.
+template <int NL>
+__global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t bytes,
+                                                       uint32_t E, int QY, int QZ, int ncols,
+                                                       int nz, int nsteps, float* sink) {
+  const int col = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) % ncols;  // y
   const int z = (int)(blockIdx.y * 64 + (threadIdx.x & 63)) % nz;
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)vol, 0, (int)bytes, 0x00020000);
-  uint32_t voff = (uint32_t)(((1 * PY) + col + 1) * PZ + z + 1) * (uint32_t)sizeof(S);
-  float s0 = 0.f, s1 = 0.f;
+  uint32_t voff = (uint32_t)((2 * QY + col + 2) * QZ + z + 2) * E;
+  const uint32_t step = (uint32_t)QY * (uint32_t)QZ * E;
+  float s = 0.f;
   for (int k = 0; k < nsteps; ++k) {
-    const float2 a = pld2<S>(r, voff, 0);
-    const float2 b = pld2<S>(r, voff, off_j);
-    s0 = s0 + a.x;
-    s1 = s1 + a.y;
-    s0 = s0 + b.x;
-    s1 = s1 + b.y;
-    if (NL == 4) {
-      const float2 c = pld2<S>(r, voff, off_i);
-      const float2 d = pld2<S>(r, voff, off_i + off_j);
-      s0 = s0 + c.x;
-      s1 = s1 + c.y;
-      s0 = s0 + d.x;
-      s1 = s1 + d.y;
-    }
-    voff += step_bytes;
+    s = s + probe_ld<16>(r, voff);
+    if (NL == 2) s = s + probe_ld<16>(r, voff + 16u);
+    voff += step;
   }
-  if (s0 + s1 == 1234.5f) sink[threadIdx.x] = s0;  // keeps the loads; never true for the probe data
+  if (s == 1234.5f) sink[threadIdx.x] = s;  // keeps the loads; never true for the probe data
 }
 
-hipError_t launch_gather_probe(const void* vol_z, int st_dt, int NX, int NY, int NZ, int nblocks_x,
-                               int nblocks_y, int nsteps, int nloads, float* sink,
-                               hipStream_t st) {
-  const int PY = NY + 2 * HALO, PZ = NZ + 2 * HALO;
+// BP pattern.  Pair element (u, row) of view v at v*VB + (row+2)*RB + (u+2)*E2.
+template <int LB>
+__global__ __launch_bounds__(256) void bp_probe_kernel(const void* sino, uint32_t bytes,
+                                                       uint32_t E2, uint32_t RB, uint32_t VB,
+                                                       int W, int H, int nviews, int nsteps,
+                                                       float* sink) {
+  const int u = (int)(blockIdx.x * 64 + (threadIdx.x & 63)) % W;
+  const int view = (int)(blockIdx.y % (unsigned)nviews);
+  int row = (int)(((blockIdx.y / (unsigned)nviews) * 4 + (threadIdx.x >> 6)) % (unsigned)H);
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)sino, 0, (int)bytes, 0x00020000);
+  const uint32_t base = (uint32_t)view * VB + 2u * RB + (uint32_t)(u + 2) * E2;
+  float s = 0.f;
+  for (int k = 0; k < nsteps; ++k) {
+    s = s + probe_ld<LB>(r, base + (uint32_t)row * RB);
+    row = row + 1 == H ? 0 : row + 1;
+  }
+  if (s == 1234.5f) sink[threadIdx.x] = s;
+}
+
+hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, int H, int nviews,
+                               float* sink, int64_t* samples, hipStream_t st) {
   const uint32_t es = st_dt == 0 ? 4u : 2u;
-  const uint32_t bytes = (uint32_t)((size_t)(NX + 2 * HALO) * PY * PZ * es);
-  const uint32_t off_j = (uint32_t)PZ * es, off_i = (uint32_t)PY * PZ * es;
-  const int steps = nsteps < NX ? nsteps : NX;
-  const dim3 g((unsigned)nblocks_x, (unsigned)nblocks_y);
-  if (st_dt == 0) {
-    if (nloads == 4)
-      gather_probe_kernel<float, 4><<<g, 256, 0, st>>>(vol_z, bytes, off_j, off_i, off_i, PY, PZ,
-                                                       NY, NZ, steps, sink);
+  if (pattern == 0) {
+    const int QY = W + 2 * HALO - 1, QZ = H + 2 * HALO;
+    const uint32_t E = 4u * es;
+    const uint32_t bytes = (uint32_t)((size_t)QY * QY * QZ * E);
+    const dim3 g((unsigned)((W + 3) / 4), (unsigned)((H + 63) / 64 * nviews));
+    if (st_dt == 0)
+      fp_probe_kernel<2><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
     else
-      gather_probe_kernel<float, 2><<<g, 256, 0, st>>>(vol_z, bytes, off_j, off_i, off_i, PY, PZ,
-                                                       NY, NZ, steps, sink);
+      fp_probe_kernel<1><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
+    *samples = (int64_t)g.x * 4 * (int64_t)g.y * 64 * W;
   } else {
-    if (nloads == 4)
-      gather_probe_kernel<_Float16, 4><<<g, 256, 0, st>>>(vol_z, bytes, off_j, off_i, off_i, PY, PZ,
-                                                          NY, NZ, steps, sink);
+    const uint32_t E2 = 2u * es, RB = (uint32_t)(W + 2 * HALO) * E2;
+    const uint32_t VB = RB * (uint32_t)(H + 2 * HALO - 1);
+    const uint32_t bytes = VB * (uint32_t)nviews;
+    const unsigned gx = (unsigned)((W + 63) / 64);
+    const unsigned groups = (unsigned)((size_t)W * W / ((size_t)4 * 64 * gx));
+    const dim3 g(gx, (unsigned)nviews * (groups > 0 ? groups : 1));
+    if (st_dt == 0)
+      bp_probe_kernel<16><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
     else
-      gather_probe_kernel<_Float16, 2><<<g, 256, 0, st>>>(vol_z, bytes, off_j, off_i, off_i, PY, PZ,
-                                                          NY, NZ, steps, sink);
+      bp_probe_kernel<8><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
+    *samples = (int64_t)g.x * (int64_t)g.y * 256 * H;
   }
   return hipGetLastError();
 }

@@ -69,7 +69,7 @@ int64_t footprint_of(const mexct_request& r) {
   const size_t W = (size_t)r.W, H = (size_t)r.H, P = (size_t)r.nProj;
   const size_t v = W * W * H * 4, s = W * H * P * 4;
   const size_t vh = (W + 4) * (W + 4) * (H + 4) * 4;
-  const size_t sh = (W + 4) * (H + 4) * P * 4;
+  const size_t sh = (W + 4) * (H + 3) * P * 8;  // row-paired haloed sinogram
   size_t b = 0;
   switch (r.op) {
     case MEXCT_OP_FP: b = v + s + vh; break;

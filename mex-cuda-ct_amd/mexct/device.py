@@ -97,12 +97,13 @@ class Context:
     def workspace_bytes(self) -> int:
         return int(lib().mexct_ctx_workspace_bytes(self._h))
 
-    def probe_gather(self, storage, W, H, nviews, nloads=4):
-        """Gather microbenchmark launch (see mexct_probe_gather_dev); returns
-        the sample count of the launch."""
+    def probe_gather(self, storage, W, H, nviews, pattern=0):
+        """Gather microbenchmark launch (see mexct_probe_gather_dev): pattern
+        0 is the forward projector's loads, 1 the backprojector's; returns the
+        sample (update) count of the launch."""
         n = C.c_int64()
         self._call(lib().mexct_probe_gather_dev, self._h, int(storage), int(W), int(H),
-                   int(nviews), int(nloads), C.byref(n))
+                   int(nviews), int(pattern), C.byref(n))
         return n.value
 
     # ------------------------------------------------------------ operators

@@ -274,3 +274,41 @@ def test_osc_iter_storage_vs_fp64(gpu, storage):
     e = np.linalg.norm(mu - mu64) / np.linalg.norm(mu64)
     print(f"oscIter storage={storage} relRMSE vs FP64 = {e:.3e}")
     assert e <= STORAGE_BOUND[storage]
+
+
+def _round_storage(a, storage):
+    """The values the device stores for `a` in FP16 / BF16 (round to nearest even)."""
+    a = np.asarray(a, dtype=np.float32)
+    if storage == mexct.F16:
+        r = a.astype(np.float16).astype(np.float32)
+    else:
+        u = a.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        r = u.astype(np.uint32).view(np.float32)
+    return np.asfortranarray(r)
+
+
+# The storage forms change only what is stored: the arithmetic is the FP32
+# arithmetic of the reference, so FP/BP with FP16/BF16 storage must equal the
+# oracle run on the rounded operand bit for bit (this also pins the mixed-
+# precision lerps of the FP16 path).
+@pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_fp_storage_exact_on_rounded_volume(gpu, geom, storage):
+    W, H, n = 40, 36, 8
+    vol = _vol(W, H, 61 + geom)
+    ang = _angles(n)
+    got = mexct.CUDAmex_FP(vol, 400.0, ang, n, geom, storage=storage)
+    ref = O.fp(_round_storage(vol, storage), 400.0, ang, n, geom)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_bp_storage_exact_on_rounded_sinogram(gpu, geom, storage):
+    W, H, n = 40, 36, 8
+    P = np.asfortranarray(_rng(71 + geom).uniform(0, 1, (W, H, n)).astype(np.float32))
+    ang = _angles(n)
+    got = mexct.CUDAmex_BP(P, 400.0, ang, geom, storage=storage)
+    ref = O.bp(_round_storage(P, storage), 400.0, ang, geom)
+    assert np.array_equal(got, ref)
