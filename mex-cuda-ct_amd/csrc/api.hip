@@ -473,11 +473,11 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
   RC(check_padded(W, H, nProj, "sinogram"));
-  // row-paired haloed copy: (W+4) x (H+3) elements of two values per view
+  // row-paired haloed copy: (W+4) x (H+3) elements of two values per view;
+  // the kernel takes views in chunks below 2 GiB, so one view must fit
   const size_t rp_bytes = (size_t)(W + 4) * (H + 3) * 2 * store_bytes(st);
-  if ((double)rp_bytes * nProj >= 2147483648.0)
-    return fail(MEXCT_EINVAL, "sinogram too large for one call (%lld x %lld x %lld)",
-                (long long)W, (long long)H, (long long)nProj);
+  if ((double)rp_bytes >= 2147483648.0)
+    return fail(MEXCT_EINVAL, "detector too large (%lld x %lld)", (long long)W, (long long)H);
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
   DevGuard g(c->device);
   void* pad;

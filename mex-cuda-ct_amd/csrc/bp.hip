@@ -54,6 +54,7 @@ struct BPKArgs {
   const float2* sc;
   int W, H, nviews, PW;  // PW: elements (pairs) per row
   int64_t PV;            // pairs per view
+  int accumulate;        // 1: continue the sums already in out (a later chunk of views)
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
 };
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const int row_bytes = a.PW * (int)es;          // one detector row of pairs
   float acc[ZR];
 #pragma unroll
-  for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
+  for (int m = 0; m < ZR; ++m) acc[m] = (a.accumulate && m < zc) ? o[(size_t)m * WW] : 0.f;
 
   for (int v = 0; v < a.nviews; ++v) {
     const float2 scv = a.sc[v];
@@ -200,17 +201,16 @@ static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, hipStream_t st) {
     bp_dispatch<32, S>(k, geom, st);
 }
 
+// Views go in chunks whose row-paired slices stay below 2 GiB (32-bit
+// buffer offsets); each chunk continues the register sums the previous one
+// stored, so the per-voxel order and the float sums are unchanged.
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   const int zr = a.zr == 64 ? 64 : 32;
+  const size_t es2 = 2u * (a.st_dt == 0 ? 4u : 2u);
   BPKArgs k;
-  k.sino = a.sino;
-  k.sino_bytes = (uint32_t)((size_t)(a.W + 2 * HALO) * (a.H + 2 * HALO - 1) * a.nviews * 2u *
-                            (a.st_dt == 0 ? 4u : 2u));
   k.out = a.out;
-  k.sc = a.sc;
   k.W = a.W;
   k.H = a.H;
-  k.nviews = a.nviews;
   k.PW = a.W + 2 * HALO;
   k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO - 1);
   k.v0 = float(a.H - 1) / 2;
@@ -219,13 +219,27 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.nz = (uint32_t)((a.H + zr - 1) / zr);
   k.n_logical = k.tiles_x * k.tiles_x * k.nz;
   if (k.n_logical == 0) return hipSuccess;
-  if (a.st_dt == 0)
-    bp_dispatch_zr<float>(k, a.geom, zr, st);
-  else if (a.st_dt == 1)
-    bp_dispatch_zr<_Float16>(k, a.geom, zr, st);
-  else
-    bp_dispatch_zr<bf16_t>(k, a.geom, zr, st);
-  return hipGetLastError();
+  const size_t view_bytes = (size_t)k.PV * es2;
+  const int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);
+  int v0 = 0;
+  do {  // at least one launch, so nviews == 0 still writes the zero volume
+    const int cn = std::min(chunk, a.nviews - v0);
+    k.sino = (const char*)a.sino + (size_t)v0 * view_bytes;
+    k.sino_bytes = (uint32_t)((size_t)cn * view_bytes);
+    k.sc = a.sc + v0;
+    k.nviews = cn;
+    k.accumulate = v0 > 0;
+    if (a.st_dt == 0)
+      bp_dispatch_zr<float>(k, a.geom, zr, st);
+    else if (a.st_dt == 1)
+      bp_dispatch_zr<_Float16>(k, a.geom, zr, st);
+    else
+      bp_dispatch_zr<bf16_t>(k, a.geom, zr, st);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    v0 += cn;
+  } while (v0 < a.nviews);
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

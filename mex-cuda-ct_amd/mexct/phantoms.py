@@ -8,10 +8,10 @@ from __future__ import annotations
 import numpy as np
 
 
-def _grid(W, H):
-    c = np.arange(W, dtype=np.float64) - (W - 1) / 2.0
-    z = np.arange(H, dtype=np.float64) - (H - 1) / 2.0
-    return np.meshgrid(c, c, z, indexing="ij")  # x, y, z with vol[x, y, z]
+def _axes(W, H):
+    """Voxel-centre coordinates along x (= y) and z."""
+    return (np.arange(W, dtype=np.float64) - (W - 1) / 2.0,
+            np.arange(H, dtype=np.float64) - (H - 1) / 2.0)
 
 
 def cylinder(W, H, radius, value=0.02):
@@ -29,25 +29,36 @@ def ellipsoid_phantom(W, H, seed, n_ellipsoids, radius, base=0.02, axes=(5.0, 20
     """Cylinder of `base` plus seeded random ellipsoids (z-rotated)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     vol = cylinder(W, H, radius, base).astype(np.float64)
-    x, y, z = _grid(W, H)
+    c, zc = _axes(W, H)
     cr = radius * 0.8 if centre_r is None else centre_r
     cz = (H / 2.0) * 0.6 if centre_z is None else centre_z
     for _ in range(n_ellipsoids):
-        a, b, c = rng.uniform(axes[0], axes[1], size=3)
+        a, b, cc = rng.uniform(axes[0], axes[1], size=3)
         val = rng.uniform(values[0], values[1])
         r = cr * np.sqrt(rng.uniform())
         phi = rng.uniform(0, 2 * np.pi)
         x0, y0, z0 = r * np.cos(phi), r * np.sin(phi), rng.uniform(-cz, cz)
         th = rng.uniform(0, np.pi)
         ct, st = np.cos(th), np.sin(th)
-        dx, dy, dz = x - x0, y - y0, z - z0
+        # the z-rotated ellipsoid lies within |dx|, |dy| <= max(a, b), |dz| <= c
+        rxy = max(a, b)
+        ix = np.nonzero(np.abs(c - x0) <= rxy)[0]
+        iy = np.nonzero(np.abs(c - y0) <= rxy)[0]
+        iz = np.nonzero(np.abs(zc - z0) <= cc)[0]
+        if ix.size == 0 or iy.size == 0 or iz.size == 0:
+            continue
+        sx = slice(ix[0], ix[-1] + 1)
+        sy = slice(iy[0], iy[-1] + 1)
+        sz = slice(iz[0], iz[-1] + 1)
+        dx, dy, dz = np.meshgrid(c[sx] - x0, c[sy] - y0, zc[sz] - z0, indexing="ij")
         u = ct * dx + st * dy
         v = -st * dx + ct * dy
-        m = (u / a) ** 2 + (v / b) ** 2 + (dz / c) ** 2 <= 1.0
+        m = (u / a) ** 2 + (v / b) ** 2 + (dz / cc) ** 2 <= 1.0
+        box = vol[sx, sy, sz]
         if additive:
-            vol[m] += val
+            box[m] += val
         else:
-            vol[m] = val
+            box[m] = val
     return np.asfortranarray(vol.astype(np.float32))
 
 
