@@ -80,7 +80,7 @@ enum Slot {
   WS_POSTW,
   WS_PLAIN,
   WS_MUZ,      // z-fastest copy of the OSC volume
-  WS_SINOMAX,  // max(|pre|, |post|) bits for the FP16 OSC copies
+  WS_MAXBITS,  // max|x| bits for the FP16 copies: [0] OSC ratio terms, [1] volume, [2] sinogram
   WS_COUNT
 };
 
@@ -254,6 +254,18 @@ int check_padded(int64_t a, int64_t b, int64_t c, const char* what) {
 
 int storage_of(const mexct_opts* o) { return o ? o->storage : MEXCT_F32; }
 
+// Device word holding max|x| of n floats (as float bits), slot k of
+// WS_MAXBITS, for the power-of-two scaling of FP16 copies.
+int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out) {
+  void* m;
+  RC(c->get(WS_MAXBITS, 256, &m));
+  uint32_t* p = (uint32_t*)m + k;
+  HIPCHK(launch_fill((float*)p, 0.f, 1, c->stream));
+  HIPCHK(launch_absmax(x, n, p, c->stream));
+  *out = p;
+  return MEXCT_OK;
+}
+
 int tv_steps_of(const mexct_opts* o) { return (o && o->tv_steps > 0) ? o->tv_steps : 20; }
 
 int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
@@ -388,24 +400,31 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   DevGuard g(c->device);
   void* pad;
   const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st);
+  // corner quads for 2-byte storage (one 16-byte load per sample); f32 keeps
+  // the plain layout, whose four 8-byte pair loads draw half the cache lines
+  // of two 16-byte quad loads and run faster
   const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * store_bytes(st);
-  const int quad = qbytes < ((size_t)1 << 32);
+  const int quad = st != MEXCT_F32 && qbytes < ((size_t)1 << 32);
   const float2* sc;
   if (quad || pbytes < ((size_t)1 << 31)) {
     // z-fastest haloed volume (corner quads when they fit 32-bit offsets),
     // one wave per 64 detector rows of a column
     RC(c->get(WS_VOL, quad ? qbytes : pbytes, &pad));
+    uint32_t* vmax = nullptr;
+    if (st == MEXCT_F16 && vol_dtype == MEXCT_F32)
+      RC(max_bits(c, (const float*)vol, (size_t)W * W * H, 1, &vmax));
     if (quad)
       HIPCHK(launch_pad3q(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
-                          c->stream));
+                          c->stream, vmax));
     else
       HIPCHK(launch_pad3z(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
-                          c->stream));
+                          c->stream, vmax));
     RC(c->sincos_table(angles, nProj, &sc));
     FPZLaunch z;
     z.vol = pad;
     z.st_dt = st;
     z.quad = quad;
+    z.vol_max = vmax;
     z.out = out;
     z.sc = sc;
     z.geom = geom;
@@ -482,13 +501,18 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   DevGuard g(c->device);
   void* pad;
   RC(c->get(WS_SINO, rp_bytes * std::max<int64_t>(nProj, 1), &pad));
+  uint32_t* smax = nullptr;
+  if (st == MEXCT_F16 && sino_dtype == MEXCT_F32 && nProj > 0)
+    RC(max_bits(c, (const float*)sino, (size_t)W * H * nProj, 2, &smax));
   if (nProj > 0)
-    HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream));
+    HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
+                       smax));
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   BPLaunch a;
   a.sino = pad;
   a.st_dt = st;
+  a.sino_max = smax;
   a.out = out;
   a.sc = sc;
   a.geom = geom;
@@ -666,15 +690,11 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   void* mz;
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
   const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
-  const int quad = qbytes < ((size_t)1 << 32);
+  const int quad = st != MEXCT_F32 && qbytes < ((size_t)1 << 32);
   RC(c->get(WS_MUZ, quad ? qbytes : (size_t)MPXY * (H + 4) * es, &mz));
   uint32_t* smax = nullptr;
   if (st == MEXCT_F16) {
-    void* m;
-    RC(c->get(WS_SINOMAX, 256, &m));
-    smax = (uint32_t*)m;
-    HIPCHK(launch_fill((float*)smax, 0.f, 1, c->stream));
-    HIPCHK(launch_absmax(pre, (size_t)W * H * nProj, smax, c->stream));
+    RC(max_bits(c, pre, (size_t)W * H * nProj, 0, &smax));
     HIPCHK(launch_absmax(post, (size_t)W * H * nProj, smax, c->stream));
   }
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
@@ -683,16 +703,19 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
     const int count = (sel >= 0 && sel < nProj) ? (nProj - 1 - sel) / numSubsets + 1 : 0;
     if (count > 0) {
       // the subset projects the volume as left by the previous update
+      uint32_t* vmax = nullptr;
+      if (st == MEXCT_F16) RC(max_bits(c, mu_pad, (size_t)MPXY * (H + 4), 1, &vmax));
       if (quad)
         HIPCHK(launch_pad3q(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
-                            c->stream));
+                            c->stream, vmax));
       else
         HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
-                            c->stream));
+                            c->stream, vmax));
       OSCFPLaunch f;
       f.mu_z = mz;
       f.st_dt = st;
       f.quad = quad;
+      f.vol_max = vmax;
       f.pre = pre;
       f.post = post;
       f.pre_pad = pp;
@@ -922,9 +945,9 @@ int mexct_probe_gather_dev(mexct_ctx* c, int32_t storage, int64_t W, int64_t H, 
       !samples)
     return fail(MEXCT_EINVAL, "bad probe arguments");
   const size_t es = (size_t)store_bytes(storage);
-  const size_t bytes = pattern == MEXCT_PROBE_FP
-                           ? (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es
-                           : (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews;
+  const size_t bytes = pattern != MEXCT_PROBE_FP ? (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews
+                       : storage == MEXCT_F32  ? (size_t)(W + 4) * (W + 4) * (H + 4) * es
+                                               : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
   if (bytes >= ((size_t)1 << 32)) return fail(MEXCT_EINVAL, "probe buffer too large");
   DevGuard g(c->device);
   void *buf, *sink;

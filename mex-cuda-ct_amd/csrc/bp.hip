@@ -21,9 +21,21 @@
 namespace mexct {
 
 // Four taps of a row-paired sinogram element pair: {(j0,i0), (j0+1,i0),
-// (j0,i0+1), (j0+1,i0+1)} in one load.
+// (j0,i0+1), (j0+1,i0+1)} in one load.  f32 and bf16 taps are widened;
+// f16 taps stay packed ({(j0,i0), (j0+1,i0)} and {(j0,i0+1), (j0+1,i0+1)}
+// per dword) for the mixed-precision lerps.
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 template <typename S>
-__device__ __forceinline__ float4 bld4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+struct Taps4 {
+  typedef float4 T;
+};
+template <>
+struct Taps4<_Float16> {
+  typedef u2v T;
+};
+template <typename S>
+__device__ __forceinline__ typename Taps4<S>::T bld4(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                     uint32_t soff);
 template <>
 __device__ __forceinline__ float4 bld4<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                               uint32_t soff) {
@@ -32,16 +44,13 @@ __device__ __forceinline__ float4 bld4<float>(__amdgpu_buffer_rsrc_t r, uint32_t
   return make_float4(v.x, v.y, v.z, v.w);
 }
 template <>
-__device__ __forceinline__ float4 bld4<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                                 uint32_t soff) {
-  typedef _Float16 h4v __attribute__((ext_vector_type(4)));
-  h4v v = __builtin_bit_cast(h4v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-  return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+__device__ __forceinline__ u2v bld4<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                              uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
 }
 template <>
 __device__ __forceinline__ float4 bld4<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                uint32_t soff) {
-  typedef uint32_t u2v __attribute__((ext_vector_type(2)));
   const u2v u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
   return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
                      __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
@@ -55,6 +64,7 @@ struct BPKArgs {
   int W, H, nviews, PW;  // PW: elements (pairs) per row
   int64_t PV;            // pairs per view
   int accumulate;        // 1: continue the sums already in out (a later chunk of views)
+  const uint32_t* sino_max;  // non-null: sino holds P * 2^-e (pow2_exp)
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
 };
@@ -122,7 +132,16 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const int row_bytes = a.PW * (int)es;          // one detector row of pairs
   float acc[ZR];
 #pragma unroll
-  for (int m = 0; m < ZR; ++m) acc[m] = (a.accumulate && m < zc) ? o[(size_t)m * WW] : 0.f;
+  for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
+  const int sexp = pow2_exp(a.sino_max);
+  if (a.accumulate) {
+    const float* q = o;
+#pragma unroll
+    for (int m = 0; m < ZR; ++m) {
+      if (m < zc) acc[m] = ldexpf(*q, -sexp);
+      q += WW;
+    }
+  }
 
   for (int v = 0; v < a.nviews; ++v) {
     const float2 scv = a.sc[v];
@@ -147,7 +166,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 #pragma unroll
     for (int m0 = 0; m0 < ZR; m0 += G) {
       float az[G];
-      float4 q[G];
+      typename Taps4<S>::T q[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const float zb = pz - 0.5f;
@@ -162,15 +181,24 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const float c0 = lerp(q[g].x, q[g].z, ax);  // row j0: columns i0, i0+1
-        const float c1 = lerp(q[g].y, q[g].w, ax);  // row j0+1
+        float c0, c1;  // rows j0 and j0+1, lerped between columns i0 and i0+1
+        if constexpr (__is_same(S, _Float16)) {
+          c0 = lerp_h<0>(q[g].x, q[g].y, ax);
+          c1 = lerp_h<1>(q[g].x, q[g].y, ax);
+        } else {
+          c0 = lerp(q[g].x, q[g].z, ax);
+          c1 = lerp(q[g].y, q[g].w, ax);
+        }
         acc[m0 + g] = acc[m0 + g] + UU * lerp(c0, c1, az[g]);
       }
     }
   }
+  float* q = o;
 #pragma unroll
-  for (int m = 0; m < ZR; ++m)
-    if (m < zc) o[(size_t)m * WW] = acc[m];
+  for (int m = 0; m < ZR; ++m) {
+    if (m < zc) *q = ldexpf(acc[m], sexp);
+    q += WW;
+  }
 }
 
 static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
@@ -209,6 +237,7 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   const size_t es2 = 2u * (a.st_dt == 0 ? 4u : 2u);
   BPKArgs k;
   k.out = a.out;
+  k.sino_max = a.sino_max;
   k.W = a.W;
   k.H = a.H;
   k.PW = a.W + 2 * HALO;

@@ -85,8 +85,39 @@ __device__ __forceinline__ float load_any(const void* p, int dtype, size_t i) {
   return bf16_to_f32(((const uint16_t*)p)[i]);
 }
 
+// FP16 copies of f32 data are stored as v * 2^-e, e the binary exponent of
+// max|v| (max = m * 2^e, m in [0.5, 1)), so that they use FP16's range
+// (subnormals below 6.1e-5, overflow above 65504) however the data is
+// scaled.  A power of two commutes with float rounding, so results computed
+// from the scaled copy and multiplied by 2^e equal those from an unscaled
+// copy wherever that one neither overflows nor underflows.
+__device__ __forceinline__ int pow2_exp(const uint32_t* maxbits) {
+  int e = 0;
+  if (maxbits) {
+    const float m = __uint_as_float(*maxbits);
+    if (m > 0.f && m < 3.0e38f) (void)frexpf(m, &e);
+  }
+  return e;
+}
+
 // texture lerp (1-t)*a + t*b evaluated as fma(t, b - a, a)
 __device__ __forceinline__ float lerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
+
+// lerp(a, b, t) of the f16 halves HI of A (a) and B (b), in float:
+// fma(b, 1, -a) is exactly the difference of the widened halves and
+// fma(t, d, a) the lerp, so the result equals lerp((float)a, (float)b, t).
+template <int HI>
+__device__ __forceinline__ float lerp_h(uint32_t A, uint32_t B, float t) {
+  float d, c;
+  if (HI) {
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
+  } else {
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
+  }
+  return c;
+}
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 

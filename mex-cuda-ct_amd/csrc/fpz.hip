@@ -11,13 +11,16 @@
 // about one line per lane.  A 256-thread workgroup holds 4 adjacent
 // columns.  Loads are 32-bit buffer offsets.
 //
-// Two z-fastest layouts.  QUAD (the default): element (x,y,z) holds the four
-// xy corners {V(x,y,z), V(x,y+1,z), V(x+1,y,z), V(x+1,y+1,z)}, so the eight
-// taps of a sample are two consecutive elements: two 16-byte loads in f32,
-// one in f16/bf16, instead of four pair loads.  It costs 4x the memory of
-// the plain layout, which MI355X's HBM holds easily; the plain layout
-// (pair loads at the (j+1), (i+1), (i+1,j+1) strides in scalar offsets)
-// serves volumes whose quad copy would pass 4 GiB.
+// Two z-fastest layouts.  Plain (f32): four (z, z+1) pair loads per sample
+// at the (j+1), (i+1), (i+1,j+1) strides in scalar offsets.  QUAD (f16 and
+// bf16): element (x,y,z) holds the four xy corners {V(x,y,z), V(x,y+1,z),
+// V(x+1,y,z), V(x+1,y+1,z)}, so the eight taps of a sample are two
+// consecutive elements, one 16-byte load, instead of four 4-byte pair loads
+// (the load unit takes about as long per wave for 4, 8 or 16 bytes per lane
+// when the data is cached).  The quad copy is 4x the plain one, which
+// MI355X's HBM holds easily; volumes whose quad copy would pass 4 GiB use
+// the plain layout.  For f32 the quad layout loses: its two 16-byte loads
+// draw twice the cache lines of the four pair loads.
 //
 // Arithmetic follows the oracle operation by operation (ray setup and clip
 // in ray.h; sample coordinate p - b, then - 0.5; lerps x, then y, then z),
@@ -37,6 +40,7 @@ struct FPZArgs {
   void* pre_pad;       // storage type S, FP16 scaled by 2^-e (e: exponent of *sino_max)
   void* post_pad;
   const uint32_t* sino_max;  // bits of max(|pre|, |post|), read when S is FP16
+  const uint32_t* vol_max;   // non-null: the volume holds V * 2^-e (pow2_exp)
   const float2* sc;
   int W, H, rows, NX, NY, NZ;
   int PY, PZ;
@@ -80,21 +84,6 @@ struct Tap8<_Float16> {
   float ax, ay, az;
 };
 
-// lerp(a, b, t) of the f16 halves HI of A (a) and B (b), in float:
-// fma(b, 1, -a) is exactly the difference of the widened halves and
-// fma(t, d, a) the lerp, so the result equals lerp((float)a, (float)b, t).
-template <int HI>
-__device__ __forceinline__ float lerp_h(uint32_t A, uint32_t B, float t) {
-  float d, c;
-  if (HI) {
-    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
-    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
-  } else {
-    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(B), "v"(A));
-    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
-  }
-  return c;
-}
 
 // Eight taps of one quad-layout sample: element e holds the xy corners at
 // z = k, element e+1 at z = k+1.
@@ -279,7 +268,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = march_z<S, STEP2, QUAD>(r, a, cur, inc, n);
+  const float dval = ldexpf(march_z<S, STEP2, QUAD>(r, a, cur, inc, n), pow2_exp(a.vol_max));
   if (EPI == 0) {
     a.out[p] = dval;
   } else {
@@ -339,15 +328,19 @@ static void fpz_dispatch_q(const FPZArgs& k, int geom, hipStream_t st) {
 
 template <int EPI, typename S>
 static void fpz_dispatch(const FPZArgs& k, int geom, int quad, hipStream_t st) {
-  if (quad)
-    fpz_dispatch_q<EPI, S, 1>(k, geom, st);
-  else
-    fpz_dispatch_q<EPI, S, 0>(k, geom, st);
+  if constexpr (sizeof(S) == 2) {
+    if (quad) {
+      fpz_dispatch_q<EPI, S, 1>(k, geom, st);
+      return;
+    }
+  }
+  fpz_dispatch_q<EPI, S, 0>(k, geom, st);
 }
 
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   FPZArgs k = make_zargs(a.vol, a.st_dt, a.quad, a.sc, a.W, a.H, a.nviews, a.sad);
   k.out = a.out;
+  k.vol_max = a.vol_max;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
     fpz_dispatch<0, float>(k, a.geom, a.quad, st);
@@ -365,6 +358,7 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.pre_pad = a.pre_pad;
   k.post_pad = a.post_pad;
   k.sino_max = a.sino_max;
+  k.vol_max = a.vol_max;
   k.first = a.first;
   k.stride = a.stride;
   k.PW = a.W + 2 * HALO;

@@ -17,16 +17,20 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 // z-fastest haloed copy: element (x,y,z) of the x-fastest input at
 // in[x + in_sy*y + in_sz*z] -> out[(x+2)*(ny+4)*(nz+4) + (y+2)*(nz+4) + z+2].
 hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
-                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st);
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max = nullptr);
 // z-fastest corner-quad copy: element (x,y,z), x,y in [-2,n], z in [-2,nz+1], at
 // ((x+2)*(ny+3) + y+2)*(nz+4) + z+2 holds {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
 // V(x+1,y+1,z)} of the haloed volume V (same input addressing as launch_pad3z).
 hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
-                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st);
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max = nullptr);
 // 2D stack, row-paired: in [w,h,n] -> out [2, w+4, h+3, n] where element
 // (px, py) holds the haloed values at rows py and py+1 of column px.
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
-                       int mode, hipStream_t st);
+                       int mode, hipStream_t st, const uint32_t* scale_max = nullptr);
+// scale_max (pad3z, pad3q, pad2): when non-null, the bits of max|in|; the
+// copy holds in * 2^-pow2_exp(scale_max) (see common.h).
 // 3D interior copy between a padded f32 volume and a plain one.
 hipError_t launch_unpad3(const float* padded, float* out, int nx, int ny, int nz, hipStream_t st);
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
@@ -53,6 +57,7 @@ struct FPZLaunch {
   const void* vol;  // launch_pad3q output (quad = 1) or launch_pad3z output (quad = 0)
   int st_dt;
   int quad;
+  const uint32_t* vol_max;  // non-null: vol holds V * 2^-e (pow2_exp); sums are rescaled
   float* out;
   const float2* sc;
   int geom, W, H, nviews;
@@ -65,6 +70,7 @@ struct OSCFPLaunch {
   const void* mu_z;     // z-fastest volume of storage st_dt with a zero (border) halo
   int st_dt;            // storage of mu_z and of pre_pad / post_pad
   int quad;             // mu_z layout: launch_pad3q (1) or launch_pad3z (0)
+  const uint32_t* vol_max;  // non-null: mu_z holds mu * 2^-e (pow2_exp)
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
   void* pre_pad;        // [W+4,H+4,count] padded copies for the BP (zero halo)
@@ -80,6 +86,7 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st);
 struct BPLaunch {
   const void* sino;  // row-paired haloed sinogram [2, W+4, H+3, nProj] (launch_pad2, edge halo)
   int st_dt;
+  const uint32_t* sino_max;  // non-null: sino holds P * 2^-e (pow2_exp); sums are rescaled
   float* out;        // [W, W, H]
   const float2* sc;
   int geom, W, H, nviews;

@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void pad3_kernel(const void* __restrict__ in, 
 template <typename S>
 __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, int in_dt,
                                                    S* __restrict__ out, int w, int h, int n,
-                                                   int mode) {
+                                                   int mode, const uint32_t* scale_max) {
+  const int sexp = pow2_exp(scale_max);
   const int PX = w + 2 * HALO, PR = h + 2 * HALO - 1;
   const size_t total = (size_t)PX * PR * n;
   for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
@@ -72,8 +73,8 @@ __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, 
         v[q] = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
       }
     }
-    out[2 * o] = Store<S>::from(v[0]);
-    out[2 * o + 1] = Store<S>::from(v[1]);
+    out[2 * o] = Store<S>::from(ldexpf(v[0], -sexp));
+    out[2 * o + 1] = Store<S>::from(ldexpf(v[1], -sexp));
   }
 }
 
@@ -85,8 +86,9 @@ template <typename S>
 __global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in, int in_dt,
                                                     int64_t in_sy, int64_t in_sz,
                                                     S* __restrict__ out, int nx, int ny, int nz,
-                                                    int mode) {
+                                                    int mode, const uint32_t* scale_max) {
   __shared__ float tile[32][33];
+  const int sexp = pow2_exp(scale_max);
   const int PX = nx + 2 * HALO, PY = ny + 2 * HALO, PZ = nz + 2 * HALO;
   const int pz0 = blockIdx.x * 32, px0 = blockIdx.y * 32, py = blockIdx.z;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in,
     float v = 0.f;
     if (ok && px0 + tx < PX && pz0 + lz < PZ)
       v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
-    tile[lz][tx] = v;
+    tile[lz][tx] = ldexpf(v, -sexp);
   }
   __syncthreads();
   for (int r = 0; r < 4; ++r) {
@@ -152,8 +154,9 @@ template <typename S>
 __global__ __launch_bounds__(256) void pad3q_kernel(const void* __restrict__ in, int in_dt,
                                                     int64_t in_sy, int64_t in_sz,
                                                     S* __restrict__ out, int nx, int ny, int nz,
-                                                    int mode) {
+                                                    int mode, const uint32_t* scale_max) {
   __shared__ float tile[2][32][33];
+  const int sexp = pow2_exp(scale_max);
   const int QX = nx + 2 * HALO - 1, QY = ny + 2 * HALO - 1, QZ = nz + 2 * HALO;
   const int qz0 = blockIdx.x * 32, qx0 = blockIdx.y * 32, qy = blockIdx.z;
   for (int e = threadIdx.x; e < 2 * 32 * 33; e += 256) {
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256) void pad3q_kernel(const void* __restrict__ in,
     } else if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
       v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
     }
-    tile[r][lz][lx] = v;
+    tile[r][lz][lx] = ldexpf(v, -sexp);
   }
   __syncthreads();
   const int tz = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -225,45 +228,47 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 }
 
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
-                       int mode, hipStream_t st) {
+                       int mode, hipStream_t st, const uint32_t* scale_max) {
   size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO - 1) * n;
   unsigned g = grid_for(total);
   if (st_dt == 0)
-    pad2_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode);
+    pad2_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode, scale_max);
   else if (st_dt == 1)
-    pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode);
+    pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode, scale_max);
   else
-    pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode);
+    pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
   return hipGetLastError();
 }
 
 hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
-                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st) {
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max) {
   const dim3 g((unsigned)((nz + 2 * HALO + 31) / 32), (unsigned)((nx + 2 * HALO - 1 + 31) / 32),
                (unsigned)(ny + 2 * HALO - 1));
   if (st_dt == 0)
-    pad3q_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode);
+    pad3q_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode, scale_max);
   else if (st_dt == 1)
     pad3q_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (_Float16*)out, nx, ny, nz,
-                                              mode);
+                                              mode, scale_max);
   else
     pad3q_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (bf16_t*)out, nx, ny, nz,
-                                            mode);
+                                            mode, scale_max);
   return hipGetLastError();
 }
 
 hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
-                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st) {
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max) {
   const dim3 g((unsigned)((nz + 2 * HALO + 31) / 32), (unsigned)((nx + 2 * HALO + 31) / 32),
                (unsigned)(ny + 2 * HALO));
   if (st_dt == 0)
-    pad3z_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode);
+    pad3z_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode, scale_max);
   else if (st_dt == 1)
     pad3z_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (_Float16*)out, nx, ny, nz,
-                                              mode);
+                                              mode, scale_max);
   else
     pad3z_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (bf16_t*)out, nx, ny, nz,
-                                            mode);
+                                            mode, scale_max);
   return hipGetLastError();
 }
 
