@@ -80,7 +80,7 @@ enum Slot {
   WS_POSTW,
   WS_PLAIN,
   WS_MUZ,      // z-fastest copy of the OSC volume
-  WS_MAXBITS,  // max|x| bits for the FP16 copies: [0] OSC ratio terms, [1] volume, [2] sinogram
+  WS_MAXBITS,  // max|x| bits for the FP16 copies: [1] volume, [2] sinogram
   WS_COUNT
 };
 
@@ -670,15 +670,16 @@ namespace {
 // One CUDAmex_oscIter pass over the subsets on the device state: mu_pad is
 // the zero-haloed volume, pre/post [W,H,nProj] are rewritten in place.
 // One CUDAmex_oscIter call on the padded f32 state mu_pad.  The subset
-// projections read a z-fastest copy of mu_pad in storage st, and the dual
-// backprojection reads the ratio terms in storage st (FP16: scaled by a
-// power of two from max(|pre|, |post|), which the ratio N/D cancels).
+// projections read a z-fastest copy of mu_pad in storage st (FP16: scaled by
+// a power of two from max|mu|), and the dual backprojection reads the ratio
+// terms in storage st (BF16 when st is FP16: their range exceeds FP16's).
 int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* post, int nProj,
                 int numSubsets, const int32_t* picks, const float2* sc, float sad, int geom,
                 int st) {
   const int cmax = (nProj + numSubsets - 1) / numSubsets;
   const size_t pv = (size_t)(W + 4) * (H + 4);
   const size_t es = (size_t)store_bytes(st);
+  const int pst = st == MEXCT_F16 ? MEXCT_BF16 : st;  // ratio-term storage
   // whole 32-bit words so the zero fill covers every 2-byte element
   const size_t pwords = (pv * std::max(cmax, 1) * es + 3) / 4;
   void *pp, *qp;
@@ -692,11 +693,6 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
   const int quad = st != MEXCT_F32 && qbytes < ((size_t)1 << 32);
   RC(c->get(WS_MUZ, quad ? qbytes : (size_t)MPXY * (H + 4) * es, &mz));
-  uint32_t* smax = nullptr;
-  if (st == MEXCT_F16) {
-    RC(max_bits(c, pre, (size_t)W * H * nProj, 0, &smax));
-    HIPCHK(launch_absmax(post, (size_t)W * H * nProj, smax, c->stream));
-  }
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
   for (int s = 0; s < numSubsets; ++s) {
     const int sel = picks[s];
@@ -720,7 +716,6 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
       f.post = post;
       f.pre_pad = pp;
       f.post_pad = qp;
-      f.sino_max = smax;
       f.sc = sc;
       f.geom = geom;
       f.W = W;
@@ -735,7 +730,7 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
     OSCBPLaunch b;
     b.pre_pad = pp;
     b.post_pad = qp;
-    b.st_dt = st;
+    b.st_dt = pst;
     b.mu_pad = mu_pad;
     b.sc = sc;
     b.geom = geom;

@@ -179,6 +179,9 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
         q[g] = bld4<S>(r, voff, soff);
         pz = pz + dz;
       }
+      // keep the group's loads ahead of its interpolation (the scheduler
+      // otherwise interleaves them and leaves one or two in flight)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         float c0, c1;  // rows j0 and j0+1, lerped between columns i0 and i0+1

@@ -37,9 +37,8 @@ struct FPZArgs {
   float* out;
   float* pre;          // OSC epilogue
   float* post;
-  void* pre_pad;       // storage type S, FP16 scaled by 2^-e (e: exponent of *sino_max)
+  void* pre_pad;       // storage OscPad<S>::T
   void* post_pad;
-  const uint32_t* sino_max;  // bits of max(|pre|, |post|), read when S is FP16
   const uint32_t* vol_max;   // non-null: the volume holds V * 2^-e (pow2_exp)
   const float2* sc;
   int W, H, rows, NX, NY, NZ;
@@ -204,25 +203,30 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
     const Tap8<S> t1 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
     dval = dval + interp_z<S, QUAD>(t0) * step;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
     t0 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    __builtin_amdgcn_sched_barrier(0);
     dval = dval + interp_z<S, QUAD>(t1) * step;
   }
   if (k < n) dval = dval + interp_z<S, QUAD>(t0) * step;
   return dval;
 }
 
-// The FP16 copies of the OSC ratio terms are scaled by an exact power of two
-// (counts reach 1e5, beyond FP16's 65504); the backprojection only forms the
-// ratio of two sums of them, which the common scale leaves unchanged.
+// Storage of the OSC ratio terms for a volume storage S.  They span the
+// counts times the line integral (up to ~1e5 and down to ~1e-5 and below),
+// beyond FP16's range, so an FP16 volume copy pairs with BF16 ratio terms.
 template <typename S>
-__device__ __forceinline__ float scale_pow2(float v, int e) {
-  if (__is_same(S, _Float16)) return ldexpf(v, -e);
-  return v;
-}
+struct OscPad {
+  typedef S T;
+};
+template <>
+struct OscPad<_Float16> {
+  typedef bf16_t T;
+};
 
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
 // written in place, plus zero-haloed copies for the dual backprojection.
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const size_t p = (size_t)view * a.W * a.H + (size_t)iy * a.W + ix;
   float preval = 0.f, postval = 0.f;
   size_t q = 0;
-  int sexp = 0;  // FP16 copies hold value * 2^-sexp, so that |value| <= 1
+  typedef typename OscPad<S>::T P;
   if (EPI == 0) {
     if (iy >= a.rows) {  // the reference's grid never writes these rows
       a.out[p] = 0.f;
@@ -251,10 +255,9 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     preval = a.pre[p];
     postval = a.post[p];
     q = (size_t)jv * a.PV + (size_t)(iy + HALO) * a.PW + (ix + HALO);
-    if (__is_same(S, _Float16)) (void)frexpf(__uint_as_float(*a.sino_max), &sexp);
     if (iy >= a.rows) {  // unprocessed rows keep their values
-      ((S*)a.pre_pad)[q] = Store<S>::from(scale_pow2<S>(preval, sexp));
-      ((S*)a.post_pad)[q] = Store<S>::from(scale_pow2<S>(postval, sexp));
+      ((P*)a.pre_pad)[q] = Store<P>::from(preval);
+      ((P*)a.post_pad)[q] = Store<P>::from(postval);
       return;
     }
   }
@@ -276,8 +279,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     const float npo = preval * expf(-dval) * dval;
     a.pre[p] = np;
     a.post[p] = npo;
-    ((S*)a.pre_pad)[q] = Store<S>::from(scale_pow2<S>(np, sexp));
-    ((S*)a.post_pad)[q] = Store<S>::from(scale_pow2<S>(npo, sexp));
+    ((P*)a.pre_pad)[q] = Store<P>::from(np);
+    ((P*)a.post_pad)[q] = Store<P>::from(npo);
   }
 }
 
@@ -357,7 +360,6 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.post = a.post;
   k.pre_pad = a.pre_pad;
   k.post_pad = a.post_pad;
-  k.sino_max = a.sino_max;
   k.vol_max = a.vol_max;
   k.first = a.first;
   k.stride = a.stride;

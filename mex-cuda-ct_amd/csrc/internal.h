@@ -68,14 +68,13 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 // OSC forward projection + ratio terms over the views first, first+stride, ...
 struct OSCFPLaunch {
   const void* mu_z;     // z-fastest volume of storage st_dt with a zero (border) halo
-  int st_dt;            // storage of mu_z and of pre_pad / post_pad
+  int st_dt;            // storage of mu_z; pre_pad / post_pad: the same, BF16 for F16
   int quad;             // mu_z layout: launch_pad3q (1) or launch_pad3z (0)
   const uint32_t* vol_max;  // non-null: mu_z holds mu * 2^-e (pow2_exp)
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
   void* pre_pad;        // [W+4,H+4,count] padded copies for the BP (zero halo)
   void* post_pad;
-  const uint32_t* sino_max;  // FP16: bits of max(|pre|,|post|); copies are scaled by 2^-e
   const float2* sc;
   int geom, W, H, nProj, first, stride, count;
   float sad;
