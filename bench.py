@@ -41,6 +41,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrivals per second")
     args = ap.parse_args()
@@ -71,6 +74,12 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     ctx = Context(0, stream=stream, order_with_torch=False)
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
 
     def run_step(storage, v, events=None):
         if events is not None:

@@ -177,8 +177,8 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
  * No reference counterpart: the gather-throughput microbenchmark behind the
  * projectors' "measured gather roofline".  Launches, on the context's stream,
  * a kernel that issues the load instructions of the forward projector
- * (MEXCT_PROBE_FP: per sample four 8-byte z-pair loads of the plain volume in
- * f32, one 16-byte load of the corner-quad volume in f16/bf16) or of the
+ * (MEXCT_PROBE_FP: per sample two 16-byte loads of the x-pair volume in f32,
+ * one 16-byte load of the corner-quad volume in f16/bf16) or of the
  * backprojector (MEXCT_PROBE_BP: per
  * update one 16-byte / 8-byte load of the row-paired sinogram) with ideal
  * coherence, over a buffer of the projector's size for W, H, nviews held in

@@ -254,6 +254,32 @@ int check_padded(int64_t a, int64_t b, int64_t c, const char* what) {
 
 int storage_of(const mexct_opts* o) { return o ? o->storage : MEXCT_F32; }
 
+// z-fastest FP volume layout for storage st (see fpz.hip): DUAL for f32,
+// QUAD for 2-byte storage while the copy stays below 4 GiB (32-bit buffer
+// offsets), else PLAIN below 2 GiB, else -1 (the x-fastest projector).
+int fp_layout(int64_t W, int64_t H, int st, size_t* bytes) {
+  const size_t es = (size_t)store_bytes(st);
+  const size_t plain = (size_t)(W + 4) * (W + 4) * (H + 4) * es;
+  const size_t wide = st == MEXCT_F32 ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
+                                      : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
+  if (wide < ((size_t)1 << 32)) {
+    *bytes = wide;
+    return st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
+  }
+  *bytes = plain;
+  return plain < ((size_t)1 << 31) ? LAYOUT_PLAIN : -1;
+}
+
+hipError_t build_fp_volume(int layout, const void* in, int in_dt, int64_t sy, int64_t sz,
+                           void* out, int st, int64_t W, int64_t H, int mode, hipStream_t s,
+                           const uint32_t* scale) {
+  if (layout == LAYOUT_DUAL)
+    return launch_pad3d(in, in_dt, sy, sz, out, st, (int)W, (int)W, (int)H, mode, s, scale);
+  if (layout == LAYOUT_QUAD)
+    return launch_pad3q(in, in_dt, sy, sz, out, st, (int)W, (int)W, (int)H, mode, s, scale);
+  return launch_pad3z(in, in_dt, sy, sz, out, st, (int)W, (int)W, (int)H, mode, s, scale);
+}
+
 // Device word holding max|x| of n floats (as float bits), slot k of
 // WS_MAXBITS, for the power-of-two scaling of FP16 copies.
 int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out) {
@@ -400,30 +426,22 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   DevGuard g(c->device);
   void* pad;
   const size_t pbytes = (size_t)(W + 4) * (W + 4) * (H + 4) * store_bytes(st);
-  // corner quads for 2-byte storage (one 16-byte load per sample); f32 keeps
-  // the plain layout, whose four 8-byte pair loads draw half the cache lines
-  // of two 16-byte quad loads and run faster
-  const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * store_bytes(st);
-  const int quad = st != MEXCT_F32 && qbytes < ((size_t)1 << 32);
+  size_t lbytes = 0;
+  const int layout = fp_layout(W, H, st, &lbytes);
   const float2* sc;
-  if (quad || pbytes < ((size_t)1 << 31)) {
-    // z-fastest haloed volume (corner quads when they fit 32-bit offsets),
-    // one wave per 64 detector rows of a column
-    RC(c->get(WS_VOL, quad ? qbytes : pbytes, &pad));
+  if (layout >= 0) {
+    // z-fastest haloed volume, one wave per 64 detector rows of a column
+    RC(c->get(WS_VOL, lbytes, &pad));
     uint32_t* vmax = nullptr;
     if (st == MEXCT_F16 && vol_dtype == MEXCT_F32)
       RC(max_bits(c, (const float*)vol, (size_t)W * W * H, 1, &vmax));
-    if (quad)
-      HIPCHK(launch_pad3q(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
-                          c->stream, vmax));
-    else
-      HIPCHK(launch_pad3z(vol, vol_dtype, W, W * W, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP,
-                          c->stream, vmax));
+    HIPCHK(build_fp_volume(layout, vol, vol_dtype, W, W * W, pad, st, W, H, ADDR_CLAMP, c->stream,
+                           vmax));
     RC(c->sincos_table(angles, nProj, &sc));
     FPZLaunch z;
     z.vol = pad;
     z.st_dt = st;
-    z.quad = quad;
+    z.layout = layout;
     z.vol_max = vmax;
     z.out = out;
     z.sc = sc;
@@ -690,9 +708,9 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   HIPCHK(launch_fill((float*)qp, 0.f, pwords, c->stream));
   void* mz;
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
-  const size_t qbytes = (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
-  const int quad = st != MEXCT_F32 && qbytes < ((size_t)1 << 32);
-  RC(c->get(WS_MUZ, quad ? qbytes : (size_t)MPXY * (H + 4) * es, &mz));
+  size_t lbytes = 0;
+  const int layout = fp_layout(W, H, st, &lbytes);  // PLAIN at least: the state is < 2 GiB
+  RC(c->get(WS_MUZ, lbytes, &mz));
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
   for (int s = 0; s < numSubsets; ++s) {
     const int sel = picks[s];
@@ -701,16 +719,12 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
       // the subset projects the volume as left by the previous update
       uint32_t* vmax = nullptr;
       if (st == MEXCT_F16) RC(max_bits(c, mu_pad, (size_t)MPXY * (H + 4), 1, &vmax));
-      if (quad)
-        HIPCHK(launch_pad3q(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
-                            c->stream, vmax));
-      else
-        HIPCHK(launch_pad3z(interior, MEXCT_F32, MPX, MPXY, mz, st, W, W, H, ADDR_BORDER,
-                            c->stream, vmax));
+      HIPCHK(build_fp_volume(layout, interior, MEXCT_F32, MPX, MPXY, mz, st, W, H, ADDR_BORDER,
+                             c->stream, vmax));
       OSCFPLaunch f;
       f.mu_z = mz;
       f.st_dt = st;
-      f.quad = quad;
+      f.layout = layout;
       f.vol_max = vmax;
       f.pre = pre;
       f.post = post;
@@ -941,7 +955,7 @@ int mexct_probe_gather_dev(mexct_ctx* c, int32_t storage, int64_t W, int64_t H, 
     return fail(MEXCT_EINVAL, "bad probe arguments");
   const size_t es = (size_t)store_bytes(storage);
   const size_t bytes = pattern != MEXCT_PROBE_FP ? (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews
-                       : storage == MEXCT_F32  ? (size_t)(W + 4) * (W + 4) * (H + 4) * es
+                       : storage == MEXCT_F32  ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
                                                : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
   if (bytes >= ((size_t)1 << 32)) return fail(MEXCT_EINVAL, "probe buffer too large");
   DevGuard g(c->device);

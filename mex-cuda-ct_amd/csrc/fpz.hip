@@ -11,25 +11,30 @@
 // about one line per lane.  A 256-thread workgroup holds 4 adjacent
 // columns.  Loads are 32-bit buffer offsets.
 //
-// Two z-fastest layouts.  Plain (f32): four (z, z+1) pair loads per sample
-// at the (j+1), (i+1), (i+1,j+1) strides in scalar offsets.  QUAD (f16 and
-// bf16): element (x,y,z) holds the four xy corners {V(x,y,z), V(x,y+1,z),
-// V(x+1,y,z), V(x+1,y+1,z)}, so the eight taps of a sample are two
-// consecutive elements, one 16-byte load, instead of four 4-byte pair loads
-// (the load unit takes about as long per wave for 4, 8 or 16 bytes per lane
-// when the data is cached).  The quad copy is 4x the plain one, which
-// MI355X's HBM holds easily; volumes whose quad copy would pass 4 GiB use
-// the plain layout.  For f32 the quad layout loses: its two 16-byte loads
-// draw twice the cache lines of the four pair loads.
-//
-// Arithmetic follows the oracle operation by operation (ray setup and clip
-// in ray.h; sample coordinate p - b, then - 0.5; lerps x, then y, then z),
-// so the projections are the reference's float results.
+// Three z-fastest layouts, each haloed by 2 on every side:
+//   PLAIN: V(x,y,z); per sample four (z, z+1) pair loads at the (j+1),
+//     (i+1), (i+1,j+1) strides in scalar offsets.
+//   DUAL (f32): element (x,y,z) = {V(x,y,z), V(x+1,y,z)}; the two elements
+//     at z, z+1 of rows j and j+1 are two 16-byte loads per sample.
+//   QUAD (f16, bf16): element (x,y,z) = {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
+//     V(x+1,y+1,z)}; one 16-byte load per sample.
+// The load unit spends about the same time on a wave's load whether its
+// lanes fetch 4, 8 or 16 bytes when the data is cached, and L2 supplies
+// about 64 bytes per clock per CU, so the best layout issues the fewest
+// loads without drawing more cache lines: DUAL halves the loads of PLAIN
+// at the same lines (the 8-byte lane stride of two consecutive z elements
+// spans what the four pair loads span), QUAD quarters them for 2-byte data
+// but doubles the lines for f32.  The DUAL and QUAD copies take 2x and 4x
+// the plain one, which MI355X's HBM holds easily; volumes whose copy would
+// pass 4 GiB use PLAIN.
+
 #include "common.h"
 #include "internal.h"
 #include "ray.h"
 
 namespace mexct {
+
+constexpr uint32_t kViewChunk = 16;
 
 struct FPZArgs {
   const void* vol;     // z-fastest haloed volume
@@ -48,6 +53,7 @@ struct FPZArgs {
   int first, stride;  // view = first + j*stride
   int PW, PV;         // OSC padded copies
   uint32_t n_ug, n_vg, n_logical;
+  uint32_t nv;  // views of the launch
 };
 
 template <typename S>
@@ -121,7 +127,14 @@ __device__ __forceinline__ void quad_ld<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32
   t.p11 = make_float2(__uint_as_float(q.y & 0xffff0000u), __uint_as_float(q.w & 0xffff0000u));
 }
 
-template <typename S, int QUAD>
+// Four taps {(i,j,k), (i+1,j,k), (i,j,k+1), (i+1,j,k+1)} of one f32 DUAL row.
+__device__ __forceinline__ float4 dual_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v q = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  return make_float4(q.x, q.y, q.z, q.w);
+}
+
+template <typename S, int LAYOUT>
 __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a,
                                            const R3& cur, float fnx, float fny, float fnz,
                                            int base) {
@@ -138,8 +151,16 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
   const int kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
   // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
   const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
-  if (QUAD) {
+  if constexpr (LAYOUT == LAYOUT_QUAD) {
     quad_ld<S>(r, e * (uint32_t)(4 * sizeof(S)), t);
+  } else if constexpr (LAYOUT == LAYOUT_DUAL && __is_same(S, float)) {
+    const uint32_t voff = e * 8u;
+    const float4 q0 = dual_ld(r, voff, 0);         // row j
+    const float4 q1 = dual_ld(r, voff, a.off_j);   // row j+1
+    t.p00 = make_float2(q0.x, q0.z);
+    t.p10 = make_float2(q0.y, q0.w);
+    t.p01 = make_float2(q1.x, q1.z);
+    t.p11 = make_float2(q1.y, q1.w);
   } else {
     const uint32_t voff = e * (uint32_t)sizeof(S);
     if constexpr (__is_same(S, _Float16)) {
@@ -157,11 +178,11 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
   return t;
 }
 
-template <typename S, int QUAD>
+template <typename S, int LAYOUT>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
   if constexpr (__is_same(S, _Float16)) {
-    if (QUAD) {
+    if (LAYOUT == LAYOUT_QUAD) {
       c00 = lerp_h<0>(t.w0, t.w1, t.ax);
       c10 = lerp_h<1>(t.w0, t.w1, t.ax);
       c01 = lerp_h<0>(t.w2, t.w3, t.ax);
@@ -186,7 +207,7 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
 // One ray through the z-fastest volume, software-pipelined: the loads of
 // sample k+1 are issued before sample k is interpolated.  A fetch past the
 // last sample reads a clamped, in-range corner and is discarded.
-template <typename S, int STEP2, int QUAD>
+template <typename S, int STEP2, int LAYOUT>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
@@ -196,23 +217,23 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
   if (n <= 0) return dval;
   // two register sets in ping-pong: while one sample is interpolated the
   // next sample's four loads are in flight
-  Tap8<S> t0 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+  Tap8<S> t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
   int k = 0;
   for (; k + 1 < n; k += 2) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    const Tap8<S> t1 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    const Tap8<S> t1 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
-    dval = dval + interp_z<S, QUAD>(t0) * step;
+    dval = dval + interp_z<S, LAYOUT>(t0) * step;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    t0 = fetch_z<S, QUAD>(r, a, cur, fnx, fny, fnz, base);
+    t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);
-    dval = dval + interp_z<S, QUAD>(t1) * step;
+    dval = dval + interp_z<S, LAYOUT>(t1) * step;
   }
-  if (k < n) dval = dval + interp_z<S, QUAD>(t0) * step;
+  if (k < n) dval = dval + interp_z<S, LAYOUT>(t0) * step;
   return dval;
 }
 
@@ -230,14 +251,23 @@ struct OscPad<_Float16> {
 
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
 // written in place, plus zero-haloed copies for the dual backprojection.
-template <int GEOM, typename S, int EPI, int QUAD>
+template <int GEOM, typename S, int EPI, int LAYOUT>
 __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
+  // Block order: (chunk of kViewChunk views, column group, view in chunk,
+  // row group), each XCD taking a contiguous range.  The blocks an XCD runs
+  // together then march the same 4 detector columns of consecutive views,
+  // whose rays lie within a few voxels of each other, so the volume lines
+  // they gather are shared in that XCD's L2 (view-major order: 47.8 ms ->
+  // 32.4 ms at 384^3 x 512 cone, f32).
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
   const uint32_t vg = b % a.n_vg;
-  const uint32_t rest = b / a.n_vg;
+  uint32_t rest = b / a.n_vg;
+  const uint32_t vw = rest % kViewChunk;
+  rest /= kViewChunk;
   const uint32_t ug = rest % a.n_ug;
-  const uint32_t jv = rest / a.n_ug;
+  const uint32_t jv = (rest / a.n_ug) * kViewChunk + vw;
+  if (jv >= a.nv) return;
   const int ix = (int)ug * 4 + (int)(threadIdx.x >> 6);
   const int iy = (int)vg * 64 + (int)(threadIdx.x & 63);
   if (ix >= a.W || iy >= a.H) return;
@@ -271,7 +301,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = ldexpf(march_z<S, STEP2, QUAD>(r, a, cur, inc, n), pow2_exp(a.vol_max));
+  const float dval = ldexpf(march_z<S, STEP2, LAYOUT>(r, a, cur, inc, n), pow2_exp(a.vol_max));
   if (EPI == 0) {
     a.out[p] = dval;
   } else {
@@ -284,7 +314,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   }
 }
 
-static FPZArgs make_zargs(const void* vol, int st_dt, int quad, const float2* sc, int W, int H,
+static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* sc, int W, int H,
                           int nviews, float sad) {
   FPZArgs k{};
   k.vol = vol;
@@ -296,11 +326,14 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int quad, const float2* sc
   k.NX = W;
   k.NY = W;
   k.NZ = H;
-  // quad: corner elements for x, y in [-2, N] (N+3 of them); plain: [-2, N+1]
-  k.PY = quad ? W + 2 * HALO - 1 : W + 2 * HALO;
+  // element counts: x and y in [-2, N+1] (N+4), except the element of the
+  // corner pair or quad, whose base runs over [-2, N] (N+3)
+  const int PX = layout == LAYOUT_PLAIN ? W + 2 * HALO : W + 2 * HALO - 1;
+  k.PY = layout == LAYOUT_QUAD ? W + 2 * HALO - 1 : W + 2 * HALO;
   k.PZ = H + 2 * HALO;
-  const uint32_t es = (st_dt == 0 ? 4u : 2u) * (quad ? 4u : 1u);
-  k.vol_bytes = (uint32_t)((size_t)k.PY * k.PY * k.PZ * es);
+  const uint32_t es = (st_dt == 0 ? 4u : 2u) *
+                      (layout == LAYOUT_QUAD ? 4u : layout == LAYOUT_DUAL ? 2u : 1u);
+  k.vol_bytes = (uint32_t)((size_t)PX * k.PY * k.PZ * es);
   k.off_j = (uint32_t)k.PZ * es;
   k.off_i = (uint32_t)k.PY * (uint32_t)k.PZ * es;
   k.off_ij = k.off_i + k.off_j;
@@ -315,47 +348,49 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int quad, const float2* sc
   k.stride = 1;
   k.n_ug = (uint32_t)((W + 3) / 4);
   k.n_vg = (uint32_t)((H + 63) / 64);
-  k.n_logical = k.n_ug * k.n_vg * (uint32_t)nviews;
+  k.nv = (uint32_t)nviews;
+  const uint32_t nvp = (k.nv + kViewChunk - 1) / kViewChunk * kViewChunk;
+  k.n_logical = k.n_ug * k.n_vg * nvp;
   return k;
 }
 
-template <int EPI, typename S, int QUAD>
-static void fpz_dispatch_q(const FPZArgs& k, int geom, hipStream_t st) {
+template <int EPI, typename S, int LAYOUT>
+static void fpz_dispatch_l(const FPZArgs& k, int geom, hipStream_t st) {
   const unsigned g = (k.n_logical + 7u) & ~7u;
   switch (geom) {
-    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
-    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
-    default: fpz_kernel<GEOM_CONE, S, EPI, QUAD><<<g, 256, 0, st>>>(k); break;
+    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
+    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
+    default: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
   }
 }
 
+// Layouts built: PLAIN for every storage, DUAL for f32, QUAD for 2-byte.
 template <int EPI, typename S>
-static void fpz_dispatch(const FPZArgs& k, int geom, int quad, hipStream_t st) {
+static void fpz_dispatch(const FPZArgs& k, int geom, int layout, hipStream_t st) {
   if constexpr (sizeof(S) == 2) {
-    if (quad) {
-      fpz_dispatch_q<EPI, S, 1>(k, geom, st);
-      return;
-    }
+    if (layout == LAYOUT_QUAD) return fpz_dispatch_l<EPI, S, LAYOUT_QUAD>(k, geom, st);
+  } else {
+    if (layout == LAYOUT_DUAL) return fpz_dispatch_l<EPI, S, LAYOUT_DUAL>(k, geom, st);
   }
-  fpz_dispatch_q<EPI, S, 0>(k, geom, st);
+  fpz_dispatch_l<EPI, S, LAYOUT_PLAIN>(k, geom, st);
 }
 
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.vol, a.st_dt, a.quad, a.sc, a.W, a.H, a.nviews, a.sad);
+  FPZArgs k = make_zargs(a.vol, a.st_dt, a.layout, a.sc, a.W, a.H, a.nviews, a.sad);
   k.out = a.out;
   k.vol_max = a.vol_max;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
-    fpz_dispatch<0, float>(k, a.geom, a.quad, st);
+    fpz_dispatch<0, float>(k, a.geom, a.layout, st);
   else if (a.st_dt == 1)
-    fpz_dispatch<0, _Float16>(k, a.geom, a.quad, st);
+    fpz_dispatch<0, _Float16>(k, a.geom, a.layout, st);
   else
-    fpz_dispatch<0, bf16_t>(k, a.geom, a.quad, st);
+    fpz_dispatch<0, bf16_t>(k, a.geom, a.layout, st);
   return hipGetLastError();
 }
 
 hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.quad, a.sc, a.W, a.H, a.count, a.sad);
+  FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.layout, a.sc, a.W, a.H, a.count, a.sad);
   k.pre = a.pre;
   k.post = a.post;
   k.pre_pad = a.pre_pad;
@@ -367,11 +402,11 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.PV = (a.W + 2 * HALO) * (a.H + 2 * HALO);
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
-    fpz_dispatch<1, float>(k, a.geom, a.quad, st);
+    fpz_dispatch<1, float>(k, a.geom, a.layout, st);
   else if (a.st_dt == 1)
-    fpz_dispatch<1, _Float16>(k, a.geom, a.quad, st);
+    fpz_dispatch<1, _Float16>(k, a.geom, a.layout, st);
   else
-    fpz_dispatch<1, bf16_t>(k, a.geom, a.quad, st);
+    fpz_dispatch<1, bf16_t>(k, a.geom, a.layout, st);
   return hipGetLastError();
 }
 

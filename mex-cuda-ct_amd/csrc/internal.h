@@ -19,6 +19,12 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 hipError_t launch_pad3z(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
                         int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
                         const uint32_t* scale_max = nullptr);
+// z-fastest x-pair ("dual") copy: element (x,y,z), x in [-2,n], y, z in
+// [-2, n+1] / [-2, nz+1], at ((x+2)*(ny+4) + y+2)*(nz+4) + z+2 holds
+// {V(x,y,z), V(x+1,y,z)} of the haloed volume V.
+hipError_t launch_pad3d(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max = nullptr);
 // z-fastest corner-quad copy: element (x,y,z), x,y in [-2,n], z in [-2,nz+1], at
 // ((x+2)*(ny+3) + y+2)*(nz+4) + z+2 holds {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
 // V(x+1,y+1,z)} of the haloed volume V (same input addressing as launch_pad3z).
@@ -53,10 +59,13 @@ struct FPLaunch {
 hipError_t launch_fp(const FPLaunch& a, hipStream_t st);
 
 // standard-geometry FP on a z-fastest haloed volume (launch_pad3z layout)
+// z-fastest volume layouts of the standard-geometry FP (see fpz.hip)
+enum { LAYOUT_PLAIN = 0, LAYOUT_QUAD = 1, LAYOUT_DUAL = 2 };
+
 struct FPZLaunch {
-  const void* vol;  // launch_pad3q output (quad = 1) or launch_pad3z output (quad = 0)
+  const void* vol;  // launch_pad3z (PLAIN), launch_pad3d (DUAL) or launch_pad3q (QUAD) output
   int st_dt;
-  int quad;
+  int layout;
   const uint32_t* vol_max;  // non-null: vol holds V * 2^-e (pow2_exp); sums are rescaled
   float* out;
   const float2* sc;
@@ -69,7 +78,7 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 struct OSCFPLaunch {
   const void* mu_z;     // z-fastest volume of storage st_dt with a zero (border) halo
   int st_dt;            // storage of mu_z; pre_pad / post_pad: the same, BF16 for F16
-  int quad;             // mu_z layout: launch_pad3q (1) or launch_pad3z (0)
+  int layout;           // mu_z layout (LAYOUT_*)
   const uint32_t* vol_max;  // non-null: mu_z holds mu * 2^-e (pow2_exp)
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place

@@ -120,6 +120,45 @@ __global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in,
   }
 }
 
+// x-pair copy (see launch_pad3d).  A block covers 32 z x 32 x of one y: the
+// (z, x..x+32) tile of input row y is staged in LDS (read along x), then
+// each lane writes one pair element (along z).
+template <typename S>
+__global__ __launch_bounds__(256) void pad3d_kernel(const void* __restrict__ in, int in_dt,
+                                                    int64_t in_sy, int64_t in_sz,
+                                                    S* __restrict__ out, int nx, int ny, int nz,
+                                                    int mode, const uint32_t* scale_max) {
+  __shared__ float tile[32][33];
+  const int sexp = pow2_exp(scale_max);
+  const int DX = nx + 2 * HALO - 1, DY = ny + 2 * HALO, DZ = nz + 2 * HALO;
+  const int dz0 = blockIdx.x * 32, dx0 = blockIdx.y * 32, dy = blockIdx.z;
+  for (int e = threadIdx.x; e < 32 * 33; e += 256) {
+    const int lz = e / 33, lx = e - lz * 33;
+    int i = dx0 + lx - HALO, j = dy - HALO, k = dz0 + lz - HALO;
+    float v = 0.f;
+    if (mode == ADDR_CLAMP) {
+      i = clampi(i, 0, nx - 1);
+      j = clampi(j, 0, ny - 1);
+      k = clampi(k, 0, nz - 1);
+      v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
+    } else if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+      v = load_any(in, in_dt, (size_t)i + (size_t)in_sy * j + (size_t)in_sz * k);
+    }
+    tile[lz][lx] = ldexpf(v, -sexp);
+  }
+  __syncthreads();
+  const int tz = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int p = 0; p < 4; ++p) {
+    const int lx = ty + 8 * p;
+    const int dx = dx0 + lx, dz = dz0 + tz;
+    if (dx < DX && dz < DZ) {
+      const size_t e = ((size_t)dx * DY + dy) * DZ + dz;
+      out[2 * e] = Store<S>::from(tile[tz][lx]);
+      out[2 * e + 1] = Store<S>::from(tile[tz][lx + 1]);
+    }
+  }
+}
+
 // Corner-quad copy (see launch_pad3q).  A block covers 32 z x 32 x of one
 // y: the (z, x..x+32) tiles of input rows y and y+1 are staged in LDS (read
 // along x), then each lane writes one quad element (along z).
@@ -237,6 +276,23 @@ hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, i
     pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode, scale_max);
   else
     pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad3d(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
+                        int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
+                        const uint32_t* scale_max) {
+  const dim3 g((unsigned)((nz + 2 * HALO + 31) / 32), (unsigned)((nx + 2 * HALO - 1 + 31) / 32),
+               (unsigned)(ny + 2 * HALO));
+  if (st_dt == 0)
+    pad3d_kernel<float><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (float*)out, nx, ny, nz, mode,
+                                           scale_max);
+  else if (st_dt == 1)
+    pad3d_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (_Float16*)out, nx, ny, nz,
+                                              mode, scale_max);
+  else
+    pad3d_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, in_sy, in_sz, (bf16_t*)out, nx, ny, nz,
+                                            mode, scale_max);
   return hipGetLastError();
 }
 

@@ -5,11 +5,10 @@
 // the memory pipeline's ceiling for that pattern (the "measured gather
 // roofline"):
 //
-//   FP: f32, the plain z-fastest volume, per sample four (z, z+1) 8-byte
-//       pair loads at the (i,j), (i,j+1), (i+1,j), (i+1,j+1) corners;
-//       f16/bf16, the corner-quad volume, one 16-byte load per sample.  The
-//       64 lanes of a wave hold 64 consecutive z, each wave one y column,
-//       and every lane steps along x.
+//   FP: f32, the x-pair (DUAL) z-fastest volume, per sample two 16-byte
+//       loads (rows j and j+1); f16/bf16, the corner-quad volume, one
+//       16-byte load per sample.  The 64 lanes of a wave hold 64
+//       consecutive z, each wave one y column, and every lane steps along x.
 //   BP: the row-paired haloed sinogram; per update one 16-byte (f32) or
 //       8-byte (f16/bf16) load of two adjacent pair elements.  The 64 lanes
 //       of a wave hold 64 consecutive detector columns; every lane walks
@@ -34,8 +33,8 @@ __device__ __forceinline__ float probe_ld(__amdgpu_buffer_rsrc_t r, uint32_t vof
 }
 
 // FP pattern.  Element (x, y, z) at ((x+2)*QY + y+2)*QZ + z+2, E bytes each.
-// QUAD: one 16-byte load per sample; plain: four 8-byte pair loads at the
-// corner strides (y+1: QZ*E, x+1: QY*QZ*E).
+// QUAD: one 16-byte load per sample; DUAL: two 16-byte loads, rows y and
+// y+1 (QZ*E apart).
 template <int QUAD>
 __global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t bytes,
                                                        uint32_t E, int QY, int QZ, int ncols,
@@ -49,14 +48,8 @@ __global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t
   const uint32_t oj = (uint32_t)QZ * E;
   float s = 0.f;
   for (int k = 0; k < nsteps; ++k) {
-    if (QUAD) {
-      s = s + probe_ld<16>(r, voff);
-    } else {
-      s = s + probe_ld<8>(r, voff);
-      s = s + probe_ld<8>(r, voff + oj);
-      s = s + probe_ld<8>(r, voff + step);
-      s = s + probe_ld<8>(r, voff + step + oj);
-    }
+    s = s + probe_ld<16>(r, voff);
+    if (!QUAD) s = s + probe_ld<16>(r, voff + oj);
     voff += step;
   }
   if (s == 1234.5f) sink[threadIdx.x] = s;  // keeps the loads; never true for the probe data
@@ -87,9 +80,10 @@ hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, i
   const uint32_t es = st_dt == 0 ? 4u : 2u;
   if (pattern == 0) {
     const int quad = st_dt != 0;
+    const int QX = W + 2 * HALO - 1;
     const int QY = W + 2 * HALO - (quad ? 1 : 0), QZ = H + 2 * HALO;
-    const uint32_t E = (quad ? 4u : 1u) * es;
-    const uint32_t bytes = (uint32_t)((size_t)QY * QY * QZ * E);
+    const uint32_t E = (quad ? 4u : 2u) * es;
+    const uint32_t bytes = (uint32_t)((size_t)QX * QY * QZ * E);
     const dim3 g((unsigned)((W + 3) / 4), (unsigned)((H + 63) / 64 * nviews));
     if (quad)
       fp_probe_kernel<1><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
