@@ -257,14 +257,17 @@ int storage_of(const mexct_opts* o) { return o ? o->storage : MEXCT_F32; }
 // z-fastest FP volume layout for storage st (see fpz.hip): DUAL for f32,
 // QUAD for 2-byte storage while the copy stays below 4 GiB (32-bit buffer
 // offsets), else PLAIN below 2 GiB, else -1 (the x-fastest projector).
+// Measured at 384^3 x 512 cone: f32 PLAIN 47.6 ms, DUAL 32.3 ms; f16 PLAIN
+// 50.8 ms, DUAL 24.1 ms, QUAD 20.1 ms.
 int fp_layout(int64_t W, int64_t H, int st, size_t* bytes) {
   const size_t es = (size_t)store_bytes(st);
   const size_t plain = (size_t)(W + 4) * (W + 4) * (H + 4) * es;
-  const size_t wide = st == MEXCT_F32 ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
-                                      : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
-  if (wide < ((size_t)1 << 32)) {
+  const int want = st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
+  const size_t wide = want == LAYOUT_DUAL ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
+                                          : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
+  if (want != LAYOUT_PLAIN && wide < ((size_t)1 << 32)) {
     *bytes = wide;
-    return st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
+    return want;
   }
   *bytes = plain;
   return plain < ((size_t)1 << 31) ? LAYOUT_PLAIN : -1;

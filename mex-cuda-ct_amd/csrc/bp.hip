@@ -56,6 +56,8 @@ __device__ __forceinline__ float4 bld4<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_
                      __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
 }
 
+constexpr uint32_t kSuper = 4;
+
 struct BPKArgs {
   const void* sino;     // row-paired haloed sinogram
   uint32_t sino_bytes;  // buffer range (the launcher guarantees < 2^31)
@@ -110,9 +112,15 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
   const uint32_t zr = b % a.nz;
+  // tiles in 4x4 super-tiles (row-major inside), so that the blocks an XCD
+  // runs together cover a compact region whose view footprints overlap
   const uint32_t tile = b / a.nz;
-  const int ix = (int)(tile % a.tiles_x) * 8 + (int)(threadIdx.x & 7);
-  const int iy = (int)(tile / a.tiles_x) * 8 + (int)(threadIdx.x >> 3);
+  const uint32_t st = tile / (kSuper * kSuper), in = tile % (kSuper * kSuper);
+  const uint32_t sbx = (a.tiles_x + kSuper - 1) / kSuper;
+  const uint32_t tx = (st % sbx) * kSuper + in % kSuper;
+  const uint32_t ty = (st / sbx) * kSuper + in / kSuper;
+  const int ix = (int)tx * 8 + (int)(threadIdx.x & 7);
+  const int iy = (int)ty * 8 + (int)(threadIdx.x >> 3);
   if (ix >= a.W || iy >= a.W) return;
   const int z0 = (int)zr * ZR;
   const int zc = min(ZR, a.H - z0);
@@ -159,10 +167,43 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
     const float colf = (fi0 + (float)HALO) * (float)es + (float)(HALO * row_bytes);
     const float rbf = (float)row_bytes;
     const uint32_t colb = (uint32_t)((int)fi0 + HALO) * es;  // integer form (large slices)
-    pz = advance_rows<GEOM>(pz, dz, z0);
     // groups of 8 z-voxels: offsets and loads of the whole group are issued
     // before its interpolation
     constexpr int G = 8;
+    if constexpr (sizeof(S) == 2) {
+      // 2-byte storage: the row coordinate starts at z0 directly and steps in
+      // texel coordinates (no replay of the reference's running sum); its
+      // rounding differs by a few ulps, far below the storage rounding
+      float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
+      const uint32_t cb = colb + (uint32_t)(HALO * row_bytes);
+#pragma unroll
+      for (int m0 = 0; m0 < ZR; m0 += G) {
+        float az[G];
+        typename Taps4<S>::T q[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          az[g] = __builtin_amdgcn_fractf(zb);
+          const int j0 = clamp_m2(cvt_flr(zb), a.H);
+          q[g] = bld4<S>(r, (uint32_t)__mul24(j0, row_bytes) + cb, soff);
+          zb = zb + dz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          float c0, c1;
+          if constexpr (__is_same(S, _Float16)) {
+            c0 = lerp_h<0>(q[g].x, q[g].y, ax);
+            c1 = lerp_h<1>(q[g].x, q[g].y, ax);
+          } else {
+            c0 = lerp(q[g].x, q[g].z, ax);
+            c1 = lerp(q[g].y, q[g].w, ax);
+          }
+          acc[m0 + g] = __builtin_fmaf(UU, lerp(c0, c1, az[g]), acc[m0 + g]);
+        }
+      }
+      continue;
+    }
+    pz = advance_rows<GEOM>(pz, dz, z0);
 #pragma unroll
     for (int m0 = 0; m0 < ZR; m0 += G) {
       float az[G];
@@ -249,7 +290,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.SR = a.sad;
   k.tiles_x = (uint32_t)((a.W + 7) / 8);
   k.nz = (uint32_t)((a.H + zr - 1) / zr);
-  k.n_logical = k.tiles_x * k.tiles_x * k.nz;
+  const uint32_t tpad = (k.tiles_x + kSuper - 1) / kSuper * kSuper;
+  k.n_logical = tpad * tpad * k.nz;
   if (k.n_logical == 0) return hipSuccess;
   const size_t view_bytes = (size_t)k.PV * es2;
   const int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);

@@ -85,6 +85,28 @@ __device__ __forceinline__ float load_any(const void* p, int dtype, size_t i) {
   return bf16_to_f32(((const uint16_t*)p)[i]);
 }
 
+// lerp(lo, hi, t) of the two f16 halves of one dword A, in float (the same
+// identity as lerp_h).
+__device__ __forceinline__ float lerp_x(uint32_t A, float t) {
+  float d, c;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%1 op_sel:[1,0,0] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(A));
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(c) : "v"(t), "v"(d), "v"(A));
+  return c;
+}
+
+// floor(x) as an int in one instruction
+__device__ __forceinline__ int cvt_flr(float x) {
+  int i;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+  return i;
+}
+// median(i, -2, hi): the corner index clamped to [-2, hi]
+__device__ __forceinline__ int clamp_m2(int i, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, -2, %2" : "=v"(r) : "v"(i), "s"(hi));
+  return r;
+}
+
 // FP16 copies of f32 data are stored as v * 2^-e, e the binary exponent of
 // max|v| (max = m * 2^e, m in [0.5, 1)), so that they use FP16's range
 // (subnormals below 6.1e-5, overflow above 65504) however the data is

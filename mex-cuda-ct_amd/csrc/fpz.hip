@@ -83,6 +83,8 @@ struct Tap8 {
 //   plain layout: w0..w3 = (z, z+1) pairs of (i,j), (i,j+1), (i+1,j), (i+1,j+1)
 //   quad layout:  w0 = {(i,j), (i,j+1)} and w1 = {(i+1,j), (i+1,j+1)} at z,
 //                 w2, w3 the same at z+1
+//   dual layout:  w0 = {(i,j), (i+1,j)} at z, w1 the same at z+1, w2, w3
+//                 the same for row j+1
 template <>
 struct Tap8<_Float16> {
   uint32_t w0, w1, w2, w3;
@@ -134,25 +136,49 @@ __device__ __forceinline__ float4 dual_ld(__amdgpu_buffer_rsrc_t r, uint32_t vof
   return make_float4(q.x, q.y, q.z, q.w);
 }
 
+// 2-byte storage marches in texel coordinates (LEAN): the position is kept
+// as p - b - 0.5 and stepped by the same increment, and the weight and
+// corner come from v_fract / v_cvt_flr.  Sample positions then differ from
+// the reference's float rounding by a few ulps, far below the storage
+// rounding, and each sample costs 9 VALU less.
 template <typename S, int LAYOUT>
 __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a,
                                            const R3& cur, float fnx, float fny, float fnz,
                                            int base) {
   Tap8<S> t;
-  const float xb = (cur.x - a.bx) - 0.5f;
-  const float yb = (cur.y - a.by) - 0.5f;
-  const float zb = (cur.z - a.bz) - 0.5f;
-  const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
-  t.ax = xb - fx;
-  t.ay = yb - fy;
-  t.az = zb - fz;
-  const int i = (int)__builtin_amdgcn_fmed3f(fx, -2.f, fnx);
-  const int j = (int)__builtin_amdgcn_fmed3f(fy, -2.f, fny);
-  const int kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
+  int i, j, kk;
+  if constexpr (sizeof(S) == 2) {
+    t.ax = __builtin_amdgcn_fractf(cur.x);
+    t.ay = __builtin_amdgcn_fractf(cur.y);
+    t.az = __builtin_amdgcn_fractf(cur.z);
+    i = clamp_m2(cvt_flr(cur.x), a.NX);
+    j = clamp_m2(cvt_flr(cur.y), a.NY);
+    kk = clamp_m2(cvt_flr(cur.z), a.NZ);
+  } else {
+    const float xb = (cur.x - a.bx) - 0.5f;
+    const float yb = (cur.y - a.by) - 0.5f;
+    const float zb = (cur.z - a.bz) - 0.5f;
+    const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    t.ax = xb - fx;
+    t.ay = yb - fy;
+    t.az = zb - fz;
+    i = (int)__builtin_amdgcn_fmed3f(fx, -2.f, fnx);
+    j = (int)__builtin_amdgcn_fmed3f(fy, -2.f, fny);
+    kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
+  }
   // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
   const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
   if constexpr (LAYOUT == LAYOUT_QUAD) {
     quad_ld<S>(r, e * (uint32_t)(4 * sizeof(S)), t);
+  } else if constexpr (LAYOUT == LAYOUT_DUAL && __is_same(S, _Float16)) {
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+    const uint32_t voff = e * 4u;
+    const u2v q0 = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);        // row j
+    const u2v q1 = __builtin_amdgcn_raw_buffer_load_b64(r, voff, a.off_j, 0);  // row j+1
+    t.w0 = q0.x;
+    t.w1 = q0.y;
+    t.w2 = q1.x;
+    t.w3 = q1.y;
   } else if constexpr (LAYOUT == LAYOUT_DUAL && __is_same(S, float)) {
     const uint32_t voff = e * 8u;
     const float4 q0 = dual_ld(r, voff, 0);         // row j
@@ -182,7 +208,12 @@ template <typename S, int LAYOUT>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
   if constexpr (__is_same(S, _Float16)) {
-    if (LAYOUT == LAYOUT_QUAD) {
+    if (LAYOUT == LAYOUT_DUAL) {
+      c00 = lerp_x(t.w0, t.ax);
+      c01 = lerp_x(t.w1, t.ax);
+      c10 = lerp_x(t.w2, t.ax);
+      c11 = lerp_x(t.w3, t.ax);
+    } else if (LAYOUT == LAYOUT_QUAD) {
       c00 = lerp_h<0>(t.w0, t.w1, t.ax);
       c10 = lerp_h<1>(t.w0, t.w1, t.ax);
       c01 = lerp_h<0>(t.w2, t.w3, t.ax);
@@ -215,8 +246,13 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
   const int base = 2 * a.PY * a.PZ + 2 * a.PZ + 2;
   float dval = 0.f;
   if (n <= 0) return dval;
+  if constexpr (sizeof(S) == 2) {  // texel coordinates (see fetch_z)
+    cur.x = (cur.x - a.bx) - 0.5f;
+    cur.y = (cur.y - a.by) - 0.5f;
+    cur.z = (cur.z - a.bz) - 0.5f;
+  }
   // two register sets in ping-pong: while one sample is interpolated the
-  // next sample's four loads are in flight
+  // next sample's loads are in flight
   Tap8<S> t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
   int k = 0;
   for (; k + 1 < n; k += 2) {
@@ -369,7 +405,8 @@ template <int EPI, typename S>
 static void fpz_dispatch(const FPZArgs& k, int geom, int layout, hipStream_t st) {
   if constexpr (sizeof(S) == 2) {
     if (layout == LAYOUT_QUAD) return fpz_dispatch_l<EPI, S, LAYOUT_QUAD>(k, geom, st);
-  } else {
+  }
+  if constexpr (!__is_same(S, bf16_t)) {
     if (layout == LAYOUT_DUAL) return fpz_dispatch_l<EPI, S, LAYOUT_DUAL>(k, geom, st);
   }
   fpz_dispatch_l<EPI, S, LAYOUT_PLAIN>(k, geom, st);
