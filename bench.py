@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "mex-cuda-ct_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker (cpu_baseline, FP64 sample)
 
 HBM_PEAK_GBS = 8000.0
+FP_TRAFFIC_BYTES = 185.5e9  # measured, cone 384^3 x 512 f32 FP launch
 
 
 def log(*a):
@@ -174,9 +175,14 @@ def main():
     # volume read once per view plus the projections written once.
     fp_bytes = n * (float(W) * W * H * 4) + n * float(W) * H * 4
     fp_gbs = fp_bytes / (fp_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "fp_kernel<cone,f32>", "achieved": round(fp_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fp_gbs / HBM_PEAK_GBS, 4),
-                "traffic": None,
+    # traffic: HBM-side bytes per FP launch from rocprofv3 --pmc FETCH_SIZE and
+    # WRITE_SIZE on this workload, scaled by what they read for a 1 GiB device
+    # copy (FETCH x2.0, WRITE x1.0); rerun the counters when the kernel changes
+    roofline = {"bound": "hbm", "kernel": "fpz_kernel<cone,f32,x-pair volume>",
+                "achieved": round(fp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(fp_gbs / HBM_PEAK_GBS, 4),
+                "traffic": FP_TRAFFIC_BYTES if (W, n) == (384, 512) else None,
+                "traffic_note": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated",
                 "algorithmic_bytes_per_launch": fp_bytes}
 
     f16 = None

@@ -39,7 +39,13 @@ extern "C" {
 #define MEXCT_ENOTSUP (-5) /* unsupported option combination                  */
 
 /* storage type of the gathered operand (volume for FP, sinogram for BP);
- * arithmetic (weights, ray sums, accumulators) is always FP32. */
+ * arithmetic (weights, ray sums, accumulators) is always FP32.  F32 is the
+ * reference's arithmetic bit for bit.  F16/BF16 also step the sample
+ * coordinates in texel space (a few ulps from the reference's rounding);
+ * F16 copies of f32 data hold x * 2^-e, e the binary exponent of max|x|, so
+ * that any data scale stays inside FP16's range.  In the OSC update the
+ * projected volume copy takes the storage type and the ratio terms take it
+ * too, except that F16 stores them in BF16 (their range exceeds FP16's). */
 #define MEXCT_F32 0
 #define MEXCT_F16 1
 #define MEXCT_BF16 2

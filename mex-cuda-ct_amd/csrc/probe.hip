@@ -12,8 +12,10 @@
 //   BP: the row-paired haloed sinogram; per update one 16-byte (f32) or
 //       8-byte (f16/bf16) load of two adjacent pair elements.  The 64 lanes
 //       of a wave hold 64 consecutive detector columns; every lane walks
-//       down the rows of one view, wrapping, so each view's slice is re-read
-//       from cache as the backprojector re-reads it.
+//       down the rows of one view, wrapping.  The blocks of one view are
+//       consecutive in the grid, so each view's slice is re-read from cache
+//       W times (once per voxel on each of its rays) as the backprojector
+//       re-reads it.
 #include "common.h"
 #include "internal.h"
 
@@ -62,8 +64,9 @@ __global__ __launch_bounds__(256) void bp_probe_kernel(const void* sino, uint32_
                                                        int W, int H, int nviews, int nsteps,
                                                        float* sink) {
   const int u = (int)(blockIdx.x * 64 + (threadIdx.x & 63)) % W;
-  const int view = (int)(blockIdx.y % (unsigned)nviews);
-  int row = (int)(((blockIdx.y / (unsigned)nviews) * 4 + (threadIdx.x >> 6)) % (unsigned)H);
+  const unsigned groups = gridDim.y / (unsigned)nviews;
+  const int view = (int)(blockIdx.y / groups);  // blocks of one view run together
+  int row = (int)(((blockIdx.y % groups) * 4 + (threadIdx.x >> 6)) % (unsigned)H);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)sino, 0, (int)bytes, 0x00020000);
   const uint32_t base = (uint32_t)view * VB + 2u * RB + (uint32_t)(u + 2) * E2;
