@@ -513,9 +513,14 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
   RC(check_padded(W, H, nProj, "sinogram"));
-  // row-paired haloed copy: (W+4) x (H+3) elements of two values per view;
-  // the kernel takes views in chunks below 2 GiB, so one view must fit
-  const size_t rp_bytes = (size_t)(W + 4) * (H + 3) * 2 * store_bytes(st);
+  // row-grouped haloed copy: (W+4) x (H+3) elements of two values per view,
+  // four for 2-byte storage when the rows of consecutive z-voxels step by
+  // less than 2 (par, fan; cone with SAD > 2.02 hw, hw = W/2-1, so that
+  // U = SAD/(SAD - s) < 1.99 for every voxel in the circle |s| < hw); the
+  // kernel takes views in chunks below 2 GiB, so one view must fit
+  const double hw = (double)(W / 2 - 1);
+  const int rows = st != MEXCT_F32 && (geom != 2 || (double)sad > 2.02 * hw) ? 4 : 2;
+  const size_t rp_bytes = (size_t)(W + 4) * (H + 3) * rows * store_bytes(st);
   if ((double)rp_bytes >= 2147483648.0)
     return fail(MEXCT_EINVAL, "detector too large (%lld x %lld)", (long long)W, (long long)H);
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
@@ -527,7 +532,7 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
     RC(max_bits(c, (const float*)sino, (size_t)W * H * nProj, 2, &smax));
   if (nProj > 0)
     HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
-                       smax));
+                       smax, rows));
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   BPLaunch a;
@@ -542,6 +547,7 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   a.nviews = (int)nProj;
   a.sad = sad;
   a.zr = pick_zrange(opts, geom, H);
+  a.rows = rows;
   HIPCHK(launch_bp(a, c->stream));
   return MEXCT_OK;
 }

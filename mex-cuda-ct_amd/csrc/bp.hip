@@ -25,6 +25,7 @@ namespace mexct {
 // f16 taps stay packed ({(j0,i0), (j0+1,i0)} and {(j0,i0+1), (j0+1,i0+1)}
 // per dword) for the mixed-precision lerps.
 typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 template <typename S>
 struct Taps4 {
   typedef float4 T;
@@ -107,7 +108,7 @@ __device__ __forceinline__ float advance_rows(float pz, float dz, int z0) {
 
 // SMALL: a view's haloed slice is below 2^24 bytes, so lane offsets are
 // exact in float and cost one FMA.
-template <int GEOM, int ZR, typename S, bool SMALL>
+template <int GEOM, int ZR, typename S, bool SMALL, int RQ>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const float y = (float)iy - ((float)hw + 0.5f);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.sino, 0, (int)a.sino_bytes, 0x00020000);
-  const uint32_t es = 2u * (uint32_t)sizeof(S);  // bytes of one row-paired element
+  const uint32_t es = (RQ ? 4u : 2u) * (uint32_t)sizeof(S);  // bytes of one row-group element
   const int row_bytes = a.PW * (int)es;          // one detector row of pairs
   float acc[ZR];
 #pragma unroll
@@ -170,6 +171,54 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
     // groups of 8 z-voxels: offsets and loads of the whole group are issued
     // before its interpolation
     constexpr int G = 8;
+    if constexpr (sizeof(S) == 2 && RQ) {
+      // 2-byte storage, row-quad sinogram: one 16-byte load per two
+      // consecutive z-voxels.  Their rows j0 <= j1 <= j0 + 2 (the row step
+      // is below 2, which the host checks), so the pair (j1, j1+1) of each
+      // column is the 4-row run shifted down by j1 - j0 halves.
+      float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
+      const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(HALO * row_bytes);
+#pragma unroll
+      for (int m0 = 0; m0 < ZR; m0 += 2 * G) {
+        float az0[G], az1[G];
+        uint32_t sh[G];
+        u4v q[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const float zb1 = zb + dz;
+          az0[g] = __builtin_amdgcn_fractf(zb);
+          az1[g] = __builtin_amdgcn_fractf(zb1);
+          const int j0 = clamp_m2(cvt_flr(zb), a.H);
+          const int j1 = clamp_m2(cvt_flr(zb1), a.H);
+          sh[g] = (uint32_t)(j1 - j0) << 4;
+          q[g] = __builtin_amdgcn_raw_buffer_load_b128(r, (uint32_t)__mul24(j0, row_bytes) + cb,
+                                                       soff, 0);
+          zb = zb1 + dz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          // q.x, q.y: rows j0..j0+3 of column i0; q.z, q.w: of column i0+1
+          const uint32_t A1 = (uint32_t)((((uint64_t)q[g].y << 32) | q[g].x) >> sh[g]);
+          const uint32_t B1 = (uint32_t)((((uint64_t)q[g].w << 32) | q[g].z) >> sh[g]);
+          float c0, c1, d0, d1;
+          if constexpr (__is_same(S, _Float16)) {
+            c0 = lerp_h<0>(q[g].x, q[g].z, ax);
+            c1 = lerp_h<1>(q[g].x, q[g].z, ax);
+            d0 = lerp_h<0>(A1, B1, ax);
+            d1 = lerp_h<1>(A1, B1, ax);
+          } else {
+            c0 = lerp(__uint_as_float(q[g].x << 16), __uint_as_float(q[g].z << 16), ax);
+            c1 = lerp(__uint_as_float(q[g].x & 0xffff0000u), __uint_as_float(q[g].z & 0xffff0000u), ax);
+            d0 = lerp(__uint_as_float(A1 << 16), __uint_as_float(B1 << 16), ax);
+            d1 = lerp(__uint_as_float(A1 & 0xffff0000u), __uint_as_float(B1 & 0xffff0000u), ax);
+          }
+          acc[m0 + 2 * g] = __builtin_fmaf(UU, lerp(c0, c1, az0[g]), acc[m0 + 2 * g]);
+          acc[m0 + 2 * g + 1] = __builtin_fmaf(UU, lerp(d0, d1, az1[g]), acc[m0 + 2 * g + 1]);
+        }
+      }
+      continue;
+    }
     if constexpr (sizeof(S) == 2) {
       // 2-byte storage: the row coordinate starts at z0 directly and steps in
       // texel coordinates (no replay of the reference's running sum); its
@@ -247,30 +296,33 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 
 static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
 
-template <int ZR, typename S, bool SMALL>
+template <int ZR, typename S, bool SMALL, int RQ>
 static void bp_dispatch_g(const BPKArgs& k, int geom, hipStream_t st) {
   const unsigned g = padded_grid(k.n_logical);
   switch (geom) {
-    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
-    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
-    default: bp_kernel<GEOM_CONE, ZR, S, SMALL><<<g, 64, 0, st>>>(k); break;
+    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
+    default: bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
   }
 }
 
 template <int ZR, typename S>
-static void bp_dispatch(const BPKArgs& k, int geom, hipStream_t st) {
+static void bp_dispatch(const BPKArgs& k, int geom, int rows, hipStream_t st) {
+  if constexpr (sizeof(S) == 2) {
+    if (rows == 4) return bp_dispatch_g<ZR, S, false, 1>(k, geom, st);
+  }
   if ((uint64_t)k.PV * 2u * sizeof(S) < (1u << 24))
-    bp_dispatch_g<ZR, S, true>(k, geom, st);
+    bp_dispatch_g<ZR, S, true, 0>(k, geom, st);
   else
-    bp_dispatch_g<ZR, S, false>(k, geom, st);
+    bp_dispatch_g<ZR, S, false, 0>(k, geom, st);
 }
 
 template <typename S>
-static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, hipStream_t st) {
+static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, int rows, hipStream_t st) {
   if (zr == 64)
-    bp_dispatch<64, S>(k, geom, st);
+    bp_dispatch<64, S>(k, geom, rows, st);
   else
-    bp_dispatch<32, S>(k, geom, st);
+    bp_dispatch<32, S>(k, geom, rows, st);
 }
 
 // Views go in chunks whose row-paired slices stay below 2 GiB (32-bit
@@ -278,7 +330,8 @@ static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, hipStream_t st) {
 // stored, so the per-voxel order and the float sums are unchanged.
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   const int zr = a.zr == 64 ? 64 : 32;
-  const size_t es2 = 2u * (a.st_dt == 0 ? 4u : 2u);
+  const int rows = a.st_dt != 0 && a.rows == 4 ? 4 : 2;
+  const size_t es2 = (size_t)rows * (a.st_dt == 0 ? 4u : 2u);
   BPKArgs k;
   k.out = a.out;
   k.sino_max = a.sino_max;
@@ -304,11 +357,11 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
     k.nviews = cn;
     k.accumulate = v0 > 0;
     if (a.st_dt == 0)
-      bp_dispatch_zr<float>(k, a.geom, zr, st);
+      bp_dispatch_zr<float>(k, a.geom, zr, rows, st);
     else if (a.st_dt == 1)
-      bp_dispatch_zr<_Float16>(k, a.geom, zr, st);
+      bp_dispatch_zr<_Float16>(k, a.geom, zr, rows, st);
     else
-      bp_dispatch_zr<bf16_t>(k, a.geom, zr, st);
+      bp_dispatch_zr<bf16_t>(k, a.geom, zr, rows, st);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     v0 += cn;

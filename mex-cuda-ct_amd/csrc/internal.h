@@ -31,10 +31,12 @@ hipError_t launch_pad3d(const void* in, int in_dt, int64_t in_sy, int64_t in_sz,
 hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
                         int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
                         const uint32_t* scale_max = nullptr);
-// 2D stack, row-paired: in [w,h,n] -> out [2, w+4, h+3, n] where element
-// (px, py) holds the haloed values at rows py and py+1 of column px.
+// 2D stack, row-grouped: in [w,h,n] -> out [rows, w+4, h+3, n] where element
+// (px, py) holds the haloed values at rows py .. py+rows-1 of column px
+// (rows = 2, or 4 for 2-byte storage).
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
-                       int mode, hipStream_t st, const uint32_t* scale_max = nullptr);
+                       int mode, hipStream_t st, const uint32_t* scale_max = nullptr,
+                       int rows = 2);
 // scale_max (pad3z, pad3q, pad2): when non-null, the bits of max|in|; the
 // copy holds in * 2^-pow2_exp(scale_max) (see common.h).
 // 3D interior copy between a padded f32 volume and a plain one.
@@ -100,6 +102,7 @@ struct BPLaunch {
   int geom, W, H, nviews;
   float sad;
   int zr;            // z-voxels per thread (32 or 64)
+  int rows;          // rows per sinogram element: 2, or 4 (2-byte storage, rows step <= 2)
 };
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
 

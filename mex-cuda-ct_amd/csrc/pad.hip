@@ -39,13 +39,15 @@ __global__ __launch_bounds__(256) void pad3_kernel(const void* __restrict__ in, 
   }
 }
 
-// Row-paired haloed sinogram for the backprojector:
-// out[((view*(PY-1) + py)*PX + px)*2 + r] = P(px-2, py-2+r), r in {0,1},
-// PX = w+4, PY = h+4, with clamp (edge copies) or border (zeros) outside.
-// Element (py, px) and its neighbour (py, px+1) hold the four taps of a
-// bilinear sample at (px-2, py-2), so one 16-byte (f32) or 8-byte
-// (f16/bf16) load fetches them all.
-template <typename S>
+// Row-grouped haloed sinogram for the backprojector (R = 2 or 4 rows):
+// out[((view*(h+3) + py)*PX + px)*R + r] = P(px-2, py-2+r), r < R, py in
+// [0, h+3), PX = w+4, with clamp (edge copies) or border (zeros) outside.
+// R = 2: element (py, px) and its neighbour (py, px+1) hold the four taps of
+// a bilinear sample at (px-2, py-2), one 16-byte (f32) or 8-byte load.
+// R = 4 (2-byte storage): the 16-byte load of two elements holds rows
+// py-2 .. py+1 of two columns, the taps of two consecutive z-voxels whose
+// rows differ by at most 2.
+template <typename S, int R>
 __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, int in_dt,
                                                    S* __restrict__ out, int w, int h, int n,
                                                    int mode, const uint32_t* scale_max) {
@@ -59,22 +61,21 @@ __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, 
     const int py = (int)(r % PR);
     const int view = (int)(r / PR);
     const size_t vbase = (size_t)w * h * view;
-    float v[2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < R; ++q) {
       int i = px - HALO, j = py - HALO + q;
+      float v;
       if (mode == ADDR_CLAMP) {
         i = clampi(i, 0, w - 1);
         j = clampi(j, 0, h - 1);
-        v[q] = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
+        v = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
       } else if (i < 0 || i >= w || j < 0 || j >= h) {
-        v[q] = 0.f;
+        v = 0.f;
       } else {
-        v[q] = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
+        v = load_any(in, in_dt, (size_t)i + (size_t)w * j + vbase);
       }
+      out[R * o + q] = Store<S>::from(ldexpf(v, -sexp));
     }
-    out[2 * o] = Store<S>::from(ldexpf(v[0], -sexp));
-    out[2 * o + 1] = Store<S>::from(ldexpf(v[1], -sexp));
   }
 }
 
@@ -267,15 +268,21 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 }
 
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
-                       int mode, hipStream_t st, const uint32_t* scale_max) {
+                       int mode, hipStream_t st, const uint32_t* scale_max, int rows) {
   size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO - 1) * n;
   unsigned g = grid_for(total);
   if (st_dt == 0)
-    pad2_kernel<float><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode, scale_max);
+    pad2_kernel<float, 2><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode, scale_max);
+  else if (st_dt == 1 && rows == 4)
+    pad2_kernel<_Float16, 4><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode,
+                                                scale_max);
   else if (st_dt == 1)
-    pad2_kernel<_Float16><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode, scale_max);
+    pad2_kernel<_Float16, 2><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode,
+                                                scale_max);
+  else if (rows == 4)
+    pad2_kernel<bf16_t, 4><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
   else
-    pad2_kernel<bf16_t><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
+    pad2_kernel<bf16_t, 2><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
   return hipGetLastError();
 }
 

@@ -306,12 +306,15 @@ def test_fp_storage_on_rounded_volume(gpu, geom, storage):
     _check(got, ref, f"FP storage={storage} geom={geom} vs rounded", 1e-4, 2e-5)
 
 
+# SAD 400 takes the row-quad sinogram (one load per two z-voxels); at SAD 30
+# the cone magnification reaches 2.7 and the row-paired one is used.
 @pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
 @pytest.mark.parametrize("geom", GEOMS)
-def test_bp_storage_on_rounded_sinogram(gpu, geom, storage):
+@pytest.mark.parametrize("sad", [400.0, 30.0])
+def test_bp_storage_on_rounded_sinogram(gpu, geom, storage, sad):
     W, H, n = 40, 36, 8
     P = np.asfortranarray(_rng(71 + geom).uniform(0, 1, (W, H, n)).astype(np.float32))
     ang = _angles(n)
-    got = mexct.CUDAmex_BP(P, 400.0, ang, geom, storage=storage)
-    ref = O.bp(_round_storage(P, storage), 400.0, ang, geom)
-    _check(got, ref, f"BP storage={storage} geom={geom} vs rounded", 1e-4, 2e-5)
+    got = mexct.CUDAmex_BP(P, sad, ang, geom, storage=storage)
+    ref = O.bp(_round_storage(P, storage), sad, ang, geom)
+    _check(got, ref, f"BP storage={storage} geom={geom} sad={sad} vs rounded", 1e-4, 2e-5)
