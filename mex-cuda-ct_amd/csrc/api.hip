@@ -79,6 +79,7 @@ enum Slot {
   WS_PREW,
   WS_POSTW,
   WS_PLAIN,
+  WS_CHAIN,    // cone f32 BP: row coordinates at the z-range starts
   WS_MUZ,      // z-fastest copy of the OSC volume
   WS_MAXBITS,  // max|x| bits for the FP16 copies: [1] volume, [2] sinogram
   WS_COUNT
@@ -548,6 +549,17 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   a.sad = sad;
   a.zr = pick_zrange(opts, geom, H);
   a.rows = rows;
+  a.chain = nullptr;
+  a.chain_views = 0;
+  const size_t cfv = geom == 2 && st == MEXCT_F32 ? bp_chain_floats_per_view((int)W, (int)H, a.zr) : 0;
+  if (cfv > 0 && nProj > 0) {
+    // at most 1 GiB of chain scratch; more views run as further launches
+    const int64_t cv = std::min<int64_t>(nProj, std::max<int64_t>(1, ((int64_t)1 << 28) / (int64_t)cfv));
+    void* ch;
+    RC(c->get(WS_CHAIN, (size_t)cv * cfv * 4, &ch));
+    a.chain = (float*)ch;
+    a.chain_views = (int)cv;
+  }
   HIPCHK(launch_bp(a, c->stream));
   return MEXCT_OK;
 }

@@ -67,9 +67,11 @@ struct BPKArgs {
   int W, H, nviews, PW;  // PW: elements (pairs) per row
   int64_t PV;            // pairs per view
   int accumulate;        // 1: continue the sums already in out (a later chunk of views)
+  const float* chain;    // cone f32: row coordinate at each z-range start (bp_chain_kernel)
   const uint32_t* sino_max;  // non-null: sino holds P * 2^-e (pow2_exp)
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
+  uint32_t ncols;  // chain columns per (view, range): padded tiles x 64
 };
 
 // per-view detector geometry of one voxel column
@@ -106,6 +108,37 @@ __device__ __forceinline__ float advance_rows(float pz, float dz, int z0) {
   return pz + (float)z0;  // k + 0.5 held exactly by the running sum
 }
 
+// Cone row coordinates at the z-range starts: the reference steps
+// proj_z += U once per slice from z = 0 (CUDAmex_BP.cu:208-219), so the
+// value at z0 is the float running sum after z0 additions.  One thread per
+// (voxel column, view) runs the sum once and stores it at z0 = ZR, 2 ZR, ...,
+// which saves every range from replaying the additions before it.
+template <int ZR>
+__global__ __launch_bounds__(256) void bp_chain_kernel(BPKArgs a, float* chain) {
+  const size_t total = (size_t)a.ncols * a.nviews;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (size_t)gridDim.x * 256) {
+    const uint32_t col = (uint32_t)(t % a.ncols);
+    const int v = (int)(t / a.ncols);
+    const uint32_t tile = col >> 6, lane = col & 63u;
+    const uint32_t st = tile / (kSuper * kSuper), in = tile % (kSuper * kSuper);
+    const uint32_t sbx = (a.tiles_x + kSuper - 1) / kSuper;
+    const int ix = (int)((st % sbx) * kSuper + in % kSuper) * 8 + (int)(lane & 7);
+    const int iy = (int)((st / sbx) * kSuper + in / kSuper) * 8 + (int)(lane >> 3);
+    if (ix >= a.W || iy >= a.W) continue;
+    const uint32_t hw = (uint32_t)a.W / 2u - 1u;
+    const float x = (float)ix - ((float)hw + 0.5f);
+    const float y = (float)iy - ((float)hw + 0.5f);
+    const float2 scv = a.sc[v];
+    float U, py, pz, dz;
+    bp_view_geom<GEOM_CONE>(x, y, scv.x, scv.y, a.SR, a.v0, hw, U, py, pz, dz);
+    float* o = chain + (size_t)v * (a.nz - 1) * a.ncols + col;
+    for (uint32_t r = 1; r < a.nz; ++r) {
+      for (int m = 0; m < ZR; ++m) pz = pz + dz;
+      o[(size_t)(r - 1) * a.ncols] = pz;
+    }
+  }
+}
+
 // SMALL: a view's haloed slice is below 2^24 bytes, so lane offsets are
 // exact in float and cost one FMA.
 template <int GEOM, int ZR, typename S, bool SMALL, int RQ>
@@ -120,6 +153,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const uint32_t sbx = (a.tiles_x + kSuper - 1) / kSuper;
   const uint32_t tx = (st % sbx) * kSuper + in % kSuper;
   const uint32_t ty = (st / sbx) * kSuper + in / kSuper;
+  const uint32_t chain_col = tile * 64u + threadIdx.x;
   const int ix = (int)tx * 8 + (int)(threadIdx.x & 7);
   const int iy = (int)ty * 8 + (int)(threadIdx.x >> 3);
   if (ix >= a.W || iy >= a.W) return;
@@ -252,7 +286,11 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       }
       continue;
     }
-    pz = advance_rows<GEOM>(pz, dz, z0);
+    if (GEOM == GEOM_CONE && a.chain) {
+      if (zr > 0) pz = a.chain[((size_t)v * (a.nz - 1) + (zr - 1)) * a.ncols + chain_col];
+    } else {
+      pz = advance_rows<GEOM>(pz, dz, z0);
+    }
 #pragma unroll
     for (int m0 = 0; m0 < ZR; m0 += G) {
       float az[G];
@@ -325,6 +363,13 @@ static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, int rows, hipStre
     bp_dispatch<32, S>(k, geom, rows, st);
 }
 
+size_t bp_chain_floats_per_view(int W, int H, int zr) {
+  const uint32_t tiles_x = (uint32_t)((W + 7) / 8);
+  const uint32_t tpad = (tiles_x + kSuper - 1) / kSuper * kSuper;
+  const uint32_t nz = (uint32_t)((H + zr - 1) / zr);
+  return nz > 1 ? (size_t)tpad * tpad * 64u * (nz - 1) : 0;
+}
+
 // Views go in chunks whose row-paired slices stay below 2 GiB (32-bit
 // buffer offsets); each chunk continues the register sums the previous one
 // stored, so the per-voxel order and the float sums are unchanged.
@@ -345,9 +390,13 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.nz = (uint32_t)((a.H + zr - 1) / zr);
   const uint32_t tpad = (k.tiles_x + kSuper - 1) / kSuper * kSuper;
   k.n_logical = tpad * tpad * k.nz;
+  k.ncols = tpad * tpad * 64u;
   if (k.n_logical == 0) return hipSuccess;
   const size_t view_bytes = (size_t)k.PV * es2;
-  const int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);
+  int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);
+  // the exact (f32) cone path reads its z-range starts from the chain buffer
+  const bool use_chain = a.geom == GEOM_CONE && a.st_dt == 0 && k.nz > 1 && a.chain;
+  if (use_chain) chunk = std::min(chunk, a.chain_views);
   int v0 = 0;
   do {  // at least one launch, so nviews == 0 still writes the zero volume
     const int cn = std::min(chunk, a.nviews - v0);
@@ -356,6 +405,16 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
     k.sc = a.sc + v0;
     k.nviews = cn;
     k.accumulate = v0 > 0;
+    k.chain = nullptr;
+    if (use_chain && cn > 0) {
+      const size_t total = (size_t)k.ncols * cn;
+      const unsigned cg = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+      if (zr == 64)
+        bp_chain_kernel<64><<<cg, 256, 0, st>>>(k, a.chain);
+      else
+        bp_chain_kernel<32><<<cg, 256, 0, st>>>(k, a.chain);
+      k.chain = a.chain;
+    }
     if (a.st_dt == 0)
       bp_dispatch_zr<float>(k, a.geom, zr, rows, st);
     else if (a.st_dt == 1)

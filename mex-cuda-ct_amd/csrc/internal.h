@@ -103,7 +103,11 @@ struct BPLaunch {
   float sad;
   int zr;            // z-voxels per thread (32 or 64)
   int rows;          // rows per sinogram element: 2, or 4 (2-byte storage, rows step <= 2)
+  float* chain;      // cone f32: scratch of chain_views x (nz-1) x padded columns floats, or null
+  int chain_views;   // views per launch the chain scratch holds
 };
+// floats of BP chain scratch per view (cone f32, z-range zr): 0 if unused
+size_t bp_chain_floats_per_view(int W, int H, int zr);
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
 
 struct OSCBPLaunch {
