@@ -212,13 +212,14 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       // column is the 4-row run shifted down by j1 - j0 halves.
       float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
       const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(HALO * row_bytes);
+      constexpr int GP = 8;  // voxel pairs per load group (4: 12.0 ms, 8: 11.3 ms)
 #pragma unroll
-      for (int m0 = 0; m0 < ZR; m0 += 2 * G) {
-        float az0[G], az1[G];
-        uint32_t sh[G];
-        u4v q[G];
+      for (int m0 = 0; m0 < ZR; m0 += 2 * GP) {
+        float az0[GP], az1[GP];
+        uint32_t sh[GP];
+        u4v q[GP];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < GP; ++g) {
           const float zb1 = zb + dz;
           az0[g] = __builtin_amdgcn_fractf(zb);
           az1[g] = __builtin_amdgcn_fractf(zb1);
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < GP; ++g) {
           // q.x, q.y: rows j0..j0+3 of column i0; q.z, q.w: of column i0+1
           const uint32_t A1 = (uint32_t)((((uint64_t)q[g].y << 32) | q[g].x) >> sh[g]);
           const uint32_t B1 = (uint32_t)((((uint64_t)q[g].w << 32) | q[g].z) >> sh[g]);
