@@ -600,9 +600,41 @@ int mexct_g3d_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t s
   if (nProj == 0) return MEXCT_OK;
   DevGuard g(c->device);
   void* pad;
+  const float2* sc;
+  size_t lbytes = 0;
+  const int layout = fp_layout(side, nSlices, st, &lbytes);
+  if (layout >= 0) {
+    // z-fastest volume as for the standard geometries; a wave marches 64
+    // consecutive detector rows of one column, whose rays are near each
+    // other for any detector whose columns are lines in space
+    RC(c->get(WS_VOL, lbytes, &pad));
+    uint32_t* vmax = nullptr;
+    if (st == MEXCT_F16 && vol_dtype == MEXCT_F32)
+      RC(max_bits(c, (const float*)vol, (size_t)side * side * nSlices, 1, &vmax));
+    HIPCHK(build_fp_volume(layout, vol, vol_dtype, side, side * side, pad, st, side, nSlices,
+                           ADDR_BORDER, c->stream, vmax));
+    RC(c->sincos_table(angles, nProj, &sc));
+    FPZLaunch z;
+    z.vol = pad;
+    z.st_dt = st;
+    z.layout = layout;
+    z.vol_max = vmax;
+    z.out = out;
+    z.sc = sc;
+    z.geom = GEOM_G3D;
+    z.W = (int)W;
+    z.H = (int)H;
+    z.nviews = nProj;
+    z.sad = 0.f;
+    z.NX = (int)side;
+    z.NZ = (int)nSlices;
+    z.pts = pts_dev;
+    z.pts_view_stride = ppv ? (int64_t)6 * W * H : 0;
+    HIPCHK(launch_fpz(z, c->stream));
+    return MEXCT_OK;
+  }
   RC(c->get(WS_VOL, (size_t)(side + 4) * (side + 4) * (nSlices + 4) * store_bytes(st), &pad));
   HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)side, (int)side, (int)nSlices, ADDR_BORDER, c->stream));
-  const float2* sc;
   RC(c->sincos_table(angles, nProj, &sc));
   FPLaunch a;
   a.vol = pad;

@@ -52,6 +52,8 @@ struct FPZArgs {
   float u0, v0, bx, by, bz, SR, RD;
   int first, stride;  // view = first + j*stride
   int PW, PV;         // OSC padded copies
+  const float* pts;   // general-3D: per-pixel ray endpoints [W,H,6] (per view: + view*stride)
+  int64_t pts_view_stride;
   uint32_t n_ug, n_vg, n_logical;
   uint32_t nv;  // views of the launch
 };
@@ -329,7 +331,12 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   }
   const float2 scv = a.sc[view];
   R3 st, en;
-  std_ray<GEOM>(ix, iy, a.u0, a.v0, scv.x, scv.y, a.SR, a.RD, st, en);
+  if (GEOM == GEOM_G3D) {
+    g3d_ray(a.pts + (size_t)view * a.pts_view_stride + (size_t)iy * a.W + ix, (size_t)a.W * a.H,
+            scv.x, scv.y, st, en);
+  } else {
+    std_ray<GEOM>(ix, iy, a.u0, a.v0, scv.x, scv.y, a.SR, a.RD, st, en);
+  }
   // CUDAmex_FP marches cone rays with step 1.0 (CUDAmex_FP.cu:231), par and
   // fan with 0.5; CUDAmex_oscIter uses 0.5 for every geometry (:244).
   constexpr int STEP2 = (EPI == 1 || GEOM != GEOM_CONE) ? 1 : 0;
@@ -350,8 +357,12 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   }
 }
 
+// W x H detector, NX x NX x NZ volume (the same sizes in the standard
+// geometries; general-3D rays have their own detector).
 static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* sc, int W, int H,
-                          int nviews, float sad) {
+                          int nviews, float sad, int NX = 0, int NZ = 0) {
+  if (NX == 0) NX = W;
+  if (NZ == 0) NZ = H;
   FPZArgs k{};
   k.vol = vol;
   k.sc = sc;
@@ -359,14 +370,14 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   k.H = H;
   const int nb = (W + 31) / 32;  // CUDAmex_FP.cu:434-440 grid extent
   k.rows = H < 32 * nb ? H : 32 * nb;
-  k.NX = W;
-  k.NY = W;
-  k.NZ = H;
+  k.NX = NX;
+  k.NY = NX;
+  k.NZ = NZ;
   // element counts: x and y in [-2, N+1] (N+4), except the element of the
   // corner pair or quad, whose base runs over [-2, N] (N+3)
-  const int PX = layout == LAYOUT_PLAIN ? W + 2 * HALO : W + 2 * HALO - 1;
-  k.PY = layout == LAYOUT_QUAD ? W + 2 * HALO - 1 : W + 2 * HALO;
-  k.PZ = H + 2 * HALO;
+  const int PX = layout == LAYOUT_PLAIN ? NX + 2 * HALO : NX + 2 * HALO - 1;
+  k.PY = layout == LAYOUT_QUAD ? NX + 2 * HALO - 1 : NX + 2 * HALO;
+  k.PZ = NZ + 2 * HALO;
   const uint32_t es = (st_dt == 0 ? 4u : 2u) *
                       (layout == LAYOUT_QUAD ? 4u : layout == LAYOUT_DUAL ? 2u : 1u);
   k.vol_bytes = (uint32_t)((size_t)PX * k.PY * k.PZ * es);
@@ -375,9 +386,9 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   k.off_ij = k.off_i + k.off_j;
   k.u0 = float(W - 1) / 2;
   k.v0 = float(H - 1) / 2;
-  k.bx = (float)(-(double)(float)W / 2.0);
+  k.bx = (float)(-(double)(float)NX / 2.0);
   k.by = k.bx;
-  k.bz = (float)(-(double)(float)H / 2.0);
+  k.bz = (float)(-(double)(float)NZ / 2.0);
   k.SR = sad;
   k.RD = sad;
   k.first = 0;
@@ -396,7 +407,10 @@ static void fpz_dispatch_l(const FPZArgs& k, int geom, hipStream_t st) {
   switch (geom) {
     case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
     case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
-    default: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
+    case GEOM_CONE: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
+    default:
+      if constexpr (EPI == 0) fpz_kernel<GEOM_G3D, S, 0, LAYOUT><<<g, 256, 0, st>>>(k);
+      break;
   }
 }
 
@@ -413,9 +427,12 @@ static void fpz_dispatch(const FPZArgs& k, int geom, int layout, hipStream_t st)
 }
 
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
-  FPZArgs k = make_zargs(a.vol, a.st_dt, a.layout, a.sc, a.W, a.H, a.nviews, a.sad);
+  FPZArgs k = make_zargs(a.vol, a.st_dt, a.layout, a.sc, a.W, a.H, a.nviews, a.sad, a.NX, a.NZ);
   k.out = a.out;
   k.vol_max = a.vol_max;
+  k.pts = a.pts;
+  k.pts_view_stride = a.pts_view_stride;
+  if (a.geom == GEOM_G3D) k.rows = a.H;  // every detector row is a ray
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
     fpz_dispatch<0, float>(k, a.geom, a.layout, st);

@@ -71,8 +71,11 @@ struct FPZLaunch {
   const uint32_t* vol_max;  // non-null: vol holds V * 2^-e (pow2_exp); sums are rescaled
   float* out;
   const float2* sc;
-  int geom, W, H, nviews;
+  int geom, W, H, nviews;  // W x H detector
   float sad;
+  int NX = 0, NZ = 0;      // volume NX x NX x NZ (0: W x W x H, the standard geometries)
+  const float* pts = nullptr;  // GEOM_G3D: ray endpoints [W,H,6] (+ view * pts_view_stride)
+  int64_t pts_view_stride = 0;
 };
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 

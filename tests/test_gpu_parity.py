@@ -318,3 +318,14 @@ def test_bp_storage_on_rounded_sinogram(gpu, geom, storage, sad):
     got = mexct.CUDAmex_BP(P, sad, ang, geom, storage=storage)
     ref = O.bp(_round_storage(P, storage), sad, ang, geom)
     _check(got, ref, f"BP storage={storage} geom={geom} sad={sad} vs rounded", 1e-4, 2e-5)
+
+
+@pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
+def test_g3d_fp_storage_on_rounded_volume(gpu, storage):
+    side, nsl, n = 24, 20, 4
+    vol = _vol(side, nsl, 35)
+    pts = np.asfortranarray(np.stack([_random_pts(10, 14, 60 + k) for k in range(n)], axis=3))
+    ang = _angles(n)
+    got = mexct.CUDAmex_general3D_FP(vol, ang, pts, n, storage=storage)
+    ref = O.g3d_fp(_round_storage(vol, storage), ang, pts, n)
+    _check(got, ref, f"g3D FP storage={storage} vs rounded", 1e-4, 2e-5)
