@@ -691,8 +691,13 @@ int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int6
   if (W < 1 || H < 1 || nProj < 0 || M < 1 || N < 1) return fail(MEXCT_EINVAL, "bad dims");
   if (!out || (nProj > 0 && (!sino || !angles || !pts_dev))) return fail(MEXCT_EINVAL, "null array argument");
   DevGuard g(c->device);
-  HIPCHK(launch_fill(out, 0.f, (size_t)M * M * N, c->stream));
-  if (nProj == 0) return MEXCT_OK;
+  if (nProj == 0) {
+    HIPCHK(launch_fill(out, 0.f, (size_t)M * M * N, c->stream));
+    return MEXCT_OK;
+  }
+  void* acc;
+  RC(c->get(WS_TVG, (size_t)M * M * N * 4, &acc));
+  HIPCHK(launch_fill((float*)acc, 0.f, (size_t)M * M * N, c->stream));
   const float2* sc;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   G3DBPLaunch a;
@@ -706,7 +711,9 @@ int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int6
   a.nviews = (int)nProj;
   a.M = (int)M;
   a.N = (int)N;
+  a.out = (float*)acc;
   HIPCHK(launch_g3d_bp(a, c->stream));
+  HIPCHK(launch_g3d_unz((const float*)acc, out, (int)M, (int)N, c->stream));
   return MEXCT_OK;
 }
 

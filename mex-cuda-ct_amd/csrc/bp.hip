@@ -618,13 +618,19 @@ __device__ __forceinline__ int ceil_index(float v) {
   return (int)fminf(fmaxf(ceilf(v), -2.f), 1073741824.f);
 }
 
+// One thread per (pixel, view).  A wave holds 64 consecutive detector rows
+// of one column, whose rays run near each other for a detector whose
+// columns are lines in space, and scatters into a z-fastest accumulator
+// acc[(i*M + j)*N + k]: the lanes' atomics at one step then land on a few
+// consecutive cache lines instead of one line per lane.  g3d_unz_kernel
+// turns the accumulator into the [M,M,N] x-fastest output.
 __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
   const uint32_t view = b / a.tiles_per_view;
   const uint32_t t = b - view * a.tiles_per_view;
-  const int ix = (int)(t % a.tiles_x) * 16 + (int)(threadIdx.x & 15);
-  const int iy = (int)(t / a.tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  const int ix = (int)(t % a.tiles_x) * 4 + (int)(threadIdx.x >> 6);
+  const int iy = (int)(t / a.tiles_x) * 64 + (int)(threadIdx.x & 63);
   if (ix >= a.W || iy >= a.H) return;
   const float2 scv = a.sc[view];
   const float sB = scv.x, cB = scv.y;
@@ -689,7 +695,12 @@ __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
   }
   const float pval = a.sino[(size_t)view * npts + (size_t)iy * a.W + ix];
   const unsigned M = (unsigned)a.M, N = (unsigned)a.N;
-  const size_t MM = (size_t)a.M * a.M;
+  // Consecutive 0.5-voxel steps often land in the same cell: their weights
+  // are summed in registers and scattered once (the scatter order is not
+  // fixed anyway, as with the reference's atomicAdd).  pend is the index of
+  // the (kL, kL+1) pair being summed, ~0 when none.
+  size_t pend = ~(size_t)0;
+  float wlo = 0.f, whi = 0.f;
   for (int k = 0; k < n; ++k) {
     const float fxv = cur.x - a.bx, fyv = cur.y - a.by, fzv = cur.z - a.bz;
     const int iL = floor_index(fxv), iU = ceil_index(fxv);
@@ -701,13 +712,33 @@ __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
       const float iYY = fyv - (float)jL + 0.5f;
       const float iZ = fzv - (float)kL + 0.5f;
       const float iZZ = 1 - iZ;
-      const size_t col = (size_t)jL * a.M + (size_t)iL;
-      atomicAdd(a.out + col + MM * kL, pval * stepsize * iXX * iYY * iZZ);
-      atomicAdd(a.out + col + MM * kU, pval * stepsize * iXX * iYY * iZ);
+      float lo = pval * stepsize * iXX * iYY * iZZ;
+      float hi = pval * stepsize * iXX * iYY * iZ;
+      if (kU == kL) {  // both corners are the same voxel
+        lo = lo + hi;
+        hi = 0.f;
+      }
+      const size_t idx = ((size_t)iL * a.M + (size_t)jL) * N + (size_t)kL;
+      if (idx == pend) {
+        wlo = wlo + lo;
+        whi = whi + hi;
+      } else {
+        if (pend != ~(size_t)0) {
+          atomicAdd(a.out + pend, wlo);
+          if (whi != 0.f) atomicAdd(a.out + pend + 1, whi);
+        }
+        pend = idx;
+        wlo = lo;
+        whi = hi;
+      }
     }
     cur.x = cur.x + sX;
     cur.y = cur.y + sY;
     cur.z = cur.z + sZ;
+  }
+  if (pend != ~(size_t)0) {
+    atomicAdd(a.out + pend, wlo);
+    if (whi != 0.f) atomicAdd(a.out + pend + 1, whi);
   }
 }
 
@@ -725,11 +756,35 @@ hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st) {
   k.bx = (float)(-(double)(float)a.M / 2.0);
   k.by = k.bx;
   k.bz = (float)(-(double)(float)a.N / 2.0);
-  k.tiles_x = (uint32_t)((a.W + 15) / 16);
-  k.tiles_per_view = k.tiles_x * (uint32_t)((a.H + 15) / 16);
+  k.tiles_x = (uint32_t)((a.W + 3) / 4);
+  k.tiles_per_view = k.tiles_x * (uint32_t)((a.H + 63) / 64);
   k.n_logical = k.tiles_per_view * (uint32_t)a.nviews;
   if (k.n_logical == 0) return hipSuccess;
   g3d_bp_kernel<<<padded_grid(k.n_logical), 256, 0, st>>>(k);
+  return hipGetLastError();
+}
+
+// acc[(i*M + j)*N + k] -> out[i + M*j + M*M*k]: 32 x 32 (k, i) tiles of one
+// j through LDS, read along k and written along i.
+__global__ __launch_bounds__(256) void g3d_unz_kernel(const float* __restrict__ acc,
+                                                      float* __restrict__ out, int M, int N) {
+  __shared__ float tile[32][33];
+  const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32, j = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + ty + 8 * r, k = k0 + tx;
+    tile[ty + 8 * r][tx] = (i < M && k < N) ? acc[((size_t)i * M + j) * N + k] : 0.f;
+  }
+  __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    const int k = k0 + ty + 8 * r, i = i0 + tx;
+    if (i < M && k < N) out[(size_t)i + (size_t)M * j + (size_t)M * M * k] = tile[tx][ty + 8 * r];
+  }
+}
+
+hipError_t launch_g3d_unz(const float* acc, float* out, int M, int N, hipStream_t st) {
+  const dim3 g((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32), (unsigned)M);
+  g3d_unz_kernel<<<g, 256, 0, st>>>(acc, out, M, N);
   return hipGetLastError();
 }
 

@@ -133,10 +133,12 @@ struct G3DBPLaunch {
   const float* pts;
   int64_t pts_view_stride;
   const float2* sc;
-  float* out;         // [M,M,N], zeroed by the caller
+  float* out;         // z-fastest accumulator acc[(i*M + j)*N + k], zeroed by the caller
   int W, H, nviews, M, N;
 };
 hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st);
+// z-fastest accumulator -> [M,M,N] x-fastest volume
+hipError_t launch_g3d_unz(const float* acc, float* out, int M, int N, hipStream_t st);
 
 // ---- TV (tv.hip)
 // Strided view of a float volume: element (i,j,k) at p[i + sy*j + sz*k].
