@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f16", action="store_true", help="skip the FP16-storage form")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the parallel-beam, general-3D and OSC-TV workload timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "scaling HBM byte counters)")
@@ -240,13 +242,13 @@ def main():
             cores = os.cpu_count() or 1
         cores = max(1, min(cores, 16))  # the host share of one GPU
         O.set_threads(cores)
-        kv = 4  # views of the FP sample
+        kv = 64  # views of the FP sample (about 5 s on 16 threads)
         log(f"[bench] cpu baseline: oracle FP on {kv} views, BP on 1024 columns, {cores} threads")
         t0 = time.perf_counter()
         O.fp(vol_np, SAD, ang[:kv], kv, 2)
         tf = time.perf_counter() - t0
         rng = np.random.Generator(np.random.PCG64(11))
-        ncol = 1024
+        ncol = 32768  # voxel columns of the BP sample (about 3 s)
         cx = rng.integers(0, W, ncol)
         cy = rng.integers(0, W, ncol)
         fil_np = np.asfortranarray(filt32.cpu().numpy().transpose(2, 1, 0))
@@ -260,6 +262,8 @@ def main():
                "sample": f"FP of {kv} of {n} views ({tf:.1f} s); BP of {ncol} random voxel "
                          f"columns x {H} z x {n} views ({tb:.1f} s); FP32 oracle"}
         log(f"[bench] cpu: FP {cfp:.3f} GUPS, BP {cbp:.3f} GUPS")
+
+    others = None if args.no_other else other_workloads(ctx, stream, log)
 
     serve = None
     if not args.no_serve:
@@ -288,9 +292,77 @@ def main():
         "gather_roofline": gather,
         "fp16_storage": f16,
         "cpu_baseline": cpu,
+        "other_workloads": others,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)
+
+
+def other_workloads(ctx, stream, log):
+    """Device times of the other workloads: parallel-beam FP/BP of the 128^3
+    phantom (180 views), helical general-3D FP/BP (320^3, 360 views of a
+    256^2 jittered detector), and few-view OSC-TV (256^3, 60 views, 10
+    subsets, 20 iterations, 10 TV steps) with FP32, FP16 and BF16 storage."""
+    import torch
+
+    import mexct
+    from mexct import phantoms
+
+    dev = torch.device("cuda:0")
+
+    def dev_ms(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return round(e0.elapsed_time(e1) / reps, 3)
+
+    res = {}
+    log("[bench] other workloads: parallel 128^3 x 180")
+    v = torch.from_numpy(phantoms.par_phantom_128(seed=1).T.copy()).to(dev)
+    a = phantoms.angles_uniform(180, np.pi)
+    p = torch.empty((180, 128, 128), dtype=torch.float32, device=dev)
+    r = torch.empty((128, 128, 128), dtype=torch.float32, device=dev)
+    res["parallel_128x180"] = {"fp_ms": dev_ms(lambda: ctx.fp(v, 1000.0, a, 180, 0, p)),
+                               "bp_ms": dev_ms(lambda: ctx.bp(p, 1000.0, a, 0, r))}
+    log("[bench] other workloads: helical general-3D 320^3 x 360 x 256^2")
+    side, nv, wd = 320, 360, 256
+    vol = phantoms.ellipsoid_phantom(side, side, 4, 12, 140.0, 0.02, (8.0, 40.0), (0.01, 0.03),
+                                     100.0, 120.0)
+    pts = phantoms.helix_pts(wd, wd, nv, side)
+    vt = torch.from_numpy(vol.T.copy()).to(dev)
+    pt = torch.from_numpy(np.ascontiguousarray(pts.transpose(3, 2, 1, 0))).to(dev)
+    z = np.zeros(nv, np.float32)
+    g = torch.empty((nv, wd, wd), dtype=torch.float32, device=dev)
+    rv = torch.empty((side, side, side), dtype=torch.float32, device=dev)
+    res["helix_g3d_320x360"] = {
+        "fp_ms": dev_ms(lambda: ctx.g3d_fp(vt, z, pt, nv, g)),
+        "fp16_fp_ms": dev_ms(lambda: ctx.g3d_fp(vt, z, pt, nv, g, storage=mexct.F16)),
+        "bp_ms": dev_ms(lambda: ctx.g3d_bp(g, z, pt, side, side, rv), reps=1)}
+    del vt, pt, g, rv
+    log("[bench] other workloads: OSC-TV 256^3 x 60")
+    W, n = 256, 60
+    vol = phantoms.osc_phantom(W, seed=3)
+    a = phantoms.angles_uniform(n, 2 * np.pi)
+    fp = mexct.CUDAmex_FP(vol, 1000.0, a, n, 2)
+    pre, post = phantoms.poisson_transmission(fp, 1e5, seed=33)
+    picks = mexct.first_picks(303, 10, 10, 1.0, 20)
+    pre_t = torch.from_numpy(pre.T.copy()).to(dev)
+    post_t = torch.from_numpy(post.T.copy()).to(dev)
+    mu = torch.empty((W, W, W), dtype=torch.float32, device=dev)
+    osc = {}
+    for name, st in (("f32", mexct.F32), ("f16", mexct.F16), ("bf16", mexct.BF16)):
+        osc[name + "_ms"] = dev_ms(lambda st=st: ctx.osc_tv_recon(pre_t, post_t, a, 1000.0, 2, 20,
+                                                                    0.1, 10, 10, 1.0, picks, mu,
+                                                                    tv_steps=10, storage=st),
+                                   reps=1)
+    res["osc_tv_256x60"] = osc
+    log(f"[bench] other workloads: {res}")
+    return res
 
 
 SERVE_SIDES = (128, 192, 256, 384, 512)
