@@ -150,18 +150,52 @@ int mexct_tvmin_dev(mexct_ctx* ctx, float* vol, int64_t Nx, int64_t Ny, int64_t 
                     const mexct_opts* opts);
 
 /* ------------------------------------------------------------------ FDK filter
- * filtered = preFilterSinogram(sino, geom, angles, filter, cutoff) without
- * Parker weighting (fbp_reconstruction/preFilterSinogram.m:29-152):
- * cosine weight (fan/cone), band-limited ramp x window on a zero-padded
- * FFT grid of order max(64, 2^nextpow2(2W)), then x (angle step)/2.
+ * filtered = preFilterSinogram(sino, geom, angles, filter, cutoff[, parker_q])
+ * (fbp_reconstruction/preFilterSinogram.m:29-152): cosine weight
+ * (fan/cone), Parker window when parker_q >= 0, band-limited ramp x window
+ * on a zero-padded FFT grid of order max(64, 2^nextpow2(2W)), then
+ * x (angle step)/2.  parker_q = MEXCT_NO_PARKER is the 5-argument call.
  * window: 0 ramp, 1 shepp-logan, 2 cosine, 3 hamming, 4 hann, 5 none.
- * The convolution is evaluated in FP64 on the device. */
+ * The convolution is evaluated in FP64 on the device.  The _dev form reads
+ * nProj angles; the host form reads n_angles, of which the Parker window
+ * takes the scan direction and extent (diff(angles), angles(end)). */
+#define MEXCT_NO_PARKER (-1.0)
 int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int32_t geom,
                     double sad, const float* angles, int64_t n_angles, int32_t window,
-                    double cutoff, float* out);
+                    double cutoff, double parker_q, float* out);
 int mexct_prefilter_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H, int64_t nProj,
                         int32_t geom, double sad, const float* angles, int32_t window,
-                        double cutoff, float* out);
+                        double cutoff, double parker_q, float* out);
+
+/* ------------------------------------------------------------------ Parker weights
+ * weightedSino = parker_weight(sino, angles, geom, q)
+ * replaces fbp_reconstruction/parker_weight.m:1-79 (Wesarg et al. 2002;
+ * q = 1 is Parker 1982).  q >= 0; SAD is ignored for par3d (alpha = 0).
+ * The window runs along the detector columns u; the reference computes it
+ * over size(sino,2) and needs W == H, where the two agree.  The
+ * reference's warnings and disp lines are not reproduced. */
+int mexct_parker_weight(const float* sino, int64_t W, int64_t H, int64_t nProj,
+                        const float* angles, int64_t n_angles, int32_t geom, double sad, double q,
+                        float* out);
+int mexct_parker_weight_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H,
+                            int64_t nProj, const float* angles, int32_t geom, double sad, double q,
+                            float* out);
+
+/* ------------------------------------------------------------------ FBP driver
+ * recon = FBP_recon(pre, post, scanAngles, geom, filter, filterCutoff, parker_q)
+ * replaces fbp_reconstruction/FBP_recon.m:1-40: -log(post./pre) with NaN
+ * and Inf set to 0 (a negative ratio, complex in MATLAB, gives 0), the
+ * pre-filter above, then CUDAmex_BP of the geometry into vol [W,W,H].
+ * The log transform is fused into the filter's staging.  opts selects the
+ * backprojector's storage and z-range. */
+int mexct_fbp_recon(const float* pre, const float* post, int64_t W, int64_t H, int64_t nProj,
+                    const float* angles, int64_t n_angles, int32_t geom, double sad,
+                    int32_t window, double cutoff, double parker_q, float* vol,
+                    const mexct_opts* opts);
+int mexct_fbp_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, int64_t W,
+                        int64_t H, int64_t nProj, const float* angles, int32_t geom, double sad,
+                        int32_t window, double cutoff, double parker_q, float* vol,
+                        const mexct_opts* opts);
 
 /* ------------------------------------------------------------------ OSC-TV driver
  * recon = OSC_TV_recon(pre, post, angles, geom, nIter, TV, sInit, sFinal, p)

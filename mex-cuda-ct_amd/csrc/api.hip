@@ -82,6 +82,8 @@ enum Slot {
   WS_CHAIN,    // cone f32 BP: row coordinates at the z-range starts
   WS_MUZ,      // z-fastest copy of the OSC volume
   WS_MAXBITS,  // max|x| bits for the FP16 copies: [1] volume, [2] sinogram
+  WS_PARKER,   // Parker window: column terms, view betas, then [nProj][W] weights
+  WS_FILTERED, // FBP: the filtered sinogram handed to the backprojector
   WS_COUNT
 };
 
@@ -121,6 +123,11 @@ struct mexct_ctx {
   int w_W = -1, w_H = -1, w_geom = -1;
   double w_sad = 0.0;
   std::vector<float> dA_key;  // angle-step table last uploaded to WS_DA
+  // Parker window cache (WS_PARKER), keyed by everything it depends on
+  std::vector<float> pk_angles;
+  int64_t pk_W = -1, pk_n = -1;
+  int pk_geom = -1;
+  double pk_sad = 0.0, pk_q = -1.0;
 
   int get(int slot, size_t bytes, void** out) {
     Buf& b = ws[slot];
@@ -926,18 +933,81 @@ int mexct_tvmin(const float* vol, int64_t Nx, int64_t Ny, int64_t Nz, float cc, 
 }
 
 // ------------------------------------------------------------------ FDK filter
-int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
-                        int32_t geom, double sad, const float* angles, int32_t window,
-                        double cutoff, float* out) {
-  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+namespace {
+
+bool parker_on(double q) { return q >= 0.0; }  // negative or NaN: the 5-argument call
+
+// The Parker / Wesarg window of the first nProj views (parker_weight.m:29-76)
+// in WS_PARKER, as [nProj][W] floats at *win.  n_all >= nProj angles set the
+// scan direction and extent (diff(angles), angles(end)).  The geometry terms
+// are formed here in MATLAB's single/double mix -- alpha and delta double,
+// anything combined with the single angles single -- and the window itself
+// on the device.  alpha runs along the detector columns (the fan direction);
+// the reference takes size(sino,2) entries and expands with repmat (:32,:73),
+// the same window when W == H.
+int parker_window(mexct_ctx* c, int64_t W, int64_t nProj, int64_t n_all, int geom, double sad,
+                  const float* angles, double q, const float** win) {
+  const size_t col_b = (size_t)W * sizeof(float4);
+  const size_t beta_b = ((size_t)nProj * sizeof(float) + 255) & ~(size_t)255;
+  void* base;
+  RC(c->get(WS_PARKER, col_b + beta_b + (size_t)W * nProj * sizeof(float), &base));
+  float* w = (float*)((char*)base + col_b + beta_b);
+  *win = w;
+  const bool same = c->pk_W == W && c->pk_n == nProj && c->pk_geom == geom && c->pk_sad == sad &&
+                    c->pk_q == q && c->pk_angles.size() == (size_t)n_all &&
+                    memcmp(c->pk_angles.data(), angles, (size_t)n_all * sizeof(float)) == 0;
+  if (same) return MEXCT_OK;
+  std::vector<double> alpha((size_t)W, 0.0);
+  if (geom != GEOM_PAR)
+    for (int64_t u = 0; u < W; ++u) alpha[u] = atan((-(double)W / 2 + 0.5 + (double)u) / sad);
+  bool increasing = true;
+  for (int64_t i = 1; i < n_all; ++i)
+    if (!(angles[i] - angles[i - 1] >= 0.0f)) increasing = false;
+  if (increasing)
+    for (double& a : alpha) a = -a;
+  const double delta = fabs(alpha[W - 1] - alpha[0]) / 2;
+  const float extent = fabsf(angles[n_all - 1] - angles[0]);
+  ParkerConsts k;
+  k.pi_f = (float)M_PI;
+  k.d2 = (float)(2 * delta);
+  k.eps = std::max(extent - (float)(M_PI + 2 * delta), 0.0f);
+  const float qf = (float)q;
+  std::vector<char> h(col_b + beta_b, 0);
+  float4* col = (float4*)h.data();
+  for (int64_t u = 0; u < W; ++u)
+    col[u] = make_float4((float)(2 * alpha[u]), qf * ((float)(2 * delta - 2 * alpha[u]) + k.eps),
+                         qf * ((float)(2 * delta + 2 * alpha[u]) + k.eps), 0.f);
+  float* beta = (float*)(h.data() + col_b);
+  for (int64_t i = 0; i < nProj; ++i) beta[i] = fabsf(angles[i] - angles[0]);
+  c->pk_W = -1;  // invalid until the kernel is queued
+  RC(c->upload(base, h.data(), h.size()));
+  HIPCHK(launch_parker((const float4*)base, (const float*)((char*)base + col_b), k, (int)W,
+                       (int)nProj, w, c->stream));
+  c->pk_angles.assign(angles, angles + n_all);
+  c->pk_W = W;
+  c->pk_n = nProj;
+  c->pk_geom = geom;
+  c->pk_sad = sad;
+  c->pk_q = q;
+  return MEXCT_OK;
+}
+
+int check_prefilter_args(int32_t geom, int64_t W, int64_t H, int64_t nProj, int32_t window) {
   RC(check_geom(geom));
   if (W < 1 || H < 1) return fail(MEXCT_EINVAL, "bad dims");
   if (nProj < 2) return fail(MEXCT_EINVAL, "the angle step needs at least 2 views");
-  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
   if (window < 0 || window > 5) return fail(MEXCT_EINVAL, "unknown filter window %d", window);
   if (3 * W - 1 > 16384) return fail(MEXCT_EINVAL, "detector too wide for the LDS filter (W=%lld)", (long long)W);
-  DevGuard g(c->device);
-  void *gp, *wp = nullptr, *dp;
+  if ((double)H * nProj >= 2147483648.0) return fail(MEXCT_EINVAL, "too many detector rows (H*nProj)");
+  return MEXCT_OK;
+}
+
+// preFilterSinogram.m:29-90 on in.sino (or on -log(in.post ./ in.sino)),
+// the first nProj of n_all angles.
+int prefilter_run(mexct_ctx* c, FilterIn in, int64_t W, int64_t H, int64_t nProj, int64_t n_all,
+                  int32_t geom, double sad, const float* angles, int32_t window, double cutoff,
+                  double parker_q, float* out) {
+  void *gp, *dp;
   RC(c->get(WS_FILT, sizeof(double) * (2 * W - 1), &gp));
   if (c->f_W != W || c->f_window != window || c->f_cut != cutoff) {
     std::vector<double> full, taps;
@@ -948,7 +1018,9 @@ int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, i
     c->f_window = window;
     c->f_cut = cutoff;
   }
+  in.wgt = nullptr;
   if (geom != GEOM_PAR) {
+    void* wp;
     RC(c->get(WS_WGT, sizeof(float) * W * H, &wp));
     if (c->w_W != W || c->w_H != H || c->w_geom != geom || c->w_sad != sad) {
       // Distance = SAD ./ sqrt(SAD^2 + U.^2 [+ V.^2]), U = (1..W) - (W-1)/2
@@ -968,7 +1040,10 @@ int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, i
       c->w_geom = geom;
       c->w_sad = sad;
     }
+    in.wgt = (const float*)wp;
   }
+  in.pw = nullptr;
+  if (parker_on(parker_q)) RC(parker_window(c, W, nProj, n_all, geom, sad, angles, parker_q, &in.pw));
   // (angles(i) - angles(i-1)) in single, view 1 takes the next step (:80-84)
   std::vector<float> dA((size_t)nProj);
   for (int64_t i = 0; i < nProj; ++i) dA[i] = i > 0 ? angles[i] - angles[i - 1] : angles[1] - angles[0];
@@ -979,16 +1054,29 @@ int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, i
     RC(c->upload(dp, dA.data(), dA.size() * sizeof(float)));
     c->dA_key.swap(dA);
   }
-  HIPCHK(launch_prefilter(sino, out, (const float*)wp, (const double*)gp, (const float*)dp, (int)W,
-                          (int)H, (int)nProj, c->stream));
+  HIPCHK(launch_prefilter(in, out, (const double*)gp, (const float*)dp, (int)W, (int)H,
+                          (int)nProj, c->stream));
   return MEXCT_OK;
+}
+
+}  // namespace
+
+int mexct_prefilter_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                        int32_t geom, double sad, const float* angles, int32_t window,
+                        double cutoff, double parker_q, float* out) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_prefilter_args(geom, W, H, nProj, window));
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  return prefilter_run(c, FilterIn{sino, nullptr, nullptr, nullptr}, W, H, nProj, nProj, geom,
+                       sad, angles, window, cutoff, parker_q, out);
 }
 
 int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int32_t geom,
                     double sad, const float* angles, int64_t n_angles, int32_t window,
-                    double cutoff, float* out) {
+                    double cutoff, double parker_q, float* out) {
   if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
-  if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  RC(check_prefilter_args(geom, W, H, nProj, window));
   if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
   mexct_ctx* c;
   RC(default_ctx(&c));
@@ -998,9 +1086,105 @@ int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int3
   RC(c->get(WS_HOST_A, sb, &ds));
   RC(c->get(WS_HOST_OUT, sb, &dout));
   HIPCHK(hipMemcpyAsync(ds, sino, sb, hipMemcpyHostToDevice, c->stream));
-  RC(mexct_prefilter_dev(c, (const float*)ds, W, H, nProj, geom, sad, angles, window, cutoff,
-                         (float*)dout));
+  RC(prefilter_run(c, FilterIn{(const float*)ds, nullptr, nullptr, nullptr}, W, H, nProj,
+                   n_angles, geom, sad, angles, window, cutoff, parker_q, (float*)dout));
   HIPCHK(hipMemcpyAsync(out, dout, sb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ Parker weights
+namespace {
+int check_parker_args(int32_t geom, int64_t W, int64_t H, int64_t nProj, double sad, double q) {
+  RC(check_geom(geom));
+  if (W < 1 || H < 1 || nProj < 1) return fail(MEXCT_EINVAL, "bad dims");
+  if (!parker_on(q)) return fail(MEXCT_EINVAL, "the Parker q must be >= 0");
+  if (geom != GEOM_PAR && !(sad > 0.0)) return fail(MEXCT_EINVAL, "SAD must be > 0");
+  return MEXCT_OK;
+}
+}  // namespace
+
+int mexct_parker_weight_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
+                            const float* angles, int32_t geom, double sad, double q, float* out) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_parker_args(geom, W, H, nProj, sad, q));
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  const float* win;
+  RC(parker_window(c, W, nProj, nProj, geom, sad, angles, q, &win));
+  HIPCHK(launch_apply_window(sino, win, out, (int)W, (int)H, (int)nProj, c->stream));
+  return MEXCT_OK;
+}
+
+int mexct_parker_weight(const float* sino, int64_t W, int64_t H, int64_t nProj,
+                        const float* angles, int64_t n_angles, int32_t geom, double sad, double q,
+                        float* out) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  RC(check_parker_args(geom, W, H, nProj, sad, q));
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t sb = (size_t)W * H * nProj * 4;
+  void *ds, *dout;
+  RC(c->get(WS_HOST_A, sb, &ds));
+  RC(c->get(WS_HOST_OUT, sb, &dout));
+  HIPCHK(hipMemcpyAsync(ds, sino, sb, hipMemcpyHostToDevice, c->stream));
+  const float* win;
+  RC(parker_window(c, W, nProj, n_angles, geom, sad, angles, q, &win));
+  HIPCHK(launch_apply_window((const float*)ds, win, (float*)dout, (int)W, (int)H, (int)nProj,
+                             c->stream));
+  HIPCHK(hipMemcpyAsync(out, dout, sb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ FBP
+namespace {
+int fbp_run(mexct_ctx* c, const float* pre, const float* post, int64_t W, int64_t H,
+            int64_t nProj, int64_t n_all, const float* angles, int32_t geom, double sad,
+            int32_t window, double cutoff, double parker_q, float* vol, const mexct_opts* opts) {
+  void* fs;
+  RC(c->get(WS_FILTERED, (size_t)W * H * nProj * 4, &fs));
+  RC(prefilter_run(c, FilterIn{pre, post, nullptr, nullptr}, W, H, nProj, n_all, geom, sad,
+                   angles, window, cutoff, parker_q, (float*)fs));
+  // CUDAmex_BP(filteredSino, geom.SAD, scanAngles, geomFlag) (FBP_recon.m:32-38)
+  return mexct_bp_dev(c, fs, MEXCT_F32, W, H, nProj, (float)sad, angles, geom, vol, opts);
+}
+}  // namespace
+
+int mexct_fbp_recon_dev(mexct_ctx* c, const float* pre, const float* post, int64_t W, int64_t H,
+                        int64_t nProj, const float* angles, int32_t geom, double sad,
+                        int32_t window, double cutoff, double parker_q, float* vol,
+                        const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(check_prefilter_args(geom, W, H, nProj, window));
+  if (!pre || !post || !vol || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  return fbp_run(c, pre, post, W, H, nProj, nProj, angles, geom, sad, window, cutoff, parker_q,
+                 vol, opts);
+}
+
+int mexct_fbp_recon(const float* pre, const float* post, int64_t W, int64_t H, int64_t nProj,
+                    const float* angles, int64_t n_angles, int32_t geom, double sad,
+                    int32_t window, double cutoff, double parker_q, float* vol,
+                    const mexct_opts* opts) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  RC(check_prefilter_args(geom, W, H, nProj, window));
+  if (!pre || !post || !vol || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t sb = (size_t)W * H * nProj * 4, vb = (size_t)W * W * H * 4;
+  void *dpre, *dpost, *dv;
+  RC(c->get(WS_HOST_A, sb, &dpre));
+  RC(c->get(WS_HOST_B, sb, &dpost));
+  RC(c->get(WS_HOST_OUT, vb, &dv));
+  HIPCHK(hipMemcpyAsync(dpre, pre, sb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dpost, post, sb, hipMemcpyHostToDevice, c->stream));
+  RC(fbp_run(c, (const float*)dpre, (const float*)dpost, W, H, nProj, n_angles, angles, geom, sad,
+             window, cutoff, parker_q, (float*)dv, opts));
+  HIPCHK(hipMemcpyAsync(vol, dv, vb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MEXCT_OK;
 }

@@ -165,10 +165,32 @@ hipError_t launch_tv_const(const double* sum, float tv, float* c_dev, hipStream_
 hipError_t launch_clean(Vol3 x, hipStream_t st);
 
 // ---- FDK prefilter (fdk.hip)
+// The staged sample: x = sino[e], or -log(post[e] / sino[e]) with NaN/Inf
+// -> 0 when post is set; then x * wgt[v][u] if wgt is set, then
+// x * pw[view][u] if pw is set, each rounded to float.
+struct FilterIn {
+  const float* sino;  // the sinogram, or pre when post is set
+  const float* post;  // FBP log-transform input, or null
+  const float* wgt;   // cosine weights [H][W], or null
+  const float* pw;    // Parker window [nviews][W], or null
+};
 // out = ((float)(sum_u' x[u'] g[u-u'+W-1]) * dA[view]) / 2 per detector row,
-// x = sino * wgt (float) promoted to double; wgt may be null (no weight).
-hipError_t launch_prefilter(const float* sino, float* out, const float* wgt, const double* g,
-                            const float* dA, int W, int H, int nviews, hipStream_t st);
+// x promoted to double.
+hipError_t launch_prefilter(const FilterIn& in, float* out, const double* g, const float* dA,
+                            int W, int H, int nviews, hipStream_t st);
+// Parker / Wesarg window: single-precision constants of the scan
+struct ParkerConsts {
+  float pi_f;  // (float)pi
+  float d2;    // (float)(2 delta)
+  float eps;   // epsilon = max(extent - (pi + 2 delta), 0), single
+};
+// win[i*W + u] from col[u] = {(float)(2 alpha_u), b(alpha_u), b(-alpha_u), 0}
+// and beta[i] = |angle_i - angle_0|.
+hipError_t launch_parker(const float4* col, const float* beta, const ParkerConsts& k, int W,
+                         int nviews, float* win, hipStream_t st);
+// out[u,v,view] = sino[u,v,view] * win[view*W + u]
+hipError_t launch_apply_window(const float* sino, const float* win, float* out, int W, int H,
+                               int nviews, hipStream_t st);
 
 // ---- gather microbenchmark (probe.hip)
 // pattern 0: the FP's quad-volume loads; 1: the BP's row-paired loads.

@@ -195,7 +195,7 @@ int run_one(Worker& w, Req& q) {
       if (r.op == MEXCT_OP_FDK) {
         TRY(grow(w.x, sb));
         TRY(mexct_prefilter_dev(w.ctx, (const float*)w.a.p, r.W, r.H, r.nProj, r.geom, r.sad,
-                                ang, r.window, r.cutoff, (float*)w.x.p));
+                                ang, r.window, r.cutoff, MEXCT_NO_PARKER, (float*)w.x.p));
         bp_in = w.x.p;
       }
       TRY(mexct_bp_dev(w.ctx, bp_in, MEXCT_F32, r.W, r.H, r.nProj, r.sad, ang, r.geom,

@@ -5,7 +5,7 @@ Host-array mirror of the MEX calls (numpy, MATLAB layout):
     CUDAmex_FP, CUDAmex_BP, CUDAmex_general3D_FP, CUDAmex_general3D_BP,
     CUDAmex_oscIter, CUDAmex_TVmin_3D,
 and of the MATLAB drivers around them:
-    preFilterSinogram, OSC_TV_recon, FBP_recon.
+    preFilterSinogram, parker_weight, OSC_TV_recon, FBP_recon.
 Device-resident use: mexct.device.Context.
 All compute runs in libmexct.so (HIP); there is no CPU fallback.
 """
@@ -24,5 +24,6 @@ from .ops import (  # noqa: F401
     calc_num_subsets,
     calc_subset_order,
     first_picks,
+    parker_weight,
     preFilterSinogram,
 )

@@ -65,8 +65,13 @@ SIGNATURES = {
     "mexct_osc_iter_dev": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _i32, _vp, _vp, _f, _i32, _op],
     "mexct_tvmin": [_vp, _i64, _i64, _i64, _f, _vp, _op],
     "mexct_tvmin_dev": [_vp, _vp, _i64, _i64, _i64, _f, _op],
-    "mexct_prefilter": [_vp, _i64, _i64, _i64, _i32, _d, _vp, _i64, _i32, _d, _vp],
-    "mexct_prefilter_dev": [_vp, _vp, _i64, _i64, _i64, _i32, _d, _vp, _i32, _d, _vp],
+    "mexct_prefilter": [_vp, _i64, _i64, _i64, _i32, _d, _vp, _i64, _i32, _d, _d, _vp],
+    "mexct_prefilter_dev": [_vp, _vp, _i64, _i64, _i64, _i32, _d, _vp, _i32, _d, _d, _vp],
+    "mexct_parker_weight": [_vp, _i64, _i64, _i64, _vp, _i64, _i32, _d, _d, _vp],
+    "mexct_parker_weight_dev": [_vp, _vp, _i64, _i64, _i64, _vp, _i32, _d, _d, _vp],
+    "mexct_fbp_recon": [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i32, _d, _i32, _d, _d, _vp, _op],
+    "mexct_fbp_recon_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i32, _d, _i32, _d, _d, _vp,
+                            _op],
     "mexct_osc_tv_recon": [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _f, _i32, _i32, _f, _i32, _i32,
                            _d, _vp, _vp, _op],
     "mexct_osc_tv_recon_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _f, _i32, _i32, _f, _i32,

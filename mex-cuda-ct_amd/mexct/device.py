@@ -163,12 +163,30 @@ class Context:
                    opts(F32, int(steps)))
         return vol
 
-    def prefilter(self, sino, geom, sad, angles, window, cutoff, out):
+    def prefilter(self, sino, geom, sad, angles, window, cutoff, out, parker_q=None):
         n, H, W = sino.shape
         ang = _host_f32(angles)
+        q = -1.0 if parker_q is None else float(parker_q)
         self._call(lib().mexct_prefilter_dev, self._h, _dptr(sino, "sino", True), W, H, n,
-                   int(geom), float(sad), _p(ang), int(window), float(cutoff),
+                   int(geom), float(sad), _p(ang), int(window), float(cutoff), q,
                    _dptr(out, "out", True))
+        return out
+
+    def parker_weight(self, sino, angles, geom, sad, q, out):
+        n, H, W = sino.shape
+        ang = _host_f32(angles)
+        self._call(lib().mexct_parker_weight_dev, self._h, _dptr(sino, "sino", True), W, H, n,
+                   _p(ang), int(geom), float(sad), float(q), _dptr(out, "out", True))
+        return out
+
+    def fbp_recon(self, pre, post, angles, geom, sad, window, cutoff, out, parker_q=None,
+                  storage=F32):
+        n, H, W = pre.shape
+        ang = _host_f32(angles)
+        q = -1.0 if parker_q is None else float(parker_q)
+        self._call(lib().mexct_fbp_recon_dev, self._h, _dptr(pre, "pre", True),
+                   _dptr(post, "post", True), W, H, n, _p(ang), int(geom), float(sad),
+                   int(window), float(cutoff), q, _dptr(out, "out", True), opts(int(storage)))
         return out
 
     def osc_tv_recon(self, pre, post, angles, sad, geom, n_iter, tv, s_init, s_final, p,

@@ -146,18 +146,40 @@ def CUDAmex_TVmin_3D(vol, c, steps=20):
     return out
 
 
-def preFilterSinogram(sino, geom, angles, filter="hamming", frequencyCutoff=1.0):
-    """filtered = preFilterSinogram(sino, geom, angles, filter, cutoff) without
-    Parker weighting (preFilterSinogram.m:29-152).  geom is a dict with
-    'type' ('par3d' | 'fan3d' | 'cone3d') and 'SAD'."""
+def _parker_q(q):
+    return -1.0 if q is None else float(q)
+
+
+def _geom_sad(geom):
+    g = GEOM[str(geom["type"]).lower()]
+    return g, float(geom.get("SAD", 0.0))
+
+
+def preFilterSinogram(sino, geom, angles, filter="hamming", frequencyCutoff=1.0, parker_q=None):
+    """filtered = preFilterSinogram(sino, geom, angles, filter, cutoff[, parker_q])
+    (preFilterSinogram.m:29-152).  geom is a dict with 'type' ('par3d' |
+    'fan3d' | 'cone3d') and 'SAD'.  parker_q None is the 5-argument call
+    (no Parker weighting)."""
     sino = _single(sino, "sino")
     W, H, n = _dims3(sino)
-    g = GEOM[str(geom["type"]).lower()]
+    g, sad = _geom_sad(geom)
     w = WINDOWS[str(filter).lower()]
     ang = _vec(angles, np.float32, "angles")
     out = np.zeros((W, H, n), dtype=np.float32, order="F")
-    check(lib().mexct_prefilter(_ptr(sino), W, H, n, g, float(geom.get("SAD", 0.0)), _ptr(ang),
-                                ang.size, w, float(frequencyCutoff), _ptr(out)))
+    check(lib().mexct_prefilter(_ptr(sino), W, H, n, g, sad, _ptr(ang), ang.size, w,
+                                float(frequencyCutoff), _parker_q(parker_q), _ptr(out)))
+    return out
+
+
+def parker_weight(sino, angles, geom, q_value):
+    """weightedSino = parker_weight(sino, angles, geom, q_value) (parker_weight.m:1-79)."""
+    sino = _single(sino, "sino")
+    W, H, n = _dims3(sino)
+    g, sad = _geom_sad(geom)
+    ang = _vec(angles, np.float32, "angles")
+    out = np.zeros((W, H, n), dtype=np.float32, order="F")
+    check(lib().mexct_parker_weight(_ptr(sino), W, H, n, _ptr(ang), ang.size, g, sad,
+                                    float(q_value), _ptr(out)))
     return out
 
 
@@ -201,14 +223,21 @@ def OSC_TV_recon(pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit,
     return mu
 
 
-def FBP_recon(pre, post, scanAngles, geom, filter="hamming", filterCutoff=1.0):
-    """recon = FBP_recon(pre, post, angles, geom, filter, cutoff) without Parker
-    weighting (FBP_recon.m:26-38): -log(post./pre), NaN/Inf -> 0, filter, BP."""
-    pre = np.asarray(pre, dtype=np.float32)
-    post = np.asarray(post, dtype=np.float32)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        sino = -np.log(post / pre)
-    sino[~np.isfinite(sino)] = 0
-    filt = preFilterSinogram(np.asfortranarray(sino, dtype=np.float32), geom, scanAngles, filter,
-                             filterCutoff)
-    return CUDAmex_BP(filt, geom.get("SAD", 0.0), scanAngles, GEOM[str(geom["type"]).lower()])
+def FBP_recon(pre, post, scanAngles, geom, filter="hamming", filterCutoff=1.0, parker_q=None,
+              storage=F32):
+    """recon = FBP_recon(pre, post, angles, geom, filter, cutoff[, parker_q])
+    (FBP_recon.m:1-40): -log(post./pre) with NaN/Inf -> 0, the pre-filter
+    and the BP of the geometry, all on the device."""
+    pre = _single(pre, "pre")
+    post = _single(post, "post")
+    W, H, n = _dims3(pre)
+    if post.shape != pre.shape:
+        raise MexctError(EINVAL, "pre and post must have the same shape")
+    g, sad = _geom_sad(geom)
+    w = WINDOWS[str(filter).lower()]
+    ang = _vec(scanAngles, np.float32, "angles")
+    vol = np.zeros((W, W, H), dtype=np.float32, order="F")
+    check(lib().mexct_fbp_recon(_ptr(pre), _ptr(post), W, H, n, _ptr(ang), ang.size, g, sad, w,
+                                float(filterCutoff), _parker_q(parker_q), _ptr(vol),
+                                opts(int(storage))))
+    return vol

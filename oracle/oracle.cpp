@@ -701,13 +701,67 @@ int filter_order(int len) {
   return std::max(64, o);
 }
 
+// S_function (parker_weight.m:82-89) on a single argument: 0 at or below
+// -1/2, 1 at or above 1/2, 0.5*(1+sin(pi*b)) between; NaN gives 0 (no mask
+// of the reference selects it).
+float parker_S(float b) {
+  if (b <= -0.5f) return 0.0f;
+  if (std::fabs(b) < 0.5f) return 0.5f * (1.0f + std::sin((float)M_PI * b));
+  if (b >= 0.5f) return 1.0f;
+  return 0.0f;
+}
+
+// parker_weight.m:29-76: the Parker / Wesarg window of every view, win[i*W+u].
+// The angles and all that is derived from them are single, alpha and delta
+// double; an operation with one single operand takes the other rounded to
+// single.  alpha runs along the detector columns u (the fan direction) with
+// W entries; the reference takes size(sino,2) entries and expands with
+// repmat (:32, :73), the same window when W == H (a size error otherwise).
+void parker_window(int W, int nProj, int geom, double sad, const float* angles, double q,
+                   float* win) {
+  std::vector<double> alpha((size_t)W, 0.0);
+  if (geom != 0)  // par3d: alpha = 0 (:29-30)
+    for (int u = 0; u < W; ++u) alpha[u] = std::atan((-(double)W / 2 + 0.5 + u) / sad);
+  bool nondecreasing = true;  // all(diff(angles) >= 0) flips alpha (:36-40)
+  for (int i = 1; i < nProj; ++i)
+    if (!(angles[i] - angles[i - 1] >= 0.0f)) nondecreasing = false;
+  if (nondecreasing)
+    for (double& a : alpha) a = -a;
+  const double delta = std::fabs(alpha[W - 1] - alpha[0]) / 2;            // :42
+  const float extent = std::fabs(angles[nProj - 1] - angles[0]);          // :43
+  const float eps = std::max(extent - (float)(M_PI + 2 * delta), 0.0f);   // :59
+  const float pi_f = (float)M_PI, d2 = (float)(2 * delta), qf = (float)q;
+  for (int i = 0; i < nProj; ++i) {
+    const float beta = std::fabs(angles[i] - angles[0]);  // :66
+    for (int u = 0; u < W; ++u) {
+      const float a2 = (float)(2 * alpha[u]);
+      // b(+-alpha) = q * (2 delta - 2 (+-alpha) + epsilon)  (:92-97)
+      const float bp = qf * ((float)(2 * delta - 2 * alpha[u]) + eps);
+      const float bm = qf * ((float)(2 * delta - 2 * -alpha[u]) + eps);
+      const float s1 = parker_S(beta / bp - 0.5f);
+      const float s2 = parker_S((((beta - d2) + a2) - eps) / bp + 0.5f);
+      const float s3 = parker_S(((beta - pi_f) + a2) / bm - 0.5f);
+      const float s4 = parker_S((((beta - pi_f) - d2) - eps) / bm + 0.5f);
+      win[(size_t)i * W + u] = 0.5f * (((s1 + s2) - s3) - s4);  // :68-71
+    }
+  }
+}
+
+bool parker_on(double q) { return q >= 0.0; }  // a negative (or NaN) q: no Parker step
+
 template <typename O>
 int prefilter(const float* sino, int W, int H, int nProj, int geom, double sad, const float* angles,
-              int window, double d, O* out) {
+              int window, double d, double parker_q, O* out) {
   if (nProj < 2) return -1;
   int order = filter_order(W);
   std::vector<double> filt(order);
   orc_design_filter(window, W, d, filt.data());
+  // Parker / Wesarg short-scan window (preFilterSinogram.m:36,48,60)
+  std::vector<float> win;
+  if (parker_on(parker_q)) {
+    win.resize((size_t)W * nProj);
+    parker_window(W, nProj, geom, sad, angles, parker_q, win.data());
+  }
   // cosine weights (preFilterSinogram.m:38-58): Distance in double, then
   // single .* double -> single (operand rounded to single first).
   std::vector<float> wgt((size_t)W * H, 1.0f);
@@ -727,6 +781,7 @@ int prefilter(const float* sino, int W, int H, int nProj, int geom, double sad, 
     const float* P = sino + (size_t)W * H * view + (size_t)W * v;
     for (int u = 0; u < W; ++u) {
       float w = (geom == 1 || geom == 2) ? P[u] * wgt[(size_t)u + (size_t)W * v] : P[u];
+      if (!win.empty()) w = w * win[(size_t)view * W + u];
       re[u] = (double)w;
     }
     fft_inplace(re, im, false);
@@ -956,12 +1011,28 @@ int orc_design_filter(int window, int len, double d, double* out) {
 }
 
 int orc_prefilter(const float* sino, int W, int H, int nProj, int geom, double sad,
-                  const float* angles, int window, double d, float* out) {
-  return prefilter<float>(sino, W, H, nProj, geom, sad, angles, window, d, out);
+                  const float* angles, int window, double d, double parker_q, float* out) {
+  return prefilter<float>(sino, W, H, nProj, geom, sad, angles, window, d, parker_q, out);
 }
 int orc_prefilter_f64(const float* sino, int W, int H, int nProj, int geom, double sad,
-                      const float* angles, int window, double d, double* out) {
-  return prefilter<double>(sino, W, H, nProj, geom, sad, angles, window, d, out);
+                      const float* angles, int window, double d, double parker_q, double* out) {
+  return prefilter<double>(sino, W, H, nProj, geom, sad, angles, window, d, parker_q, out);
+}
+
+int orc_parker_window(int W, int nProj, int geom, double sad, const float* angles, double q,
+                      float* win) {
+  if (W < 1 || nProj < 1) return -1;
+  parker_window(W, nProj, geom, sad, angles, q, win);
+  return 0;
+}
+
+void orc_log_transform(const float* pre, const float* post, int64_t n, float* out) {
+  // sino = -log(post./pre); NaN and +-Inf -> 0  (FBP_recon.m:26-28).  A
+  // negative ratio, complex in MATLAB, is NaN here and so 0.
+  for (int64_t i = 0; i < n; ++i) {
+    const float y = -std::log(post[i] / pre[i]);
+    out[i] = std::isfinite(y) ? y : 0.0f;
+  }
 }
 
 int orc_calc_num_subsets(int sInit, int sFinal, double p, int K, int k) {

@@ -103,12 +103,19 @@ void orc_tv_gradient(const float* x, int Nx, int Ny, int Nz, float* g);
  * filt[order] (double).  window: 0 ramp, 1 shepp-logan, 2 cosine,
  * 3 hamming, 4 hann, 5 none.  Pass filt = NULL to query order. */
 int orc_design_filter(int window, int len, double d, double* filt);
-/* preFilterSinogram.m:29-90 without Parker weighting.
- * geom: 0 par3d (no weight), 1 fan3d, 2 cone3d.  sino [W,H,nProj] -> out. */
+/* preFilterSinogram.m:29-90.  geom: 0 par3d (no cosine weight), 1 fan3d,
+ * 2 cone3d.  parker_q >= 0 applies parker_weight.m with q = parker_q after
+ * the cosine weight; parker_q < 0 skips it (the 5-argument call).
+ * sino [W,H,nProj] -> out. */
 int orc_prefilter(const float* sino, int W, int H, int nProj, int geom, double sad,
-                  const float* angles, int window, double d, float* out);
+                  const float* angles, int window, double d, double parker_q, float* out);
 int orc_prefilter_f64(const float* sino, int W, int H, int nProj, int geom, double sad,
-                      const float* angles, int window, double d, double* out);
+                      const float* angles, int window, double d, double parker_q, double* out);
+/* parker_weight.m:29-76: win[i*W + u], the weight of view i, column u. */
+int orc_parker_window(int W, int nProj, int geom, double sad, const float* angles, double q,
+                      float* win);
+/* FBP_recon.m:26-28: out = -log(post./pre) with NaN/Inf -> 0. */
+void orc_log_transform(const float* pre, const float* post, int64_t n, float* out);
 
 /* OSC_TV_recon.m:323-349 helpers. */
 int orc_calc_num_subsets(int sInit, int sFinal, double p, int K, int k);

@@ -64,8 +64,10 @@ def lib():
             "orc_tvmin_f64": (None, [_f64p, i, i, i, d, i]),
             "orc_tv_gradient": (None, [_f32p, i, i, i, _f32p]),
             "orc_design_filter": (i, [i, i, d, C.c_void_p]),
-            "orc_prefilter": (i, [_f32p, i, i, i, i, d, _f32p, i, d, _f32p]),
-            "orc_prefilter_f64": (i, [_f32p, i, i, i, i, d, _f32p, i, d, _f64p]),
+            "orc_prefilter": (i, [_f32p, i, i, i, i, d, _f32p, i, d, d, _f32p]),
+            "orc_prefilter_f64": (i, [_f32p, i, i, i, i, d, _f32p, i, d, d, _f64p]),
+            "orc_parker_window": (i, [i, i, i, d, _f32p, d, _f32p]),
+            "orc_log_transform": (None, [_f32p, _f32p, i64, _f32p]),
             "orc_calc_num_subsets": (i, [i, i, d, i, i]),
             "orc_calc_subset_order": (None, [i, i, _i32p]),
             "orc_osc_tv_recon": (i, [_f32p, _f32p, i, i, i, _f32p, f, i, i, f, i, i, d, i, _i32p,
@@ -255,21 +257,61 @@ def design_filter(window, length, d=1.0):
 GEOM_TYPES = {"par3d": 0, "fan3d": 1, "cone3d": 2, "cone": 2}
 
 
-def prefilter(sino, geom, sad, angles, window="hamming", d=1.0, f64=False):
-    """preFilterSinogram(sino, geom, angles, filter, cutoff) without Parker."""
+def _geom(geom):
+    return GEOM_TYPES[geom] if isinstance(geom, str) else int(geom)
+
+
+def prefilter(sino, geom, sad, angles, window="hamming", d=1.0, f64=False, parker_q=None):
+    """preFilterSinogram(sino, geom, angles, filter, cutoff[, parker_q]);
+    parker_q None is the 5-argument call (no Parker weighting)."""
     sino = _f32(sino)
     W, H, n = sino.shape
-    g = GEOM_TYPES[geom] if isinstance(geom, str) else int(geom)
+    g = _geom(geom)
     w = WINDOWS[window] if isinstance(window, str) else int(window)
+    q = -1.0 if parker_q is None else float(parker_q)
     if f64:
         out = np.zeros((W, H, n), dtype=np.float64, order="F")
-        rc = lib().orc_prefilter_f64(sino, W, H, n, g, float(sad), _angles(angles), w, float(d), out)
+        rc = lib().orc_prefilter_f64(sino, W, H, n, g, float(sad), _angles(angles), w, float(d), q,
+                                     out)
     else:
         out = np.zeros((W, H, n), dtype=np.float32, order="F")
-        rc = lib().orc_prefilter(sino, W, H, n, g, float(sad), _angles(angles), w, float(d), out)
+        rc = lib().orc_prefilter(sino, W, H, n, g, float(sad), _angles(angles), w, float(d), q, out)
     if rc:
         raise ValueError(f"orc_prefilter failed ({rc})")
     return out
+
+
+def parker_window(W, angles, geom, sad, q):
+    """parker_weight.m:29-76 window, shape (W, nProj): column u of view i."""
+    ang = _angles(angles)
+    out = np.zeros((W, ang.size), dtype=np.float32, order="F")
+    if lib().orc_parker_window(int(W), ang.size, _geom(geom), float(sad), ang, float(q), out):
+        raise ValueError("orc_parker_window failed")
+    return out
+
+
+def parker_weight(sino, angles, geom, sad, q):
+    """weightedSino = parker_weight(sino, angles, geom, q) (parker_weight.m:1-79)."""
+    sino = _f32(sino)
+    win = parker_window(sino.shape[0], angles, geom, sad, q)
+    return np.asfortranarray(sino * win[:, None, :])
+
+
+def log_transform(pre, post):
+    """-log(post./pre) with NaN/Inf -> 0 (FBP_recon.m:26-28)."""
+    pre = _f32(pre)
+    post = _f32(post)
+    out = np.zeros(pre.shape, dtype=np.float32, order="F")
+    lib().orc_log_transform(pre, post, pre.size, out)
+    return out
+
+
+def fbp_recon(pre, post, angles, geom, sad, window="hamming", d=1.0, parker_q=None):
+    """FBP_recon.m:26-38: log transform, pre-filter, then the BP of the
+    geometry (cone for 'cone'/'cone3d')."""
+    sino = log_transform(pre, post)
+    filt = prefilter(sino, geom, sad, angles, window, d, parker_q=parker_q)
+    return bp(filt, np.float32(sad), angles, _geom(geom))
 
 
 def calc_num_subsets(s_init, s_final, p, K, k):

@@ -93,6 +93,25 @@ def test_prefilter_needs_two_views():
                                 np.zeros(1, np.float32))
 
 
+def test_parker_and_fbp_argument_checks():
+    # rejected on the host before any device work
+    sino = _f32((8, 4, 3))
+    ang = np.zeros(3, np.float32)
+    g = {"type": "fan3d", "SAD": 100.0}
+    with pytest.raises(mexct.MexctError, match="q must be"):
+        mexct.parker_weight(sino, ang, g, -1.0)
+    with pytest.raises(mexct.MexctError, match="SAD"):
+        mexct.parker_weight(sino, ang, {"type": "fan3d", "SAD": 0.0}, 1.0)
+    with pytest.raises(mexct.MexctError, match="nProj exceeds"):
+        mexct.parker_weight(sino, ang[:2], g, 1.0)
+    with pytest.raises(mexct.MexctError, match="same shape"):
+        mexct.FBP_recon(sino, _f32((8, 4, 2)), ang, g)
+    with pytest.raises(mexct.MexctError, match="window"):
+        L.check(L.lib().mexct_fbp_recon(sino.ctypes.data, sino.ctypes.data, 8, 4, 3,
+                                        ang.ctypes.data, 3, 1, 100.0, 9, 1.0, -1.0,
+                                        sino.ctypes.data, None))
+
+
 def test_subset_helpers_match_oracle():
     import oracle as O
 

@@ -187,6 +187,94 @@ def test_prefilter(gpu, geom, window):
     _check(got, ref, f"prefilter {geom} {window}", 1e-6, 1e-7)
 
 
+def _short_scan(W, sad, extra, n, geom):
+    delta = np.arctan((W / 2 - 0.5) / sad) if geom != "par3d" else 0.0
+    return np.linspace(0, np.pi + 2 * delta + extra, n).astype(np.float32)
+
+
+@pytest.mark.parametrize("geom", ["par3d", "fan3d", "cone3d"])
+@pytest.mark.parametrize("q", [1.0, 0.1])
+def test_prefilter_parker(gpu, geom, q):
+    W, H, n, sad = 48, 6, 40, 80.0
+    sino = np.asfortranarray(_rng(82).uniform(size=(W, H, n)).astype(np.float32))
+    ang = _short_scan(W, sad, 0.1, n, geom)
+    got = mexct.preFilterSinogram(sino, {"type": geom, "SAD": sad}, ang, "hamming", 1.0,
+                                  parker_q=q)
+    ref = O.prefilter(sino, geom, sad, ang, "hamming", 1.0, parker_q=q)
+    _check(got, ref, f"prefilter+Parker {geom} q={q}", 1e-6, 1e-7)
+
+
+@pytest.mark.parametrize("case", ["fan-up", "fan-down", "cone-exact", "par-over", "longer-angles"])
+def test_parker_weight(gpu, case):
+    # the window on a sinogram of ones: device sinf against glibc's, within
+    # an ulp or two of the window's [0, 1] range.  fan-down runs the angles
+    # backwards (alpha not flipped); cone-exact has epsilon = 0 and the 0/0
+    # edge ray; longer-angles passes more angles than views (extent and
+    # direction from all of them).
+    W, H, sad = 64, 3, 50.0
+    geom = {"fan-up": "fan3d", "fan-down": "fan3d", "cone-exact": "cone3d",
+            "par-over": "par3d", "longer-angles": "fan3d"}[case]
+    ang = _short_scan(W, sad, 0.0 if case == "cone-exact" else 0.3, 120, geom)
+    if case == "fan-down":
+        ang = ang[::-1].copy()
+    n = 100 if case == "longer-angles" else ang.size
+    sino = np.asfortranarray(np.ones((W, H, n), np.float32))
+    got = mexct.parker_weight(sino, ang, {"type": geom, "SAD": sad}, 0.1)
+    win = O.parker_window(W, ang, geom, sad, 0.1)[:, :n]
+    ref = np.broadcast_to(win[:, None, :], (W, H, n))
+    d = np.abs(got - ref).max()
+    print(f"Parker window {case}: max|diff|={d:.3e} exact={np.mean(got == ref):.6f}")
+    assert d <= 3e-7
+    assert np.array_equal(got[:, 0, :], got[:, 2, :])
+    assert 0.02 < win.mean() < 0.98  # a real window, not all 0 or 1
+
+
+@pytest.mark.parametrize("geom", ["par3d", "fan3d", "cone3d"])
+@pytest.mark.parametrize("q", [None, 0.1])
+def test_fbp_recon(gpu, geom, q):
+    # FBP_recon: log transform (with zero, negative and infinite ratios),
+    # pre-filter with or without Parker, then the BP of the geometry
+    W, H, sad = 40, 8, 120.0
+    ang = _short_scan(W, sad, 0.2, 60, geom) if q is not None else _angles(60)
+    r = _rng(83)
+    pre = r.uniform(0.8, 1.2, (W, H, ang.size)).astype(np.float32)
+    post = (pre * np.exp(-r.uniform(0, 2, pre.shape))).astype(np.float32)
+    post[0, 0, :3] = 0          # -log(0) = Inf -> 0
+    pre[1, 0, :3] = 0           # x/0 -> -Inf -> 0
+    post[2, 1, :3] = -1         # negative ratio -> 0
+    pre = np.asfortranarray(pre)
+    post = np.asfortranarray(post)
+    g = {"type": geom, "SAD": sad}
+    got = mexct.FBP_recon(pre, post, ang, g, "hamming", 1.0, parker_q=q)
+    ref = O.fbp_recon(pre, post, ang, geom, sad, "hamming", 1.0, parker_q=q)
+    _check(got, ref, f"FBP_recon {geom} q={q}", 1e-5, 1e-6)
+
+
+def test_fbp_recon_device_context(gpu):
+    # the _dev entry on torch tensors, with F16 BP storage: against the host
+    # entry's F32 result on the same filtered data
+    import torch
+    from mexct import device
+    W, H, sad = 48, 16, 200.0
+    ang = _short_scan(W, sad, 0.1, 90, "cone3d")
+    r = _rng(84)
+    pre = np.asfortranarray(r.uniform(0.9, 1.1, (W, H, ang.size)).astype(np.float32))
+    post = np.asfortranarray((pre * np.exp(-r.uniform(0, 1, pre.shape))).astype(np.float32))
+    ref = mexct.FBP_recon(pre, post, ang, {"type": "cone3d", "SAD": sad}, "hamming", 1.0,
+                          parker_q=1.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).cuda()
+    ctx = device.Context(0)
+    out = torch.zeros((H, W, W), dtype=torch.float32, device="cuda")
+    ctx.fbp_recon(t(pre), t(post), ang, 2, sad, 3, 1.0, out, parker_q=1.0)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().transpose(2, 1, 0)
+    _check(got, ref, "FBP_recon _dev f32", 0.0, 0.0)
+    ctx.fbp_recon(t(pre), t(post), ang, 2, sad, 3, 1.0, out, parker_q=1.0, storage=mexct.F16)
+    torch.cuda.synchronize()
+    _check(out.cpu().numpy().transpose(2, 1, 0), ref, "FBP_recon _dev f16", 2e-3, 1e-3)
+    ctx.close()
+
+
 def test_osc_tv_recon(gpu):
     W, H, n = 24, 12, 12
     from mexct import phantoms
