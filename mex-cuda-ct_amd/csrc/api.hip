@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -261,6 +262,11 @@ int check_padded(int64_t a, int64_t b, int64_t c, const char* what) {
 }
 
 int storage_of(const mexct_opts* o) { return o ? o->storage : MEXCT_F32; }
+// FPZLaunch::long_ray from the options: 0 keeps the kernel's default
+int long_ray_of(const mexct_opts* o) {
+  if (!o || o->long_ray == 0) return 0;
+  return o->long_ray < 0 ? INT_MAX : o->long_ray;
+}
 
 // z-fastest FP volume layout for storage st (see fpz.hip): DUAL for f32,
 // QUAD for 2-byte storage while the copy stays below 4 GiB (32-bit buffer
@@ -461,6 +467,7 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
     z.H = (int)H;
     z.nviews = nProj;
     z.sad = sad;
+    z.long_ray = long_ray_of(opts);
     HIPCHK(launch_fpz(z, c->stream));
     return MEXCT_OK;
   }
@@ -637,6 +644,7 @@ int mexct_g3d_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t s
     z.NZ = (int)nSlices;
     z.pts = pts_dev;
     z.pts_view_stride = ppv ? (int64_t)6 * W * H : 0;
+    z.long_ray = long_ray_of(opts);
     HIPCHK(launch_fpz(z, c->stream));
     return MEXCT_OK;
   }

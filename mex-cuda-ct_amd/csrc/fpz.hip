@@ -35,6 +35,8 @@
 namespace mexct {
 
 constexpr uint32_t kViewChunk = 16;
+// default ray length (samples) above which 2-byte ray sums are compensated
+constexpr int kLongRay = 1024;
 
 struct FPZArgs {
   const void* vol;     // z-fastest haloed volume
@@ -56,6 +58,7 @@ struct FPZArgs {
   int64_t pts_view_stride;
   uint32_t n_ug, n_vg, n_logical;
   uint32_t nv;  // views of the launch
+  int long_ray;  // 2-byte storage: waves with a ray of more samples use compensated sums
 };
 
 template <typename S>
@@ -240,19 +243,17 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
 // One ray through the z-fastest volume, software-pipelined: the loads of
 // sample k+1 are issued before sample k is interpolated.  A fetch past the
 // last sample reads a clamped, in-range corner and is discarded.
-template <typename S, int STEP2, int LAYOUT>
-__device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
-                                         const R3& inc, int n) {
+// The sample loop.  COMP (2-byte storage, long rays): each pair of samples
+// is added to the ray sum with Kahan's compensated summation, so the sum's
+// rounding error stays near one ulp however many samples the ray has
+// instead of growing with their count.  Otherwise the samples are added one
+// by one in ray order, as the reference does.
+template <typename S, int STEP2, int LAYOUT, bool COMP>
+__device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
+                                            const R3& inc, int n, float fnx, float fny, float fnz,
+                                            int base) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
-  const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
-  const int base = 2 * a.PY * a.PZ + 2 * a.PZ + 2;
-  float dval = 0.f;
-  if (n <= 0) return dval;
-  if constexpr (sizeof(S) == 2) {  // texel coordinates (see fetch_z)
-    cur.x = (cur.x - a.bx) - 0.5f;
-    cur.y = (cur.y - a.by) - 0.5f;
-    cur.z = (cur.z - a.bz) - 0.5f;
-  }
+  float dval = 0.f, comp = 0.f;
   // two register sets in ping-pong: while one sample is interpolated the
   // next sample's loads are in flight
   Tap8<S> t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
@@ -263,16 +264,50 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     cur.z = cur.z + inc.z;
     const Tap8<S> t1 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
-    dval = dval + interp_z<S, LAYOUT>(t0) * step;
+    float x = interp_z<S, LAYOUT>(t0) * step;
+    if constexpr (!COMP) dval = dval + x;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
     t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);
-    dval = dval + interp_z<S, LAYOUT>(t1) * step;
+    if constexpr (COMP) {
+      x = x + interp_z<S, LAYOUT>(t1) * step;
+      const float y = x - comp;
+      const float t = dval + y;
+      comp = (t - dval) - y;
+      dval = t;
+    } else {
+      dval = dval + interp_z<S, LAYOUT>(t1) * step;
+    }
   }
-  if (k < n) dval = dval + interp_z<S, LAYOUT>(t0) * step;
+  if (k < n) {
+    const float x = interp_z<S, LAYOUT>(t0) * step;
+    if constexpr (COMP) {
+      const float y = x - comp;
+      dval = dval + y;
+    } else {
+      dval = dval + x;
+    }
+  }
   return dval;
+}
+
+template <typename S, int STEP2, int LAYOUT>
+__device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
+                                         const R3& inc, int n) {
+  const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
+  const int base = 2 * a.PY * a.PZ + 2 * a.PZ + 2;
+  if (n <= 0) return 0.f;
+  if constexpr (sizeof(S) == 2) {  // texel coordinates (see fetch_z)
+    cur.x = (cur.x - a.bx) - 0.5f;
+    cur.y = (cur.y - a.by) - 0.5f;
+    cur.z = (cur.z - a.bz) - 0.5f;
+    // one loop per wave: compensated when any of its rays is long
+    if (__builtin_amdgcn_ballot_w64(n > a.long_ray) != 0)
+      return march_loop<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
+  }
+  return march_loop<S, STEP2, LAYOUT, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
 }
 
 // Storage of the OSC ratio terms for a volume storage S.  They span the
@@ -396,6 +431,7 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   k.n_ug = (uint32_t)((W + 3) / 4);
   k.n_vg = (uint32_t)((H + 63) / 64);
   k.nv = (uint32_t)nviews;
+  k.long_ray = kLongRay;
   const uint32_t nvp = (k.nv + kViewChunk - 1) / kViewChunk * kViewChunk;
   k.n_logical = k.n_ug * k.n_vg * nvp;
   return k;
@@ -432,6 +468,7 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   k.vol_max = a.vol_max;
   k.pts = a.pts;
   k.pts_view_stride = a.pts_view_stride;
+  if (a.long_ray != 0) k.long_ray = a.long_ray;
   if (a.geom == GEOM_G3D) k.rows = a.H;  // every detector row is a ray
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
@@ -454,6 +491,7 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.stride = a.stride;
   k.PW = a.W + 2 * HALO;
   k.PV = (a.W + 2 * HALO) * (a.H + 2 * HALO);
+  if (a.long_ray != 0) k.long_ray = a.long_ray;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
     fpz_dispatch<1, float>(k, a.geom, a.layout, st);

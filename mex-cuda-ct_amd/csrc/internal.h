@@ -76,6 +76,7 @@ struct FPZLaunch {
   int NX = 0, NZ = 0;      // volume NX x NX x NZ (0: W x W x H, the standard geometries)
   const float* pts = nullptr;  // GEOM_G3D: ray endpoints [W,H,6] (+ view * pts_view_stride)
   int64_t pts_view_stride = 0;
+  int long_ray = 0;  // 2-byte storage: compensated sums above this many samples (0: 1024)
 };
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 
@@ -92,6 +93,7 @@ struct OSCFPLaunch {
   const float2* sc;
   int geom, W, H, nProj, first, stride, count;
   float sad;
+  int long_ray = 0;  // as FPZLaunch::long_ray
 };
 hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st);
 

@@ -26,7 +26,7 @@ class MexctError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("storage", C.c_int32), ("tv_steps", C.c_int32), ("zrange", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("long_ray", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -112,5 +112,5 @@ def check(rc: int) -> None:
         raise MexctError(rc, lib().mexct_last_error().decode(errors="replace"))
 
 
-def opts(storage: int = F32, tv_steps: int = 0, zrange: int = 0):
-    return C.byref(Opts(storage, tv_steps, zrange, 0))
+def opts(storage: int = F32, tv_steps: int = 0, zrange: int = 0, long_ray: int = 0):
+    return C.byref(Opts(storage, tv_steps, zrange, long_ray))

@@ -46,11 +46,13 @@ def _vec(a, dtype, name):
     return v
 
 
-def CUDAmex_FP(vol, SAD, angles, nProjections, geomFlag, storage=F32):
+def CUDAmex_FP(vol, SAD, angles, nProjections, geomFlag, storage=F32, long_ray=0):
     """fp = CUDAmex_FP(vol, SAD, angles, nProj, geomFlag) (CUDAmex_FP.cu:350-477).
 
     vol [W, W, H] -> fp [W, H, nProj].  SAD goes through float, nProj and
-    geomFlag through int, as mxGetScalar -> C conversion in the reference."""
+    geomFlag through int, as mxGetScalar -> C conversion in the reference.
+    long_ray: F16/BF16 storage sums rays of more samples with compensated
+    summation (0: the default 1024, negative: never)."""
     vol = _single(vol, "vol")
     W, Wy, H = _dims3(vol)
     if Wy != W:
@@ -59,7 +61,7 @@ def CUDAmex_FP(vol, SAD, angles, nProjections, geomFlag, storage=F32):
     nproj = int(nProjections)
     out = np.zeros((W, H, max(nproj, 0)), dtype=np.float32, order="F")
     check(lib().mexct_fp(_ptr(vol), W, H, float(np.float32(SAD)), _ptr(ang), ang.size, nproj,
-                         int(geomFlag), _ptr(out), opts(storage)))
+                         int(geomFlag), _ptr(out), opts(storage, long_ray=int(long_ray))))
     return out
 
 
@@ -76,7 +78,7 @@ def CUDAmex_BP(P, SAD, angles, geomFlag, storage=F32, zrange=0):
     return out
 
 
-def CUDAmex_general3D_FP(vol, angles, pts, nProjections, storage=F32):
+def CUDAmex_general3D_FP(vol, angles, pts, nProjections, storage=F32, long_ray=0):
     """fp = CUDAmex_general3D_FP(vol, angles, pts, nProj) (CUDAmex_general3D_FP.cu:154-252).
 
     pts [W, H, 6] is shared by every view and rotated by its angle; a
@@ -94,7 +96,7 @@ def CUDAmex_general3D_FP(vol, angles, pts, nProjections, storage=F32):
     ang = _vec(angles, np.float32, "angles")
     out = np.zeros((W, H, max(nproj, 0)), dtype=np.float32, order="F")
     check(lib().mexct_g3d_fp(_ptr(vol), side, nsl, _ptr(ang), ang.size, _ptr(pts), W, H, nproj,
-                             ppv, _ptr(out), opts(storage)))
+                             ppv, _ptr(out), opts(storage, long_ray=int(long_ray))))
     return out
 
 

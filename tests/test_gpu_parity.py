@@ -187,6 +187,34 @@ def test_prefilter(gpu, geom, window):
     _check(got, ref, f"prefilter {geom} {window}", 1e-6, 1e-7)
 
 
+def test_fp_long_ray_compensated_sum(gpu):
+    # beta = 0 parallel rays run along x through texel centres and midpoints
+    # (0.5 steps), so with zero x-faces each sample of an F16-exact volume is
+    # exact and the ray sum is the column sum, known exactly in FP64.  The
+    # 2048-sample rays exceed the default long-ray bound of 1024 samples: the
+    # compensated sum stays within a few ulps, the plain running sum (long_ray
+    # < 0) drifts with the sample count.
+    W, H = 1024, 2
+    r = _rng(85)
+    vol = np.zeros((W, W, H), np.float32, order="F")
+    vals = r.uniform(0.5, 1.0, (W - 2, W, H)) * np.exp2(-r.integers(0, 9, (W - 2, W, H)))
+    vol[1:-1] = vals.astype(np.float16).astype(np.float32)
+    ang = np.zeros(1, np.float32)
+    exact = vol.astype(np.float64).sum(axis=0)
+    comp = mexct.CUDAmex_FP(vol, 1000.0, ang, 1, 0, storage=mexct.F16)[:, :, 0]
+    plain = mexct.CUDAmex_FP(vol, 1000.0, ang, 1, 0, storage=mexct.F16, long_ray=-1)[:, :, 0]
+    e_comp = np.abs(comp - exact).max() / np.abs(exact).max()
+    e_plain = np.abs(plain - exact).max() / np.abs(exact).max()
+    print(f"2048-sample rays: compensated rel err {e_comp:.3e}, plain {e_plain:.3e}")
+    assert e_comp <= 3e-7
+    assert e_plain > 3 * e_comp
+    # a short-ray launch takes the plain loop whatever long_ray says
+    small = np.asfortranarray(vol[:64, :64, :])
+    a = mexct.CUDAmex_FP(small, 1000.0, ang, 1, 0, storage=mexct.F16)
+    b = mexct.CUDAmex_FP(small, 1000.0, ang, 1, 0, storage=mexct.F16, long_ray=-1)
+    assert np.array_equal(a, b)
+
+
 def _short_scan(W, sad, extra, n, geom):
     delta = np.arctan((W / 2 - 0.5) / sad) if geom != "par3d" else 0.0
     return np.linspace(0, np.pi + 2 * delta + extra, n).astype(np.float32)
