@@ -329,6 +329,25 @@ def other_workloads(ctx, stream, log):
     r = torch.empty((128, 128, 128), dtype=torch.float32, device=dev)
     res["parallel_128x180"] = {"fp_ms": dev_ms(lambda: ctx.fp(v, 1000.0, a, 180, 0, p)),
                                "bp_ms": dev_ms(lambda: ctx.bp(p, 1000.0, a, 0, r))}
+    log("[bench] other workloads: 8 parallel 128^3 x 180 in one batched launch vs 8 launches")
+    G = 8
+    vb = v.unsqueeze(0).repeat(G, 1, 1, 1).contiguous()
+    pb = torch.empty((G, 180, 128, 128), dtype=torch.float32, device=dev)
+    rb = torch.empty((G, 128, 128, 128), dtype=torch.float32, device=dev)
+
+    def fp_each():
+        for g in range(G):
+            ctx.fp(vb[g], 1000.0, a, 180, 0, pb[g])
+
+    def bp_each():
+        for g in range(G):
+            ctx.bp(pb[g], 1000.0, a, 0, rb[g])
+    res["batched_parallel_128x180x8"] = {
+        "fp_batch_ms": dev_ms(lambda: ctx.fp_batch(vb, 1000.0, a, 180, 0, pb)),
+        "fp_8_launches_ms": dev_ms(fp_each),
+        "bp_batch_ms": dev_ms(lambda: ctx.bp_batch(pb, 1000.0, a, 0, rb)),
+        "bp_8_launches_ms": dev_ms(bp_each)}
+    del vb, pb, rb
     log("[bench] other workloads: helical general-3D 320^3 x 360 x 256^2")
     side, nv, wd = 320, 360, 256
     vol = phantoms.ellipsoid_phantom(side, side, 4, 12, 140.0, 0.02, (8.0, 40.0), (0.01, 0.03),
@@ -363,6 +382,28 @@ def other_workloads(ctx, stream, log):
     res["osc_tv_256x60"] = osc
     log(f"[bench] other workloads: {res}")
     return res
+
+
+def same_geometry_burst(Server, phantoms, n=16, max_group=8):
+    """n same-geometry FP requests (parallel 128^3, 180 views) queued at once:
+    a server that batches up to max_group of them per launch against one that
+    runs them one by one (max_group 1)."""
+    vol = phantoms.par_phantom_128(seed=1)
+    ang = phantoms.angles_uniform(180, np.pi)
+    out = {"workload": f"{n} FP requests, parallel 128^3, 180 views, f32, queued at once",
+           "slots": 2}
+    for mg in (1, max_group):
+        with Server(hbm_budget=16 << 30, slots=2, max_group=mg) as s:
+            s.result(s.submit_fp(vol, 1000.0, ang, 180, 0))  # warm the workers
+            t0 = time.perf_counter()
+            ts = [s.submit_fp(vol, 1000.0, ang, 180, 0) for _ in range(n)]
+            for t in ts:
+                s.result(t)
+            wall = time.perf_counter() - t0
+            st = s.stats()
+        out[f"max_group_{mg}"] = {"throughput_req_s": round(n / wall, 2),
+                                  "batches": st["batches"], "batched": st["batched"]}
+    return out
 
 
 SERVE_SIDES = (128, 192, 256, 384, 512)
@@ -484,6 +525,9 @@ def serving(n_req, rate):
     lat_open, wall_open, st_open = served(True)
     log(f"[serve] open loop: p50 {np.percentile(lat_open, 50) * 1e3:.1f} ms "
         f"p99 {np.percentile(lat_open, 99) * 1e3:.1f} ms")
+    log("[serve] same-geometry burst")
+    burst = same_geometry_burst(Server, phantoms)
+    log(f"[serve] burst: {burst}")
     log("[serve] one-at-a-time FP32 loop")
     svc = []
     for r in reqs:
@@ -513,7 +557,9 @@ def serving(n_req, rate):
                       "peak_reserved_gb": round(st_open["peak_reserved_bytes"] / 1e9, 2)},
         "saturated": {"throughput_req_s": round(n_req / wall_sat, 3),
                       "p50_ms": pct(lat_sat, 50), "p99_ms": pct(lat_sat, 99),
-                      "groups": st_sat["groups"], "grouped": st_sat["grouped"]},
+                      "groups": st_sat["groups"], "grouped": st_sat["grouped"],
+                      "batched": st_sat["batched"]},
+        "same_geometry_burst": burst,
         "one_at_a_time_fp32": {"throughput_req_s": round(n_req / float(svc.sum()), 3),
                                "p50_ms": pct(lat_seq, 50), "p99_ms": pct(lat_seq, 99),
                                "note": "measured service times replayed through a FIFO "

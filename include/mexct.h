@@ -96,6 +96,36 @@ int mexct_bp_dev(mexct_ctx* ctx, const void* sino, int32_t sino_dtype, int64_t W
                  int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
                  const mexct_opts* opts);
 
+/* ------------------------------------------------------------------ batched FP / BP
+ * No counterpart in the reference (one MEX call per volume).  `count`
+ * (1..256) volumes or sinograms of one geometry, stored one after another,
+ * projected or backprojected by one kernel launch (per chunk of views for
+ * the BP); outputs one after another.  Item i's result has the same bits as
+ * the single call on item i. */
+int mexct_fp_batch_dev(mexct_ctx* ctx, const void* vols, int32_t vol_dtype, int32_t count,
+                       int64_t W, int64_t H, float sad, const float* angles, int32_t nProj,
+                       int32_t geom, float* outs, const mexct_opts* opts);
+int mexct_bp_batch_dev(mexct_ctx* ctx, const void* sinos, int32_t sino_dtype, int32_t count,
+                       int64_t W, int64_t H, int64_t nProj, float sad, const float* angles,
+                       int32_t geom, float* outs, const mexct_opts* opts);
+
+/* ------------------------------------------------------------------ autotuner
+ * No counterpart in the reference.  The FP and BP kernels' block order (FP:
+ * views per chunk; BP: super-tile side, and the z-range for FP32 storage
+ * unless opts fixes it) is tuned per problem shape (operator, geometry,
+ * storage, batch count, W, H, nProj): with tuning on, the first call of a
+ * shape times every candidate on its own data and keeps the fastest for
+ * the process.  Candidates write identical bits.  mode 1: tune at a
+ * shape's first call; 2: at its second (one-off shapes pay nothing; the
+ * request server's contexts use this); 0 off (default: built-in choices).
+ * A context's own setting (-1: follow the global one) overrides.  mexct_tuned_config fills cfg[4] = {found, FP
+ * views-per-chunk log2 or BP super-tile side, BP z-range, winner's time in
+ * microseconds}. */
+int mexct_set_autotune(int32_t mode);
+int mexct_ctx_set_autotune(mexct_ctx* ctx, int32_t mode);
+int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int32_t count, int64_t W,
+                       int64_t H, int64_t nProj, int32_t* cfg);
+
 /* ------------------------------------------------------------------ general-3D FP
  * fp = CUDAmex_general3D_FP(vol, angles, pts, nProjections)
  * replaces projection_kernels/CUDAmex_general3D_FP.cu:154-252 (kernel :59-150).
@@ -279,7 +309,8 @@ int mexct_server_submit(mexct_server* sv, const mexct_request* req, uint64_t* ti
 int mexct_server_wait(mexct_server* sv, uint64_t ticket, int32_t* status, double* latency_s,
                       double* queue_s);
 /* stats[0..n): done, groups, requests run in groups, queued, bytes reserved,
- * peak bytes reserved, workspace releases. */
+ * peak bytes reserved, workspace releases, requests run in batched
+ * launches, batched groups. */
 int mexct_server_stats(mexct_server* sv, int64_t* stats, int32_t n);
 /* Finishes running requests, drops queued ones, frees everything. */
 int mexct_server_destroy(mexct_server* sv);

@@ -15,9 +15,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "common.h"
@@ -129,6 +133,7 @@ struct mexct_ctx {
   int64_t pk_W = -1, pk_n = -1;
   int pk_geom = -1;
   double pk_sad = 0.0, pk_q = -1.0;
+  int autotune = -1;  // -1: follow mexct_set_autotune; 0 off; 1 tune unseen shapes
 
   int get(int slot, size_t bytes, void** out) {
     Buf& b = ws[slot];
@@ -297,14 +302,19 @@ hipError_t build_fp_volume(int layout, const void* in, int in_dt, int64_t sy, in
   return launch_pad3z(in, in_dt, sy, sz, out, st, (int)W, (int)W, (int)H, mode, s, scale);
 }
 
-// Device word holding max|x| of n floats (as float bits), slot k of
-// WS_MAXBITS, for the power-of-two scaling of FP16 copies.
-int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out) {
+// Device words holding max|x| of each of `items` arrays of n floats, `stride`
+// floats apart (as float bits), from slot k of WS_MAXBITS on, for the
+// power-of-two scaling of FP16 copies.  Slots: 1 the OSC volume, then
+// kMaxItems each for FP volumes and BP sinograms.
+constexpr int kMaxItems = 256;  // batch items per launch
+constexpr int kSlotVol = 64, kSlotSino = 64 + kMaxItems;
+int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out, int items = 1,
+             size_t stride = 0) {
   void* m;
-  RC(c->get(WS_MAXBITS, 256, &m));
+  RC(c->get(WS_MAXBITS, 4 * (64 + 2 * kMaxItems), &m));
   uint32_t* p = (uint32_t*)m + k;
-  HIPCHK(launch_fill((float*)p, 0.f, 1, c->stream));
-  HIPCHK(launch_absmax(x, n, p, c->stream));
+  HIPCHK(launch_fill((float*)p, 0.f, (size_t)items, c->stream));
+  for (int g = 0; g < items; ++g) HIPCHK(launch_absmax(x + g * stride, n, p + g, c->stream));
   *out = p;
   return MEXCT_OK;
 }
@@ -315,6 +325,89 @@ int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
   if (o && (o->zrange == 32 || o->zrange == 64)) return o->zrange;
   (void)geom;
   return H >= 256 ? 64 : 32;
+}
+
+// ---- autotuner of the kernels' block order
+//
+// Candidates differ only in which workgroup computes which rays or voxels
+// (FP: views per chunk of the block order; BP: super-tile side, and the
+// z-range for FP32 storage, whose row starts replay the same running sum
+// whatever the range), so every candidate writes the same bits.  The first
+// call of a problem shape times each candidate on that call's own data (the
+// launches are idempotent, so the last one leaves a valid result) and keeps
+// the fastest; later calls of the shape use it.
+struct TuneKey {
+  int op, geom, st, items;
+  int64_t W, H, n;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(op, geom, st, items, W, H, n) < std::tie(o.op, o.geom, o.st, o.items, o.W, o.H, o.n);
+  }
+};
+struct TuneVal {
+  int a = -1, b = -1;  // FP: vchunk_log2; BP: super-tile side, z-range
+  float ms = 0.f;      // the winner's time
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+std::map<TuneKey, int> g_seen;  // calls per untuned shape
+std::atomic<int> g_autotune{0};
+
+bool tune_find(const TuneKey& k, TuneVal* v) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  *v = it->second;
+  return true;
+}
+
+void tune_store(const TuneKey& k, const TuneVal& v) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[k] = v;
+}
+
+// Whether this call of an untuned shape should tune it: mode 1 tunes at the
+// first call, mode 2 at the second (so that one-off shapes pay nothing).
+bool want_tune(const mexct_ctx* c, const TuneKey& k) {
+  const int mode = c->autotune >= 0 ? c->autotune : g_autotune.load();
+  if (mode <= 0) return false;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return ++g_seen[k] >= mode;
+}
+
+// Times run(i) for each candidate i on the context's stream (best of two)
+// and returns the fastest index.
+template <typename F>
+int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(MEXCT_EHIP, "hipEventCreate failed");
+  }
+  int best = 0;
+  float bt = 1e30f;
+  int rc = MEXCT_OK;
+  for (int i = 0; i < ncand && rc == MEXCT_OK; ++i) {
+    for (int rep = 0; rep < 2 && rc == MEXCT_OK; ++rep) {
+      if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(MEXCT_EHIP, "event record failed"); break; }
+      rc = run(i);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(MEXCT_EHIP, "event timing failed");
+        break;
+      }
+      if (ms < bt) {
+        bt = ms;
+        best = i;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *best_ms = bt;
+  return rc ? rc : best;  // error codes are negative
 }
 
 // ---- FDK filter design (preFilterSinogram.m:95-152), host FP64
@@ -428,12 +521,16 @@ int64_t mexct_ctx_workspace_bytes(const mexct_ctx* ctx) {
 }
 
 // ------------------------------------------------------------------ FP
-int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
-                 float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
-                 const mexct_opts* opts) {
+namespace {
+// FP of `items` volumes of one geometry stored one after another (vol_dtype
+// elements), outputs one after another: one launch for all of them.
+int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t W, int64_t H,
+           float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+           const mexct_opts* opts) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims W=%lld H=%lld nProj=%d", (long long)W, (long long)H, nProj);
+  if (items < 1 || items > kMaxItems) return fail(MEXCT_EINVAL, "batch of %d volumes (1..%d)", items, kMaxItems);
   if (nProj > 0 && (!vol || !angles || !out)) return fail(MEXCT_EINVAL, "null array argument");
   RC(check_dtype(vol_dtype, "vol"));
   const int st = storage_of(opts);
@@ -446,14 +543,18 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   size_t lbytes = 0;
   const int layout = fp_layout(W, H, st, &lbytes);
   const float2* sc;
+  const size_t vel = (size_t)W * W * H;  // elements of one input volume
   if (layout >= 0) {
-    // z-fastest haloed volume, one wave per 64 detector rows of a column
-    RC(c->get(WS_VOL, lbytes, &pad));
+    // z-fastest haloed volumes, one wave per 64 detector rows of a column
+    RC(c->get(WS_VOL, lbytes * items, &pad));
     uint32_t* vmax = nullptr;
     if (st == MEXCT_F16 && vol_dtype == MEXCT_F32)
-      RC(max_bits(c, (const float*)vol, (size_t)W * W * H, 1, &vmax));
-    HIPCHK(build_fp_volume(layout, vol, vol_dtype, W, W * W, pad, st, W, H, ADDR_CLAMP, c->stream,
-                           vmax));
+      RC(max_bits(c, (const float*)vol, vel, kSlotVol, &vmax, items, vel));
+    const size_t in_es = (size_t)store_bytes(vol_dtype);
+    for (int i = 0; i < items; ++i)
+      HIPCHK(build_fp_volume(layout, (const char*)vol + i * vel * in_es, vol_dtype, W, W * W,
+                             (char*)pad + i * lbytes, st, W, H, ADDR_CLAMP, c->stream,
+                             vmax ? vmax + i : nullptr));
     RC(c->sincos_table(angles, nProj, &sc));
     FPZLaunch z;
     z.vol = pad;
@@ -468,9 +569,30 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
     z.nviews = nProj;
     z.sad = sad;
     z.long_ray = long_ray_of(opts);
+    z.items = items;
+    z.vol_stride = lbytes;
+    z.out_stride = (uint64_t)W * H * nProj;
+    const TuneKey key{MEXCT_OP_FP, geom, st, items, W, H, nProj};
+    TuneVal tv;
+    if (tune_find(key, &tv)) {
+      z.vchunk_log2 = tv.a;
+    } else if (want_tune(c, key)) {
+      static const int cand[] = {2, 3, 4, 5};  // 4 .. 32 views per chunk
+      const int best = tune_pick(c, 4, [&](int i) {
+        FPZLaunch t = z;
+        t.vchunk_log2 = cand[i];
+        HIPCHK(launch_fpz(t, c->stream));
+        return MEXCT_OK;
+      }, &tv.ms);
+      if (best < 0) return best;
+      tv.a = cand[best];
+      tune_store(key, tv);
+      return MEXCT_OK;  // the candidates' launches wrote the result
+    }
     HIPCHK(launch_fpz(z, c->stream));
     return MEXCT_OK;
   }
+  if (items != 1) return fail(MEXCT_ENOTSUP, "volume too large to batch");
   RC(c->get(WS_VOL, pbytes, &pad));
   HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
   RC(c->sincos_table(angles, nProj, &sc));
@@ -493,6 +615,19 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
   a.pts_view_stride = 0;
   HIPCHK(launch_fp(a, c->stream));
   return MEXCT_OK;
+}
+}  // namespace
+
+int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
+                 float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+                 const mexct_opts* opts) {
+  return fp_run(c, vol, vol_dtype, 1, W, H, sad, angles, nProj, geom, out, opts);
+}
+
+int mexct_fp_batch_dev(mexct_ctx* c, const void* vols, int32_t vol_dtype, int32_t count,
+                       int64_t W, int64_t H, float sad, const float* angles, int32_t nProj,
+                       int32_t geom, float* outs, const mexct_opts* opts) {
+  return fp_run(c, vols, vol_dtype, count, W, H, sad, angles, nProj, geom, outs, opts);
 }
 
 int mexct_fp(const float* vol, int64_t W, int64_t H, float sad, const float* angles,
@@ -517,12 +652,17 @@ int mexct_fp(const float* vol, int64_t W, int64_t H, float sad, const float* ang
 }
 
 // ------------------------------------------------------------------ BP
-int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
-                 int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
-                 const mexct_opts* opts) {
+namespace {
+// BP of `items` sinograms of one geometry stored one after another
+// (sino_dtype elements), volumes one after another: one launch per chunk of
+// views for all of them.
+int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_t W, int64_t H,
+           int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+           const mexct_opts* opts) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
+  if (items < 1 || items > kMaxItems) return fail(MEXCT_EINVAL, "batch of %d sinograms (1..%d)", items, kMaxItems);
   if (!out || (nProj > 0 && (!sino || !angles))) return fail(MEXCT_EINVAL, "null array argument");
   RC(check_dtype(sino_dtype, "sino"));
   const int st = storage_of(opts);
@@ -541,13 +681,18 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
   DevGuard g(c->device);
   void* pad;
-  RC(c->get(WS_SINO, rp_bytes * std::max<int64_t>(nProj, 1), &pad));
+  const size_t sino_stride = rp_bytes * (size_t)std::max<int64_t>(nProj, 1);
+  RC(c->get(WS_SINO, sino_stride * items, &pad));
   uint32_t* smax = nullptr;
+  const size_t sel = (size_t)W * H * nProj;  // elements of one input sinogram
   if (st == MEXCT_F16 && sino_dtype == MEXCT_F32 && nProj > 0)
-    RC(max_bits(c, (const float*)sino, (size_t)W * H * nProj, 2, &smax));
+    RC(max_bits(c, (const float*)sino, sel, kSlotSino, &smax, items, sel));
+  const size_t in_es = (size_t)store_bytes(sino_dtype);
   if (nProj > 0)
-    HIPCHK(launch_pad2(sino, sino_dtype, pad, st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
-                       smax, rows));
+    for (int i = 0; i < items; ++i)
+      HIPCHK(launch_pad2((const char*)sino + i * sel * in_es, sino_dtype, (char*)pad + i * sino_stride,
+                         st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
+                         smax ? smax + i : nullptr, rows));
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   BPLaunch a;
@@ -565,7 +710,36 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
   a.rows = rows;
   a.chain = nullptr;
   a.chain_views = 0;
-  const size_t cfv = geom == 2 && st == MEXCT_F32 ? bp_chain_floats_per_view((int)W, (int)H, a.zr) : 0;
+  a.items = items;
+  a.sino_stride = sino_stride;
+  a.out_stride = (uint64_t)W * W * H;
+  const TuneKey key{MEXCT_OP_BP, geom, st, items, W, H, nProj};
+  TuneVal tv;
+  const bool found = tune_find(key, &tv);
+  const bool tune = !found && nProj > 0 && want_tune(c, key);
+  // candidates: super-tile side 2, 4, 8; the z-range too for FP32 storage
+  // unless the caller fixed it
+  const bool zr_free = st == MEXCT_F32 && !(opts && (opts->zrange == 32 || opts->zrange == 64));
+  std::vector<std::pair<int, int>> cand;
+  for (int sup : {2, 4, 8}) {
+    if (zr_free)
+      for (int zr : {32, 64}) cand.push_back({sup, zr});
+    else
+      cand.push_back({sup, a.zr});
+  }
+  if (found) {
+    a.super_ = tv.a;
+    a.zr = tv.b;
+  }
+  // chain scratch for the largest candidate (super-tile 8, z-range 32)
+  const bool chain = geom == 2 && st == MEXCT_F32;
+  size_t cfv = 0;
+  if (chain) {
+    if (tune)
+      for (auto& cd : cand) cfv = std::max(cfv, bp_chain_floats_per_view((int)W, (int)H, cd.second, cd.first));
+    else
+      cfv = bp_chain_floats_per_view((int)W, (int)H, a.zr, a.super_);
+  }
   if (cfv > 0 && nProj > 0) {
     // at most 1 GiB of chain scratch; more views run as further launches
     const int64_t cv = std::min<int64_t>(nProj, std::max<int64_t>(1, ((int64_t)1 << 28) / (int64_t)cfv));
@@ -574,7 +748,60 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
     a.chain = (float*)ch;
     a.chain_views = (int)cv;
   }
+  if (tune) {
+    const int best = tune_pick(c, (int)cand.size(), [&](int i) {
+      BPLaunch t = a;
+      t.super_ = cand[i].first;
+      t.zr = cand[i].second;
+      HIPCHK(launch_bp(t, c->stream));
+      return MEXCT_OK;
+    }, &tv.ms);
+    if (best < 0) return best;
+    tv.a = cand[best].first;
+    tv.b = cand[best].second;
+    tune_store(key, tv);
+    return MEXCT_OK;  // the candidates' launches wrote the result
+  }
   HIPCHK(launch_bp(a, c->stream));
+  return MEXCT_OK;
+}
+}  // namespace
+
+int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
+                 int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+                 const mexct_opts* opts) {
+  return bp_run(c, sino, sino_dtype, 1, W, H, nProj, sad, angles, geom, out, opts);
+}
+
+int mexct_bp_batch_dev(mexct_ctx* c, const void* sinos, int32_t sino_dtype, int32_t count,
+                       int64_t W, int64_t H, int64_t nProj, float sad, const float* angles,
+                       int32_t geom, float* outs, const mexct_opts* opts) {
+  return bp_run(c, sinos, sino_dtype, count, W, H, nProj, sad, angles, geom, outs, opts);
+}
+
+// ------------------------------------------------------------------ autotuner
+int mexct_set_autotune(int32_t mode) {
+  if (mode < 0 || mode > 2) return fail(MEXCT_EINVAL, "autotune mode must be 0, 1 or 2");
+  g_autotune.store(mode);
+  return MEXCT_OK;
+}
+
+int mexct_ctx_set_autotune(mexct_ctx* c, int32_t mode) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  if (mode < -1 || mode > 2) return fail(MEXCT_EINVAL, "autotune mode must be -1, 0, 1 or 2");
+  c->autotune = mode;
+  return MEXCT_OK;
+}
+
+int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int32_t count, int64_t W,
+                       int64_t H, int64_t nProj, int32_t* cfg) {
+  if (!cfg) return fail(MEXCT_EINVAL, "cfg is null");
+  TuneVal v;
+  const bool f = tune_find(TuneKey{op, geom, storage, count, W, H, nProj}, &v);
+  cfg[0] = f ? 1 : 0;
+  cfg[1] = f ? v.a : -1;
+  cfg[2] = f ? v.b : -1;
+  cfg[3] = f ? (int32_t)(v.ms * 1000.f) : 0;
   return MEXCT_OK;
 }
 

@@ -57,7 +57,7 @@ __device__ __forceinline__ float4 bld4<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_
                      __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
 }
 
-constexpr uint32_t kSuper = 4;
+constexpr uint32_t kSuper = 4;  // default super-tile side (tiles)
 
 struct BPKArgs {
   const void* sino;     // row-paired haloed sinogram
@@ -72,7 +72,21 @@ struct BPKArgs {
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
   uint32_t ncols;  // chain columns per (view, range): padded tiles x 64
+  uint32_t super_;       // super-tile side in tiles (block order)
+  uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
+  uint64_t sino_stride;  // bytes between the items' haloed sinograms
+  uint64_t out_stride;   // floats between the items' volumes
 };
+
+// Voxel column (ix, iy) of lane `lane` of tile `tile`: 8x8 tiles ordered in
+// super x super super-tiles, row-major inside each.
+__device__ __forceinline__ void tile_xy(uint32_t tile, uint32_t lane, uint32_t tiles_x,
+                                        uint32_t sup, int& ix, int& iy) {
+  const uint32_t st = tile / (sup * sup), in = tile % (sup * sup);
+  const uint32_t sbx = (tiles_x + sup - 1) / sup;
+  ix = (int)((st % sbx) * sup + in % sup) * 8 + (int)(lane & 7);
+  iy = (int)((st / sbx) * sup + in / sup) * 8 + (int)(lane >> 3);
+}
 
 // per-view detector geometry of one voxel column
 template <int GEOM>
@@ -119,11 +133,8 @@ __global__ __launch_bounds__(256) void bp_chain_kernel(BPKArgs a, float* chain) 
   for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (size_t)gridDim.x * 256) {
     const uint32_t col = (uint32_t)(t % a.ncols);
     const int v = (int)(t / a.ncols);
-    const uint32_t tile = col >> 6, lane = col & 63u;
-    const uint32_t st = tile / (kSuper * kSuper), in = tile % (kSuper * kSuper);
-    const uint32_t sbx = (a.tiles_x + kSuper - 1) / kSuper;
-    const int ix = (int)((st % sbx) * kSuper + in % kSuper) * 8 + (int)(lane & 7);
-    const int iy = (int)((st / sbx) * kSuper + in / kSuper) * 8 + (int)(lane >> 3);
+    int ix, iy;
+    tile_xy(col >> 6, col & 63u, a.tiles_x, a.super_, ix, iy);
     if (ix >= a.W || iy >= a.W) continue;
     const uint32_t hw = (uint32_t)a.W / 2u - 1u;
     const float x = (float)ix - ((float)hw + 0.5f);
@@ -143,24 +154,22 @@ __global__ __launch_bounds__(256) void bp_chain_kernel(BPKArgs a, float* chain) 
 // exact in float and cost one FMA.
 template <int GEOM, int ZR, typename S, bool SMALL, int RQ>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
-  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-  if (b >= a.n_logical) return;
+  const uint32_t b0 = xcd_remap(blockIdx.x, gridDim.x);
+  if (b0 >= a.n_logical) return;
+  const uint32_t item = b0 / a.item_blocks;  // batch item
+  const uint32_t b = b0 - item * a.item_blocks;
   const uint32_t zr = b % a.nz;
-  // tiles in 4x4 super-tiles (row-major inside), so that the blocks an XCD
-  // runs together cover a compact region whose view footprints overlap
+  // tiles in super-tiles (row-major inside), so that the blocks an XCD runs
+  // together cover a compact region whose view footprints overlap
   const uint32_t tile = b / a.nz;
-  const uint32_t st = tile / (kSuper * kSuper), in = tile % (kSuper * kSuper);
-  const uint32_t sbx = (a.tiles_x + kSuper - 1) / kSuper;
-  const uint32_t tx = (st % sbx) * kSuper + in % kSuper;
-  const uint32_t ty = (st / sbx) * kSuper + in / kSuper;
   const uint32_t chain_col = tile * 64u + threadIdx.x;
-  const int ix = (int)tx * 8 + (int)(threadIdx.x & 7);
-  const int iy = (int)ty * 8 + (int)(threadIdx.x >> 3);
+  int ix, iy;
+  tile_xy(tile, threadIdx.x, a.tiles_x, a.super_, ix, iy);
   if (ix >= a.W || iy >= a.W) return;
   const int z0 = (int)zr * ZR;
   const int zc = min(ZR, a.H - z0);
   const size_t WW = (size_t)a.W * a.W;
-  float* o = a.out + (size_t)ix + (size_t)a.W * iy + WW * z0;
+  float* o = a.out + item * a.out_stride + (size_t)ix + (size_t)a.W * iy + WW * z0;
   const uint32_t hw = (uint32_t)a.W / 2u - 1u;
   const uint32_t dx = (uint32_t)ix - hw, dy = (uint32_t)iy - hw;
   if (!(dx * dx + dy * dy < hw * hw)) {  // outside the circle of reconstruction
@@ -169,14 +178,14 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   }
   const float x = (float)ix - ((float)hw + 0.5f);
   const float y = (float)iy - ((float)hw + 0.5f);
-  const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc((void*)a.sino, 0, (int)a.sino_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)((const char*)a.sino + item * a.sino_stride), 0, (int)a.sino_bytes, 0x00020000);
   const uint32_t es = (RQ ? 4u : 2u) * (uint32_t)sizeof(S);  // bytes of one row-group element
   const int row_bytes = a.PW * (int)es;          // one detector row of pairs
   float acc[ZR];
 #pragma unroll
   for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
-  const int sexp = pow2_exp(a.sino_max);
+  const int sexp = pow2_exp(a.sino_max ? a.sino_max + item : nullptr);
   if (a.accumulate) {
     const float* q = o;
 #pragma unroll
@@ -364,9 +373,10 @@ static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, int rows, hipStre
     bp_dispatch<32, S>(k, geom, rows, st);
 }
 
-size_t bp_chain_floats_per_view(int W, int H, int zr) {
+size_t bp_chain_floats_per_view(int W, int H, int zr, int super_) {
+  const uint32_t sup = super_ > 0 ? (uint32_t)super_ : kSuper;
   const uint32_t tiles_x = (uint32_t)((W + 7) / 8);
-  const uint32_t tpad = (tiles_x + kSuper - 1) / kSuper * kSuper;
+  const uint32_t tpad = (tiles_x + sup - 1) / sup * sup;
   const uint32_t nz = (uint32_t)((H + zr - 1) / zr);
   return nz > 1 ? (size_t)tpad * tpad * 64u * (nz - 1) : 0;
 }
@@ -389,9 +399,15 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.SR = a.sad;
   k.tiles_x = (uint32_t)((a.W + 7) / 8);
   k.nz = (uint32_t)((a.H + zr - 1) / zr);
-  const uint32_t tpad = (k.tiles_x + kSuper - 1) / kSuper * kSuper;
-  k.n_logical = tpad * tpad * k.nz;
+  k.super_ = a.super_ > 0 ? (uint32_t)a.super_ : kSuper;
+  const uint32_t tpad = (k.tiles_x + k.super_ - 1) / k.super_ * k.super_;
+  const int items = a.items > 0 ? a.items : 1;
+  if ((uint64_t)tpad * tpad * k.nz * (uint64_t)items >= (1ull << 31)) return hipErrorInvalidValue;
+  k.item_blocks = tpad * tpad * k.nz;
+  k.n_logical = k.item_blocks * (uint32_t)items;
   k.ncols = tpad * tpad * 64u;
+  k.sino_stride = a.sino_stride;
+  k.out_stride = a.out_stride;
   if (k.n_logical == 0) return hipSuccess;
   const size_t view_bytes = (size_t)k.PV * es2;
   int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);

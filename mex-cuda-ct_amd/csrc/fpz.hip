@@ -34,7 +34,7 @@
 
 namespace mexct {
 
-constexpr uint32_t kViewChunk = 16;
+constexpr uint32_t kViewChunkLog2 = 4;  // default: 16 views per chunk
 // default ray length (samples) above which 2-byte ray sums are compensated
 constexpr int kLongRay = 1024;
 
@@ -59,6 +59,10 @@ struct FPZArgs {
   uint32_t n_ug, n_vg, n_logical;
   uint32_t nv;  // views of the launch
   int long_ray;  // 2-byte storage: waves with a ray of more samples use compensated sums
+  uint32_t vchunk_log2;  // views per chunk of the block order: 1 << vchunk_log2
+  uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
+  uint64_t vol_stride;   // bytes between the items' volumes
+  uint64_t out_stride;   // floats between the items' outputs
 };
 
 template <typename S>
@@ -326,32 +330,35 @@ struct OscPad<_Float16> {
 // written in place, plus zero-haloed copies for the dual backprojection.
 template <int GEOM, typename S, int EPI, int LAYOUT>
 __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
-  // Block order: (chunk of kViewChunk views, column group, view in chunk,
-  // row group), each XCD taking a contiguous range.  The blocks an XCD runs
-  // together then march the same 4 detector columns of consecutive views,
-  // whose rays lie within a few voxels of each other, so the volume lines
-  // they gather are shared in that XCD's L2 (view-major order: 47.8 ms ->
-  // 32.4 ms at 384^3 x 512 cone, f32).
-  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-  if (b >= a.n_logical) return;
+  // Block order: (batch item, chunk of 2^vchunk_log2 views, column group,
+  // view in chunk, row group), each XCD taking a contiguous range.  The
+  // blocks an XCD runs together then march the same 4 detector columns of
+  // consecutive views, whose rays lie within a few voxels of each other, so
+  // the volume lines they gather are shared in that XCD's L2 (view-major
+  // order: 47.8 ms -> 32.4 ms at 384^3 x 512 cone, f32).
+  const uint32_t b0 = xcd_remap(blockIdx.x, gridDim.x);
+  if (b0 >= a.n_logical) return;
+  const uint32_t item = b0 / a.item_blocks;
+  const uint32_t b = b0 - item * a.item_blocks;
   const uint32_t vg = b % a.n_vg;
   uint32_t rest = b / a.n_vg;
-  const uint32_t vw = rest % kViewChunk;
-  rest /= kViewChunk;
+  const uint32_t vw = rest & ((1u << a.vchunk_log2) - 1u);
+  rest >>= a.vchunk_log2;
   const uint32_t ug = rest % a.n_ug;
-  const uint32_t jv = (rest / a.n_ug) * kViewChunk + vw;
+  const uint32_t jv = ((rest / a.n_ug) << a.vchunk_log2) + vw;
   if (jv >= a.nv) return;
   const int ix = (int)ug * 4 + (int)(threadIdx.x >> 6);
   const int iy = (int)vg * 64 + (int)(threadIdx.x & 63);
   if (ix >= a.W || iy >= a.H) return;
   const int view = a.first + (int)jv * a.stride;
   const size_t p = (size_t)view * a.W * a.H + (size_t)iy * a.W + ix;
+  float* const out = a.out + item * a.out_stride;
   float preval = 0.f, postval = 0.f;
   size_t q = 0;
   typedef typename OscPad<S>::T P;
   if (EPI == 0) {
     if (iy >= a.rows) {  // the reference's grid never writes these rows
-      a.out[p] = 0.f;
+      out[p] = 0.f;
       return;
     }
   } else {
@@ -377,11 +384,12 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   constexpr int STEP2 = (EPI == 1 || GEOM != GEOM_CONE) ? 1 : 0;
   R3 cur, inc;
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
-  const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = ldexpf(march_z<S, STEP2, LAYOUT>(r, a, cur, inc, n), pow2_exp(a.vol_max));
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)((const char*)a.vol + item * a.vol_stride), 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
+  const float dval = ldexpf(march_z<S, STEP2, LAYOUT>(r, a, cur, inc, n),
+                            pow2_exp(a.vol_max ? a.vol_max + item : nullptr));
   if (EPI == 0) {
-    a.out[p] = dval;
+    out[p] = dval;
   } else {
     const float np = (preval * expf(-dval) - postval);
     const float npo = preval * expf(-dval) * dval;
@@ -390,6 +398,15 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     ((P*)a.pre_pad)[q] = Store<P>::from(np);
     ((P*)a.post_pad)[q] = Store<P>::from(npo);
   }
+}
+
+// The block order's view chunk and the number of batch items.
+static void set_view_chunk(FPZArgs& k, uint32_t log2, uint32_t items) {
+  k.vchunk_log2 = log2;
+  const uint32_t c = 1u << log2;
+  const uint32_t nvp = (k.nv + c - 1) / c * c;
+  k.item_blocks = k.n_ug * k.n_vg * nvp;
+  k.n_logical = k.item_blocks * items;
 }
 
 // W x H detector, NX x NX x NZ volume (the same sizes in the standard
@@ -432,8 +449,7 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   k.n_vg = (uint32_t)((H + 63) / 64);
   k.nv = (uint32_t)nviews;
   k.long_ray = kLongRay;
-  const uint32_t nvp = (k.nv + kViewChunk - 1) / kViewChunk * kViewChunk;
-  k.n_logical = k.n_ug * k.n_vg * nvp;
+  set_view_chunk(k, kViewChunkLog2, 1);
   return k;
 }
 
@@ -470,6 +486,11 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   k.pts_view_stride = a.pts_view_stride;
   if (a.long_ray != 0) k.long_ray = a.long_ray;
   if (a.geom == GEOM_G3D) k.rows = a.H;  // every detector row is a ray
+  const int items = a.items > 0 ? a.items : 1;
+  set_view_chunk(k, a.vchunk_log2 >= 0 ? (uint32_t)a.vchunk_log2 : kViewChunkLog2, (uint32_t)items);
+  k.vol_stride = a.vol_stride;
+  k.out_stride = a.out_stride;
+  if ((uint64_t)k.item_blocks * (uint64_t)items >= (1ull << 31)) return hipErrorInvalidValue;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)
     fpz_dispatch<0, float>(k, a.geom, a.layout, st);

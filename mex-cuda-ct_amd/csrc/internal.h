@@ -77,6 +77,11 @@ struct FPZLaunch {
   const float* pts = nullptr;  // GEOM_G3D: ray endpoints [W,H,6] (+ view * pts_view_stride)
   int64_t pts_view_stride = 0;
   int long_ray = 0;  // 2-byte storage: compensated sums above this many samples (0: 1024)
+  // batch: `items` volumes vol_stride bytes apart (haloed), outputs out_stride
+  // floats apart, vol_max[item] when scaled; one launch for all of them
+  int items = 1;
+  uint64_t vol_stride = 0, out_stride = 0;
+  int vchunk_log2 = -1;  // block order: views per chunk 2^vchunk_log2 (-1: default 16)
 };
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 
@@ -110,9 +115,14 @@ struct BPLaunch {
   int rows;          // rows per sinogram element: 2, or 4 (2-byte storage, rows step <= 2)
   float* chain;      // cone f32: scratch of chain_views x (nz-1) x padded columns floats, or null
   int chain_views;   // views per launch the chain scratch holds
+  // batch: `items` sinograms sino_stride bytes apart (haloed), volumes
+  // out_stride floats apart, sino_max[item] when scaled; one launch per chunk
+  int items = 1;
+  uint64_t sino_stride = 0, out_stride = 0;
+  int super_ = 0;    // block order: super-tile side in 8x8 tiles (0: default 4)
 };
-// floats of BP chain scratch per view (cone f32, z-range zr): 0 if unused
-size_t bp_chain_floats_per_view(int W, int H, int zr);
+// floats of BP chain scratch per view (cone f32, z-range zr, super-tile side)
+size_t bp_chain_floats_per_view(int W, int H, int zr, int super_);
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
 
 struct OSCBPLaunch {

@@ -9,11 +9,16 @@
 // arrive while others run start as soon as a worker and memory are free,
 // and their kernels overlap on the device.  A worker that admits a request
 // also takes up to max_group-1 queued requests with the same key (operator,
-// geometry, sizes, angles, SAD, window, storage) and runs them back to back
-// with the geometry tables and buffers already resident.
+// geometry, sizes, angles, SAD, window, storage), as many as the budget
+// holds.  FP, BP and FDK groups run as one batch: every kernel of the
+// projector or backprojector is one launch for the whole group
+// (mexct_fp_batch_dev / mexct_bp_batch_dev), each item computed exactly as
+// alone.  OSC-TV groups run back to back with the tables resident.  The
+// workers' contexts autotune the block order of a problem shape at its
+// second call.
 //
 // Memory accounting: every worker reserves max(bytes it holds, footprint
-// of the request it runs).  The sum of reservations never exceeds the
+// of the group it runs).  The sum of reservations never exceeds the
 // budget.  When the oldest request cannot be admitted and releasing the
 // memory held by idle workers would let it fit, the idle workers release
 // it.  Later requests may overtake a blocked oldest request at most
@@ -66,6 +71,7 @@ struct Req {
 // worker's staging arrays plus the C ABI's workspace for that operator
 // (haloed copies at 4 bytes per element, the widest storage).
 int64_t footprint_of(const mexct_request& r) {
+  // (per request; a batched group of G needs G times this)
   const size_t W = (size_t)r.W, H = (size_t)r.H, P = (size_t)r.nProj;
   const size_t v = W * W * H * 4, s = W * H * P * 4;
   const size_t vh = (W + 4) * (W + 4) * (H + 4) * 4;
@@ -102,6 +108,7 @@ struct mexct_server {
     int64_t reserved = 0;  // this worker's share of in_use
     bool running = false;
   };
+  int64_t n_batched = 0, n_batches = 0;  // requests run in multi-item launches, such groups
   int device = 0;
   int64_t budget = 0;
   int64_t in_use = 0;
@@ -141,6 +148,11 @@ int64_t held_bytes(const Worker& w) {
 
 // Frees a worker's staging arrays and context workspace.  The worker's own
 // thread calls this while the worker is idle; its stream is drained first.
+int new_ctx(mexct_server* sv, Worker& w) {
+  const int rc = mexct_ctx_create(&w.ctx, sv->device, nullptr);
+  return rc ? rc : mexct_ctx_set_autotune(w.ctx, 2);  // tune shapes that recur
+}
+
 int release(mexct_server* sv, Worker& w) {
   (void)mexct_ctx_sync(w.ctx);
   for (Buf* b : {&w.a, &w.b, &w.o, &w.x}) {
@@ -150,7 +162,7 @@ int release(mexct_server* sv, Worker& w) {
   }
   mexct_ctx_destroy(w.ctx);
   w.ctx = nullptr;
-  return mexct_ctx_create(&w.ctx, sv->device, nullptr);
+  return new_ctx(sv, w);
 }
 
 int copy_in(Worker& w, void* dst, const void* src, size_t bytes) {
@@ -217,6 +229,42 @@ int run_one(Worker& w, Req& q) {
   }
 }
 
+bool batchable(const mexct_request& r) {
+  return r.op == MEXCT_OP_FP || r.op == MEXCT_OP_BP || r.op == MEXCT_OP_FDK;
+}
+
+// Runs a same-key group of FP/BP/FDK requests as one batch: inputs staged one
+// after another, one batched projector or backprojector, outputs copied out.
+int run_batch(Worker& w, const std::vector<Req*>& g) {
+  const mexct_request& r = g[0]->r;
+  const mexct_opts o{r.storage, r.tv_steps, 0, 0};
+  const int n = (int)g.size();
+  const size_t vb = (size_t)r.W * r.W * r.H * 4, sb = (size_t)r.W * r.H * r.nProj * 4;
+  const float* ang = g[0]->angles.data();
+  const size_t in_b = r.op == MEXCT_OP_FP ? vb : sb, out_b = r.op == MEXCT_OP_FP ? sb : vb;
+  TRY(grow(w.a, in_b * n));
+  TRY(grow(w.o, out_b * n));
+  for (int i = 0; i < n; ++i) TRY(copy_in(w, (char*)w.a.p + i * in_b, g[i]->r.input, in_b));
+  if (r.op == MEXCT_OP_FP) {
+    TRY(mexct_fp_batch_dev(w.ctx, w.a.p, MEXCT_F32, n, r.W, r.H, r.sad, ang, (int32_t)r.nProj,
+                           r.geom, (float*)w.o.p, &o));
+  } else {
+    const void* bp_in = w.a.p;
+    if (r.op == MEXCT_OP_FDK) {
+      TRY(grow(w.x, sb * n));
+      for (int i = 0; i < n; ++i)
+        TRY(mexct_prefilter_dev(w.ctx, (const float*)((const char*)w.a.p + i * sb), r.W, r.H,
+                                r.nProj, r.geom, r.sad, ang, r.window, r.cutoff, MEXCT_NO_PARKER,
+                                (float*)((char*)w.x.p + i * sb)));
+      bp_in = w.x.p;
+    }
+    TRY(mexct_bp_batch_dev(w.ctx, bp_in, MEXCT_F32, n, r.W, r.H, r.nProj, r.sad, ang, r.geom,
+                           (float*)w.o.p, &o));
+  }
+  for (int i = 0; i < n; ++i) TRY(copy_out(w, g[i]->r.output, (char*)w.o.p + i * out_b, out_b));
+  return MEXCT_OK;
+}
+
 enum class Pick { kWait, kRun, kRelease };
 
 // Chooses an idle worker's next action (server lock held).
@@ -241,18 +289,23 @@ Pick choose(mexct_server* sv, Worker& w, std::vector<Req*>& group) {
     }
     if (q != head) ++head->overtaken;  // head was blocked to get here
     group.push_back(q);
-    w.reserved = need;
-    sv->in_use = others + need;
-    sv->peak_in_use = std::max(sv->peak_in_use, sv->in_use);
     it = sv->queue.erase(it);
+    // a batch holds every item's buffers at once; back-to-back runs reuse one set
+    const bool batch = batchable(q->r);
+    int64_t group_fp = q->footprint;
     for (auto jt = it; jt != sv->queue.end() && (int)group.size() < sv->max_group;) {
-      if (same_key(**jt, *q)) {
+      const int64_t more = batch ? group_fp + (*jt)->footprint : group_fp;
+      if (same_key(**jt, *q) && others + std::max(w.reserved, more) <= sv->budget) {
         group.push_back(*jt);
+        group_fp = more;
         jt = sv->queue.erase(jt);
       } else {
         ++jt;
       }
     }
+    w.reserved = std::max(w.reserved, group_fp);
+    sv->in_use = others + w.reserved;
+    sv->peak_in_use = std::max(sv->peak_in_use, sv->in_use);
     return Pick::kRun;
   }
   return Pick::kWait;
@@ -293,7 +346,24 @@ void worker_loop(mexct_server* sv, Worker* w) {
       sv->cv_done.notify_all();
       continue;
     }
+    bool batched = false;
+    if (group.size() > 1 && batchable(group[0]->r) && run_batch(*w, group) == MEXCT_OK) {
+      batched = true;
+      std::lock_guard<std::mutex> lk(sv->mu);
+      const double t = now_s();
+      for (Req* q : group) {
+        q->status = 0;
+        q->t_done = t;
+        ++sv->n_done;
+      }
+      sv->n_batched += (int64_t)group.size();
+      ++sv->n_batches;
+      sv->cv_done.notify_all();
+    }
+    // one by one: OSC-TV groups, single requests, and a batch that failed
+    // (so that a failing request does not take the others with it)
     for (Req* q : group) {
+      if (batched) break;
       const int rc = run_one(*w, *q);
       const std::string msg = rc ? std::string(mexct_last_error()) : std::string();
       std::lock_guard<std::mutex> lk(sv->mu);
@@ -330,8 +400,9 @@ int mexct_server_create(mexct_server** out, int device, int64_t hbm_budget, int3
   sv->max_group = max_group > 0 ? max_group : 1;
   for (int i = 0; i < slots; ++i) {
     std::unique_ptr<Worker> w(new Worker());
-    const int rc = mexct_ctx_create(&w->ctx, device, nullptr);
+    const int rc = new_ctx(sv.get(), *w);
     if (rc) {
+      if (w->ctx) mexct_ctx_destroy(w->ctx);
       for (auto& p : sv->workers) mexct_ctx_destroy(p->ctx);
       return rc;
     }
@@ -389,9 +460,10 @@ int mexct_server_wait(mexct_server* sv, uint64_t ticket, int32_t* status, double
 int mexct_server_stats(mexct_server* sv, int64_t* stats, int32_t n) {
   if (!sv || !stats || n < 1) return mexct::set_error(MEXCT_EINVAL, "server: null argument");
   std::lock_guard<std::mutex> lk(sv->mu);
-  const int64_t v[7] = {sv->n_done, sv->n_groups,      sv->n_grouped, (int64_t)sv->queue.size(),
-                        sv->in_use, sv->peak_in_use, sv->n_releases};
-  for (int i = 0; i < n && i < 7; ++i) stats[i] = v[i];
+  const int64_t v[9] = {sv->n_done,   sv->n_groups,      sv->n_grouped,
+                        (int64_t)sv->queue.size(), sv->in_use, sv->peak_in_use,
+                        sv->n_releases, sv->n_batched, sv->n_batches};
+  for (int i = 0; i < n && i < 9; ++i) stats[i] = v[i];
   return MEXCT_OK;
 }
 

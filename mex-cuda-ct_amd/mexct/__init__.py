@@ -26,4 +26,6 @@ from .ops import (  # noqa: F401
     first_picks,
     parker_weight,
     preFilterSinogram,
+    set_autotune,
+    tuned_config,
 )

@@ -129,6 +129,33 @@ class Context:
                    opts(storage, 0, zrange))
         return out
 
+    def fp_batch(self, vols, sad, angles, nproj, geom, outs, storage=F32):
+        """vols (G, H, W, W) -> outs (G, nProj, H, W) f32, one launch."""
+        G, H, W, W2 = vols.shape
+        ang = _host_f32(angles)
+        if W2 != W or ang.size < nproj or tuple(outs.shape) != (G, nproj, H, W):
+            raise MexctError(EINVAL, "fp_batch: shape mismatch")
+        self._call(lib().mexct_fp_batch_dev, self._h, _dptr(vols, "vols"), _dtype_code(vols), G,
+                   W, H, float(sad), _p(ang), int(nproj), int(geom), _dptr(outs, "outs", True),
+                   opts(storage))
+        return outs
+
+    def bp_batch(self, sinos, sad, angles, geom, outs, storage=F32, zrange=0):
+        """sinos (G, nProj, H, W) -> outs (G, H, W, W) f32, one launch per view chunk."""
+        G, n, H, W = sinos.shape
+        ang = _host_f32(angles)
+        if ang.size < n or tuple(outs.shape) != (G, H, W, W):
+            raise MexctError(EINVAL, "bp_batch: shape mismatch")
+        self._call(lib().mexct_bp_batch_dev, self._h, _dptr(sinos, "sinos"), _dtype_code(sinos), G,
+                   W, H, n, float(sad), _p(ang), int(geom), _dptr(outs, "outs", True),
+                   opts(storage, 0, zrange))
+        return outs
+
+    def set_autotune(self, mode):
+        """-1: follow mexct.set_autotune; 0 off; 1 tune a problem shape at its
+        first call; 2 at its second."""
+        check(lib().mexct_ctx_set_autotune(self._h, int(mode)))
+
     def g3d_fp(self, vol, angles, pts, nproj, out, storage=F32):
         """pts (6, H, W) shared or (nProj, 6, H, W) per view."""
         nsl, side, _ = vol.shape

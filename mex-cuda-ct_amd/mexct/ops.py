@@ -243,3 +243,19 @@ def FBP_recon(pre, post, scanAngles, geom, filter="hamming", filterCutoff=1.0, p
                                 float(filterCutoff), _parker_q(parker_q), _ptr(vol),
                                 opts(int(storage))))
     return vol
+
+
+def set_autotune(mode):
+    """Block-order autotuning of the FP/BP kernels for the process: 1 tunes
+    each problem shape at its first call (timing every candidate on that
+    call's data), 2 at its second, 0 off."""
+    check(lib().mexct_set_autotune(int(mode)))
+
+
+def tuned_config(op, geom, storage, count, W, H, nProj):
+    """(found, FP views-per-chunk log2 or BP super-tile side, BP z-range,
+    winner's time in us) for a problem shape; op 0 FP, 1 BP."""
+    cfg = np.zeros(4, dtype=np.int32)
+    check(lib().mexct_tuned_config(int(op), int(geom), int(storage), int(count), int(W), int(H),
+                                   int(nProj), _ptr(cfg)))
+    return tuple(int(x) for x in cfg)

@@ -20,7 +20,7 @@ from ._lib import (EINVAL, F32, OP_BP, OP_FDK, OP_FP, OP_OSCTV, MexctError, Requ
 from .ops import GEOM, WINDOWS, _dims3, _ptr, _single, _vec, first_picks
 
 STAT_NAMES = ("done", "groups", "grouped", "queued", "reserved_bytes", "peak_reserved_bytes",
-              "releases")
+              "releases", "batched", "batches")
 
 
 class Server:

@@ -125,6 +125,30 @@ def test_same_shape_requests_are_grouped(gpu):
     for v, o in zip(vols, outs):
         assert np.array_equal(o, mexct.CUDAmex_FP(v, 300.0, ang, n, 2))
     assert st["grouped"] == 8 and st["groups"] < 8
+    # a group of more than one FP request runs as one batched launch
+    assert st["batches"] >= 1 and st["batched"] == 8 - (st["groups"] - st["batches"])
+
+
+@pytest.mark.parametrize("storage", [mexct.F32, mexct.F16])
+def test_bp_and_fdk_groups_run_batched(gpu, storage):
+    W, H, n, sad = 40, 20, 16, 200.0
+    ang = _angles(n)
+    g = {"type": "cone3d", "SAD": sad}
+    sinos = [_sino(W, H, n, 600 + k) for k in range(6)]
+    alone_bp = [mexct.CUDAmex_BP(P, sad, ang, 2, storage=storage) for P in sinos]
+    alone_fdk = [mexct.CUDAmex_BP(mexct.preFilterSinogram(P, g, ang, "hann", 0.9), sad, ang, 2,
+                                  storage=storage) for P in sinos]
+    with Server(hbm_budget=1 << 30, slots=1, max_group=3) as s:
+        tb = [s.submit_bp(P, sad, ang, 2, storage=storage) for P in sinos]
+        tf = [s.submit_fdk(P, g, ang, "hann", 0.9, storage=storage) for P in sinos]
+        outs_b = [s.result(t) for t in tb]
+        outs_f = [s.result(t) for t in tf]
+        st = s.stats()
+    for k in range(6):
+        assert np.array_equal(outs_b[k], alone_bp[k]), f"bp {k}"
+        assert np.array_equal(outs_f[k], alone_fdk[k]), f"fdk {k}"
+    print(f"stats {st}")
+    assert st["batches"] >= 2 and st["batched"] >= 6
 
 
 def test_failing_request_is_isolated(gpu):
