@@ -113,14 +113,15 @@ int mexct_bp_batch_dev(mexct_ctx* ctx, const void* sinos, int32_t sino_dtype, in
  * No counterpart in the reference.  The FP and BP kernels' block order (FP:
  * views per chunk; BP: super-tile side, and the z-range for FP32 storage
  * unless opts fixes it) is tuned per problem shape (operator, geometry,
- * storage, batch count, W, H, nProj): with tuning on, the first call of a
- * shape times every candidate on its own data and keeps the fastest for
- * the process.  Candidates write identical bits.  mode 1: tune at a
- * shape's first call; 2: at its second (one-off shapes pay nothing; the
- * request server's contexts use this); 0 off (default: built-in choices).
- * A context's own setting (-1: follow the global one) overrides.  mexct_tuned_config fills cfg[4] = {found, FP
- * views-per-chunk log2 or BP super-tile side, BP z-range, winner's time in
- * microseconds}. */
+ * storage, batch count, W, H, nProj): the tuning call of a shape times
+ * every candidate once on its first 32 views, keeps the fastest for the
+ * process and runs the call with it.  Candidates write identical bits.
+ * mode 1: tune at a shape's first call; 2: at its second (one-off shapes
+ * pay nothing; the request server's contexts use this); 0 off (default:
+ * built-in choices).  A context's own setting (-1: follow the global one)
+ * overrides.  mexct_tuned_config fills cfg[4] = {found, FP views-per-chunk
+ * log2 or BP super-tile side, BP z-range, the winner's time on the tuning
+ * views in microseconds}. */
 int mexct_set_autotune(int32_t mode);
 int mexct_ctx_set_autotune(mexct_ctx* ctx, int32_t mode);
 int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int32_t count, int64_t W,

@@ -332,10 +332,11 @@ int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
 // Candidates differ only in which workgroup computes which rays or voxels
 // (FP: views per chunk of the block order; BP: super-tile side, and the
 // z-range for FP32 storage, whose row starts replay the same running sum
-// whatever the range), so every candidate writes the same bits.  The first
-// call of a problem shape times each candidate on that call's own data (the
-// launches are idempotent, so the last one leaves a valid result) and keeps
-// the fastest; later calls of the shape use it.
+// whatever the range), so every candidate writes the same bits.  The tuning
+// call of a problem shape times each candidate once on the first
+// kTuneViews views of its own data, keeps the fastest for later calls of
+// the shape, and then runs the whole call with it.  The timed launches cost
+// about candidates x kTuneViews / nProj of the call.
 struct TuneKey {
   int op, geom, st, items;
   int64_t W, H, n;
@@ -374,8 +375,10 @@ bool want_tune(const mexct_ctx* c, const TuneKey& k) {
   return ++g_seen[k] >= mode;
 }
 
-// Times run(i) for each candidate i on the context's stream (best of two)
-// and returns the fastest index.
+constexpr int kTuneViews = 32;
+
+// Times run(i) for each candidate i on the context's stream and returns the
+// fastest index.
 template <typename F>
 int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
   hipEvent_t e0, e1;
@@ -388,7 +391,7 @@ int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
   float bt = 1e30f;
   int rc = MEXCT_OK;
   for (int i = 0; i < ncand && rc == MEXCT_OK; ++i) {
-    for (int rep = 0; rep < 2 && rc == MEXCT_OK; ++rep) {
+    for (int rep = 0; rep < 1 && rc == MEXCT_OK; ++rep) {
       if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(MEXCT_EHIP, "event record failed"); break; }
       rc = run(i);
       if (rc) break;
@@ -581,13 +584,14 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
       const int best = tune_pick(c, 4, [&](int i) {
         FPZLaunch t = z;
         t.vchunk_log2 = cand[i];
+        t.nviews = std::min<int>(nProj, kTuneViews);
         HIPCHK(launch_fpz(t, c->stream));
         return MEXCT_OK;
       }, &tv.ms);
       if (best < 0) return best;
       tv.a = cand[best];
       tune_store(key, tv);
-      return MEXCT_OK;  // the candidates' launches wrote the result
+      z.vchunk_log2 = tv.a;
     }
     HIPCHK(launch_fpz(z, c->stream));
     return MEXCT_OK;
@@ -753,6 +757,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
       BPLaunch t = a;
       t.super_ = cand[i].first;
       t.zr = cand[i].second;
+      t.nviews = (int)std::min<int64_t>(nProj, kTuneViews);
       HIPCHK(launch_bp(t, c->stream));
       return MEXCT_OK;
     }, &tv.ms);
@@ -760,7 +765,8 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
     tv.a = cand[best].first;
     tv.b = cand[best].second;
     tune_store(key, tv);
-    return MEXCT_OK;  // the candidates' launches wrote the result
+    a.super_ = tv.a;
+    a.zr = tv.b;
   }
   HIPCHK(launch_bp(a, c->stream));
   return MEXCT_OK;

@@ -261,8 +261,12 @@ int run_batch(Worker& w, const std::vector<Req*>& g) {
     TRY(mexct_bp_batch_dev(w.ctx, bp_in, MEXCT_F32, n, r.W, r.H, r.nProj, r.sad, ang, r.geom,
                            (float*)w.o.p, &o));
   }
-  for (int i = 0; i < n; ++i) TRY(copy_out(w, g[i]->r.output, (char*)w.o.p + i * out_b, out_b));
-  return MEXCT_OK;
+  hipStream_t st = mexct::ctx_stream(w.ctx);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < n && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(g[i]->r.output, (char*)w.o.p + i * out_b, out_b, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
 }
 
 enum class Pick { kWait, kRun, kRelease };
@@ -293,7 +297,13 @@ Pick choose(mexct_server* sv, Worker& w, std::vector<Req*>& group) {
     // a batch holds every item's buffers at once; back-to-back runs reuse one set
     const bool batch = batchable(q->r);
     int64_t group_fp = q->footprint;
-    for (auto jt = it; jt != sv->queue.end() && (int)group.size() < sv->max_group;) {
+    // share the queued requests of this key among all workers, so that a
+    // burst keeps every worker's copies and kernels overlapping
+    int same = 1;
+    for (auto jt = it; jt != sv->queue.end(); ++jt) same += same_key(**jt, *q) ? 1 : 0;
+    const int nw = (int)sv->workers.size();
+    const int cap = std::min(sv->max_group, (same + nw - 1) / nw);
+    for (auto jt = it; jt != sv->queue.end() && (int)group.size() < cap;) {
       const int64_t more = batch ? group_fp + (*jt)->footprint : group_fp;
       if (same_key(**jt, *q) && others + std::max(w.reserved, more) <= sv->budget) {
         group.push_back(*jt);
