@@ -221,45 +221,57 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       // column is the 4-row run shifted down by j1 - j0 halves.
       float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
       const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(HALO * row_bytes);
-      constexpr int GP = 8;  // voxel pairs per load group (4: 12.0 ms, 8: 11.3 ms)
-#pragma unroll
-      for (int m0 = 0; m0 < ZR; m0 += 2 * GP) {
-        float az0[GP], az1[GP];
-        uint32_t sh[GP];
-        u4v q[GP];
+      // Groups of GP voxel pairs, software-pipelined: the loads of group
+      // g+1 are issued before group g is interpolated, so loads stay in
+      // flight while the wave computes.
+      constexpr int GP = 4;
+      constexpr int NG = ZR / (2 * GP);
+      float az0[2][GP], az1[2][GP];
+      uint32_t sh[2][GP];
+      u4v q[2][GP];
+      auto issue = [&](int b) {
 #pragma unroll
         for (int g = 0; g < GP; ++g) {
           const float zb1 = zb + dz;
-          az0[g] = __builtin_amdgcn_fractf(zb);
-          az1[g] = __builtin_amdgcn_fractf(zb1);
+          az0[b][g] = __builtin_amdgcn_fractf(zb);
+          az1[b][g] = __builtin_amdgcn_fractf(zb1);
           const int j0 = clamp_m2(cvt_flr(zb), a.H);
           const int j1 = clamp_m2(cvt_flr(zb1), a.H);
-          sh[g] = (uint32_t)(j1 - j0) << 4;
-          q[g] = __builtin_amdgcn_raw_buffer_load_b128(r, (uint32_t)__mul24(j0, row_bytes) + cb,
-                                                       soff, 0);
+          sh[b][g] = (uint32_t)(j1 - j0) << 4;
+          q[b][g] = __builtin_amdgcn_raw_buffer_load_b128(r, (uint32_t)__mul24(j0, row_bytes) + cb,
+                                                          soff, 0);
           zb = zb1 + dz;
         }
-        __builtin_amdgcn_sched_barrier(0);
+      };
+      auto consume = [&](int b, int m0) {
 #pragma unroll
         for (int g = 0; g < GP; ++g) {
           // q.x, q.y: rows j0..j0+3 of column i0; q.z, q.w: of column i0+1
-          const uint32_t A1 = (uint32_t)((((uint64_t)q[g].y << 32) | q[g].x) >> sh[g]);
-          const uint32_t B1 = (uint32_t)((((uint64_t)q[g].w << 32) | q[g].z) >> sh[g]);
+          const u4v v = q[b][g];
+          const uint32_t A1 = (uint32_t)((((uint64_t)v.y << 32) | v.x) >> sh[b][g]);
+          const uint32_t B1 = (uint32_t)((((uint64_t)v.w << 32) | v.z) >> sh[b][g]);
           float c0, c1, d0, d1;
           if constexpr (__is_same(S, _Float16)) {
-            c0 = lerp_h<0>(q[g].x, q[g].z, ax);
-            c1 = lerp_h<1>(q[g].x, q[g].z, ax);
+            c0 = lerp_h<0>(v.x, v.z, ax);
+            c1 = lerp_h<1>(v.x, v.z, ax);
             d0 = lerp_h<0>(A1, B1, ax);
             d1 = lerp_h<1>(A1, B1, ax);
           } else {
-            c0 = lerp(__uint_as_float(q[g].x << 16), __uint_as_float(q[g].z << 16), ax);
-            c1 = lerp(__uint_as_float(q[g].x & 0xffff0000u), __uint_as_float(q[g].z & 0xffff0000u), ax);
+            c0 = lerp(__uint_as_float(v.x << 16), __uint_as_float(v.z << 16), ax);
+            c1 = lerp(__uint_as_float(v.x & 0xffff0000u), __uint_as_float(v.z & 0xffff0000u), ax);
             d0 = lerp(__uint_as_float(A1 << 16), __uint_as_float(B1 << 16), ax);
             d1 = lerp(__uint_as_float(A1 & 0xffff0000u), __uint_as_float(B1 & 0xffff0000u), ax);
           }
-          acc[m0 + 2 * g] = __builtin_fmaf(UU, lerp(c0, c1, az0[g]), acc[m0 + 2 * g]);
-          acc[m0 + 2 * g + 1] = __builtin_fmaf(UU, lerp(d0, d1, az1[g]), acc[m0 + 2 * g + 1]);
+          acc[m0 + 2 * g] = __builtin_fmaf(UU, lerp(c0, c1, az0[b][g]), acc[m0 + 2 * g]);
+          acc[m0 + 2 * g + 1] = __builtin_fmaf(UU, lerp(d0, d1, az1[b][g]), acc[m0 + 2 * g + 1]);
         }
+      };
+      issue(0);
+#pragma unroll
+      for (int gi = 0; gi < NG; ++gi) {
+        if (gi + 1 < NG) issue((gi + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(gi & 1, gi * 2 * GP);
       }
       continue;
     }

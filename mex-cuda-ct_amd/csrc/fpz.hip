@@ -150,7 +150,10 @@ __device__ __forceinline__ float4 dual_ld(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // corner come from v_fract / v_cvt_flr.  Sample positions then differ from
 // the reference's float rounding by a few ulps, far below the storage
 // rounding, and each sample costs 9 VALU less.
-template <typename S, int LAYOUT>
+// CLAMP false (2-byte storage only): the caller has shown that every sample
+// lies where the 2-texel halo serves it, floor(t) in [-2, N], so the clamp
+// would change nothing.
+template <typename S, int LAYOUT, bool CLAMP = true>
 __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a,
                                            const R3& cur, float fnx, float fny, float fnz,
                                            int base) {
@@ -160,9 +163,14 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
     t.ax = __builtin_amdgcn_fractf(cur.x);
     t.ay = __builtin_amdgcn_fractf(cur.y);
     t.az = __builtin_amdgcn_fractf(cur.z);
-    i = clamp_m2(cvt_flr(cur.x), a.NX);
-    j = clamp_m2(cvt_flr(cur.y), a.NY);
-    kk = clamp_m2(cvt_flr(cur.z), a.NZ);
+    i = cvt_flr(cur.x);
+    j = cvt_flr(cur.y);
+    kk = cvt_flr(cur.z);
+    if constexpr (CLAMP) {
+      i = clamp_m2(i, a.NX);
+      j = clamp_m2(j, a.NY);
+      kk = clamp_m2(kk, a.NZ);
+    }
   } else {
     const float xb = (cur.x - a.bx) - 0.5f;
     const float yb = (cur.y - a.by) - 0.5f;
@@ -247,12 +255,30 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
 // One ray through the z-fastest volume, software-pipelined: the loads of
 // sample k+1 are issued before sample k is interpolated.  A fetch past the
 // last sample reads a clamped, in-range corner and is discarded.
+// Whether every sample cur + k inc (k < n, texel coordinates, running sum)
+// has floor in [-2, N] on each axis, the range the 2-texel halo serves.  The
+// samples lie on the segment between the first and the last, each off by
+// at most the running sum's rounding, bounded here by n half-ulps of the
+// largest coordinate; the ends must then lie in [-1.5, N + 0.5].
+__device__ __forceinline__ bool in_halo(const R3& c, const R3& inc, int n, float fnx, float fny,
+                                        float fnz) {
+  const float k = (float)(n - 1);
+  const float ex = __builtin_fmaf(k, inc.x, c.x), ey = __builtin_fmaf(k, inc.y, c.y),
+              ez = __builtin_fmaf(k, inc.z, c.z);
+  const float big = fmaxf(fmaxf(fmaxf(fabsf(c.x), fabsf(ex)), fmaxf(fabsf(c.y), fabsf(ey))),
+                          fmaxf(fabsf(c.z), fabsf(ez))) + 1.f;
+  const float d = (float)n * big * 1.2e-7f + 0.01f;  // drift bound, 2^-23 per step
+  return fminf(c.x, ex) - d >= -1.5f && fmaxf(c.x, ex) + d <= fnx + 0.5f &&
+         fminf(c.y, ey) - d >= -1.5f && fmaxf(c.y, ey) + d <= fny + 0.5f &&
+         fminf(c.z, ez) - d >= -1.5f && fmaxf(c.z, ez) + d <= fnz + 0.5f;
+}
+
 // The sample loop.  COMP (2-byte storage, long rays): each pair of samples
 // is added to the ray sum with Kahan's compensated summation, so the sum's
 // rounding error stays near one ulp however many samples the ray has
 // instead of growing with their count.  Otherwise the samples are added one
 // by one in ray order, as the reference does.
-template <typename S, int STEP2, int LAYOUT, bool COMP>
+template <typename S, int STEP2, int LAYOUT, bool COMP, bool CLAMP>
 __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                             const R3& inc, int n, float fnx, float fny, float fnz,
                                             int base) {
@@ -260,20 +286,20 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
   float dval = 0.f, comp = 0.f;
   // two register sets in ping-pong: while one sample is interpolated the
   // next sample's loads are in flight
-  Tap8<S> t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
+  Tap8<S> t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
   int k = 0;
   for (; k + 1 < n; k += 2) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    const Tap8<S> t1 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
+    const Tap8<S> t1 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
     float x = interp_z<S, LAYOUT>(t0) * step;
     if constexpr (!COMP) dval = dval + x;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    t0 = fetch_z<S, LAYOUT>(r, a, cur, fnx, fny, fnz, base);
+    t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (COMP) {
       x = x + interp_z<S, LAYOUT>(t1) * step;
@@ -307,11 +333,17 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     cur.x = (cur.x - a.bx) - 0.5f;
     cur.y = (cur.y - a.by) - 0.5f;
     cur.z = (cur.z - a.bz) - 0.5f;
-    // one loop per wave: compensated when any of its rays is long
-    if (__builtin_amdgcn_ballot_w64(n > a.long_ray) != 0)
-      return march_loop<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
+    // one loop per wave: compensated when any of its rays is long; without
+    // the index clamps when every ray stays inside the halo
+    const bool comp = __builtin_amdgcn_ballot_w64(n > a.long_ray) != 0;
+    const bool clamp = __builtin_amdgcn_ballot_w64(!in_halo(cur, inc, n, fnx, fny, fnz)) != 0;
+    if (comp)
+      return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
+                   : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
+    return clamp ? march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
+                 : march_loop<S, STEP2, LAYOUT, false, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
   }
-  return march_loop<S, STEP2, LAYOUT, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
+  return march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
 }
 
 // Storage of the OSC ratio terms for a volume storage S.  They span the
