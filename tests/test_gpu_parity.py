@@ -436,6 +436,26 @@ def test_bp_storage_on_rounded_sinogram(gpu, geom, storage, sad):
     _check(got, ref, f"BP storage={storage} geom={geom} sad={sad} vs rounded", 1e-4, 2e-5)
 
 
+# The 2-byte cone row-quad BP at its limits: magnification up to 1.98 (SAD
+# just above 2.02 hw), heights that are not a multiple of the z-range (the
+# last range loads rows past H, which the row clamp maps to the edge) and
+# detectors much taller than wide.
+@pytest.mark.parametrize("W,H,n,sad", [(48, 100, 10, 47.0), (64, 70, 12, 90.0),
+                                       (40, 150, 6, 400.0), (96, 33, 9, 1000.0)])
+def test_bp_cone_row_quad_limits(gpu, W, H, n, sad):
+    P = np.asfortranarray(_rng(W + H).uniform(0, 1, (W, H, n)).astype(np.float32))
+    ang = _angles(n)
+    got = mexct.CUDAmex_BP(P, sad, ang, 2, storage=mexct.F16)
+    Pr = _round_storage(P, mexct.F16)
+    # against the FP64 twin: the FP32 oracle replays the reference's running
+    # row sum, whose drift over tall detectors (H = 150: ~1e-4 relative) the
+    # 2-byte path's direct row start does not share
+    ref = O.bp(Pr, sad, ang, 2, f64=True)
+    m32, _ = rel_errors(got, O.bp(Pr, sad, ang, 2))
+    print(f"  (vs the FP32 oracle: rel_max={m32:.3e})")
+    _check(got, ref, f"BP f16 cone W={W} H={H} sad={sad} vs rounded, FP64", 1e-4, 2e-5)
+
+
 @pytest.mark.parametrize("storage", [mexct.F16, mexct.BF16])
 def test_g3d_fp_storage_on_rounded_volume(gpu, storage):
     side, nsl, n = 24, 20, 4
