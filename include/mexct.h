@@ -114,7 +114,8 @@ int mexct_bp_batch_dev(mexct_ctx* ctx, const void* sinos, int32_t sino_dtype, in
  * views per chunk; BP: super-tile side, and the z-range for FP32 storage
  * unless opts fixes it) is tuned per problem shape (operator, geometry,
  * storage, batch count, W, H, nProj): the tuning call of a shape times
- * every candidate once on its first 32 views, keeps the fastest for the
+ * every candidate once on its first 128 (FP) or 32 (BP) views, keeps the
+ * fastest for the
  * process and runs the call with it.  Candidates write identical bits.
  * mode 1: tune at a shape's first call; 2: at its second (one-off shapes
  * pay nothing; the request server's contexts use this); 0 off (default:

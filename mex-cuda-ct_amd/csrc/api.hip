@@ -334,9 +334,10 @@ int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
 // z-range for FP32 storage, whose row starts replay the same running sum
 // whatever the range), so every candidate writes the same bits.  The tuning
 // call of a problem shape times each candidate once on the first
-// kTuneViews views of its own data, keeps the fastest for later calls of
-// the shape, and then runs the whole call with it.  The timed launches cost
-// about candidates x kTuneViews / nProj of the call.
+// kTuneViews (BP) or kTuneViewsFP (FP) views of its own data, keeps the
+// fastest for later calls of the shape, and then runs the whole call with
+// it.  The timed launches cost about candidates x those views / nProj of
+// the call.
 struct TuneKey {
   int op, geom, st, items;
   int64_t W, H, n;
@@ -375,12 +376,14 @@ bool want_tune(const mexct_ctx* c, const TuneKey& k) {
   return ++g_seen[k] >= mode;
 }
 
-constexpr int kTuneViews = 32;
+constexpr int kTuneViews = 32;     // BP candidates
+constexpr int kTuneViewsFP = 128;  // FP: the view chunk's L2 reuse needs several chunks
 
 // Times run(i) for each candidate i on the context's stream and returns the
-// fastest index.
+// fastest index; candidate `dflt` (the built-in choice) is kept unless
+// another is at least 3% faster, so that timing noise does not move it.
 template <typename F>
-int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
+int tune_pick(mexct_ctx* c, int ncand, int dflt, F run, float* best_ms) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -388,7 +391,7 @@ int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
     return fail(MEXCT_EHIP, "hipEventCreate failed");
   }
   int best = 0;
-  float bt = 1e30f;
+  float bt = 1e30f, dt = 1e30f;
   int rc = MEXCT_OK;
   for (int i = 0; i < ncand && rc == MEXCT_OK; ++i) {
     for (int rep = 0; rep < 1 && rc == MEXCT_OK; ++rep) {
@@ -401,6 +404,7 @@ int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
         rc = fail(MEXCT_EHIP, "event timing failed");
         break;
       }
+      if (i == dflt) dt = ms;
       if (ms < bt) {
         bt = ms;
         best = i;
@@ -409,6 +413,10 @@ int tune_pick(mexct_ctx* c, int ncand, F run, float* best_ms) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (dflt >= 0 && dflt < ncand && !(bt < 0.97f * dt)) {
+    best = dflt;
+    bt = dt;
+  }
   *best_ms = bt;
   return rc ? rc : best;  // error codes are negative
 }
@@ -581,10 +589,10 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
       z.vchunk_log2 = tv.a;
     } else if (want_tune(c, key)) {
       static const int cand[] = {2, 3, 4, 5};  // 4 .. 32 views per chunk
-      const int best = tune_pick(c, 4, [&](int i) {
+      const int best = tune_pick(c, 4, 2, [&](int i) {
         FPZLaunch t = z;
         t.vchunk_log2 = cand[i];
-        t.nviews = std::min<int>(nProj, kTuneViews);
+        t.nviews = std::min<int>(nProj, kTuneViewsFP);
         HIPCHK(launch_fpz(t, c->stream));
         return MEXCT_OK;
       }, &tv.ms);
@@ -753,7 +761,10 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
     a.chain_views = (int)cv;
   }
   if (tune) {
-    const int best = tune_pick(c, (int)cand.size(), [&](int i) {
+    int dflt = 0;  // the built-in choice: super-tile 4 at the default z-range
+    for (size_t i = 0; i < cand.size(); ++i)
+      if (cand[i].first == 4 && cand[i].second == pick_zrange(opts, geom, H)) dflt = (int)i;
+    const int best = tune_pick(c, (int)cand.size(), dflt, [&](int i) {
       BPLaunch t = a;
       t.super_ = cand[i].first;
       t.zr = cand[i].second;
