@@ -113,7 +113,7 @@ int mexct_bp_batch_dev(mexct_ctx* ctx, const void* sinos, int32_t sino_dtype, in
  * No counterpart in the reference.  The FP and BP kernels' block order (FP:
  * views per chunk; BP: super-tile side, and the z-range for FP32 storage
  * unless opts fixes it) is tuned per problem shape (operator, geometry,
- * storage, batch count, W, H, nProj): the tuning call of a shape times
+ * storage, W, H, nProj; any batch count): the tuning call of a shape times
  * every candidate once on its first 128 (FP) or 32 (BP) views, keeps the
  * fastest for the
  * process and runs the call with it.  Candidates write identical bits.
@@ -125,8 +125,8 @@ int mexct_bp_batch_dev(mexct_ctx* ctx, const void* sinos, int32_t sino_dtype, in
  * views in microseconds}. */
 int mexct_set_autotune(int32_t mode);
 int mexct_ctx_set_autotune(mexct_ctx* ctx, int32_t mode);
-int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int32_t count, int64_t W,
-                       int64_t H, int64_t nProj, int32_t* cfg);
+int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int64_t W, int64_t H,
+                       int64_t nProj, int32_t* cfg);
 
 /* ------------------------------------------------------------------ general-3D FP
  * fp = CUDAmex_general3D_FP(vol, angles, pts, nProjections)

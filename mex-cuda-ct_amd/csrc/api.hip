@@ -338,11 +338,13 @@ int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
 // fastest for later calls of the shape, and then runs the whole call with
 // it.  The timed launches cost about candidates x those views / nProj of
 // the call.
+// One choice per shape serves every batch size, so that batches of varying
+// size do not each pay for tuning.
 struct TuneKey {
-  int op, geom, st, items;
+  int op, geom, st;
   int64_t W, H, n;
   bool operator<(const TuneKey& o) const {
-    return std::tie(op, geom, st, items, W, H, n) < std::tie(o.op, o.geom, o.st, o.items, o.W, o.H, o.n);
+    return std::tie(op, geom, st, W, H, n) < std::tie(o.op, o.geom, o.st, o.W, o.H, o.n);
   }
 };
 struct TuneVal {
@@ -583,7 +585,7 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
     z.items = items;
     z.vol_stride = lbytes;
     z.out_stride = (uint64_t)W * H * nProj;
-    const TuneKey key{MEXCT_OP_FP, geom, st, items, W, H, nProj};
+    const TuneKey key{MEXCT_OP_FP, geom, st, W, H, nProj};
     TuneVal tv;
     if (tune_find(key, &tv)) {
       z.vchunk_log2 = tv.a;
@@ -725,7 +727,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   a.items = items;
   a.sino_stride = sino_stride;
   a.out_stride = (uint64_t)W * W * H;
-  const TuneKey key{MEXCT_OP_BP, geom, st, items, W, H, nProj};
+  const TuneKey key{MEXCT_OP_BP, geom, st, W, H, nProj};
   TuneVal tv;
   const bool found = tune_find(key, &tv);
   const bool tune = !found && nProj > 0 && want_tune(c, key);
@@ -810,11 +812,11 @@ int mexct_ctx_set_autotune(mexct_ctx* c, int32_t mode) {
   return MEXCT_OK;
 }
 
-int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int32_t count, int64_t W,
-                       int64_t H, int64_t nProj, int32_t* cfg) {
+int mexct_tuned_config(int32_t op, int32_t geom, int32_t storage, int64_t W, int64_t H,
+                       int64_t nProj, int32_t* cfg) {
   if (!cfg) return fail(MEXCT_EINVAL, "cfg is null");
   TuneVal v;
-  const bool f = tune_find(TuneKey{op, geom, storage, count, W, H, nProj}, &v);
+  const bool f = tune_find(TuneKey{op, geom, storage, W, H, nProj}, &v);
   cfg[0] = f ? 1 : 0;
   cfg[1] = f ? v.a : -1;
   cfg[2] = f ? v.b : -1;

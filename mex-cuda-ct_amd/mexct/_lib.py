@@ -61,7 +61,7 @@ SIGNATURES = {
     "mexct_bp_batch_dev": [_vp, _vp, _i32, _i32, _i64, _i64, _i64, _f, _vp, _i32, _vp, _op],
     "mexct_set_autotune": [_i32],
     "mexct_ctx_set_autotune": [_vp, _i32],
-    "mexct_tuned_config": [_i32, _i32, _i32, _i32, _i64, _i64, _i64, _vp],
+    "mexct_tuned_config": [_i32, _i32, _i32, _i64, _i64, _i64, _vp],
     "mexct_g3d_fp": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _i32, _i32, _vp, _op],
     "mexct_g3d_fp_dev": [_vp, _vp, _i32, _i64, _i64, _vp, _vp, _i64, _i64, _i32, _i32, _vp, _op],
     "mexct_g3d_bp": [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i32, _i64, _i64, _vp, _op],

@@ -252,10 +252,10 @@ def set_autotune(mode):
     check(lib().mexct_set_autotune(int(mode)))
 
 
-def tuned_config(op, geom, storage, count, W, H, nProj):
+def tuned_config(op, geom, storage, W, H, nProj):
     """(found, FP views-per-chunk log2 or BP super-tile side, BP z-range,
     winner's time in us) for a problem shape; op 0 FP, 1 BP."""
     cfg = np.zeros(4, dtype=np.int32)
-    check(lib().mexct_tuned_config(int(op), int(geom), int(storage), int(count), int(W), int(H),
-                                   int(nProj), _ptr(cfg)))
+    check(lib().mexct_tuned_config(int(op), int(geom), int(storage), int(W), int(H), int(nProj),
+                                   _ptr(cfg)))
     return tuple(int(x) for x in cfg)

@@ -101,7 +101,7 @@ def test_autotune_same_bits_and_remembered(torch_cuda, storage):
     bp0 = torch.zeros((H, W, W), dtype=torch.float32, device="cuda")
     plain.fp(vol, sad, ang, n, 2, fp0, storage=storage)
     plain.bp(sino, sad, ang, 2, bp0, storage=storage)
-    assert mexct.tuned_config(0, 2, storage, 1, W, H, n)[0] == 0
+    assert mexct.tuned_config(0, 2, storage, W, H, n)[0] == 0
     tuned = device.Context(0)
     tuned.set_autotune(1)
     fp1 = torch.zeros_like(fp0)
@@ -110,8 +110,8 @@ def test_autotune_same_bits_and_remembered(torch_cuda, storage):
     tuned.bp(sino, sad, ang, 2, bp1, storage=storage)
     torch.cuda.synchronize()
     assert torch.equal(fp0, fp1) and torch.equal(bp0, bp1)
-    cf = mexct.tuned_config(0, 2, storage, 1, W, H, n)
-    cb = mexct.tuned_config(1, 2, storage, 1, W, H, n)
+    cf = mexct.tuned_config(0, 2, storage, W, H, n)
+    cb = mexct.tuned_config(1, 2, storage, W, H, n)
     print(f"tuned FP {cf}, BP {cb}")
     assert cf[0] == 1 and 2 <= cf[1] <= 5
     assert cb[0] == 1 and cb[1] in (2, 4, 8) and cb[2] in (32, 64)
@@ -135,9 +135,9 @@ def test_autotune_mode2_tunes_recurring_shapes(torch_cuda):
     ctx.set_autotune(2)
     outs = [torch.zeros((n, H, W), dtype=torch.float32, device="cuda") for _ in range(3)]
     ctx.fp(vol, sad, ang, n, 1, outs[0])
-    assert mexct.tuned_config(0, 1, mexct.F32, 1, W, H, n)[0] == 0  # one-off so far
+    assert mexct.tuned_config(0, 1, mexct.F32, W, H, n)[0] == 0  # one-off so far
     ctx.fp(vol, sad, ang, n, 1, outs[1])
-    assert mexct.tuned_config(0, 1, mexct.F32, 1, W, H, n)[0] == 1  # tuned at the 2nd call
+    assert mexct.tuned_config(0, 1, mexct.F32, W, H, n)[0] == 1  # tuned at the 2nd call
     ctx.fp(vol, sad, ang, n, 1, outs[2])
     torch.cuda.synchronize()
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
