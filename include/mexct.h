@@ -53,7 +53,7 @@ extern "C" {
 typedef struct mexct_opts {
   int32_t storage;  /* MEXCT_F32 (default, reference-exact), MEXCT_F16, MEXCT_BF16 */
   int32_t tv_steps; /* TVmin gradient steps; 0 -> 20 as CUDAmex_TVmin_3D.cu:221  */
-  int32_t zrange;   /* BP z-voxels per thread; 0 -> autotuned                     */
+  int32_t zrange;   /* BP z-voxels per thread: 32, 48 or 64; 0 -> chosen by H    */
   int32_t long_ray; /* F16/BF16 FP: rays of more samples than this are summed with  */
                     /* compensated (Kahan) summation; 0 -> 1024, < 0 -> never       */
 } mexct_opts;

@@ -322,9 +322,11 @@ int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out, int 
 int tv_steps_of(const mexct_opts* o) { return (o && o->tv_steps > 0) ? o->tv_steps : 20; }
 
 int pick_zrange(const mexct_opts* o, int geom, int64_t H) {
-  if (o && (o->zrange == 32 || o->zrange == 64)) return o->zrange;
+  if (o && (o->zrange == 32 || o->zrange == 48 || o->zrange == 64)) return o->zrange;
   (void)geom;
-  return H >= 256 ? 64 : 32;
+  // 48 at 384 rows: 8 ranges, 18432 blocks at 384^3 fill the SIMDs in whole
+  // rounds (64: 13824 blocks, 4.5 rounds); 1-3% faster than 64 there
+  return H >= 256 ? 48 : 32;
 }
 
 // ---- autotuner of the kernels' block order
@@ -733,11 +735,12 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   const bool tune = !found && nProj > 0 && want_tune(c, key);
   // candidates: super-tile side 2, 4, 8; the z-range too for FP32 storage
   // unless the caller fixed it
-  const bool zr_free = st == MEXCT_F32 && !(opts && (opts->zrange == 32 || opts->zrange == 64));
+  const bool zr_free =
+      st == MEXCT_F32 && !(opts && (opts->zrange == 32 || opts->zrange == 48 || opts->zrange == 64));
   std::vector<std::pair<int, int>> cand;
   for (int sup : {2, 4, 8}) {
     if (zr_free)
-      for (int zr : {32, 64}) cand.push_back({sup, zr});
+      for (int zr : {32, 48, 64}) cand.push_back({sup, zr});
     else
       cand.push_back({sup, a.zr});
   }

@@ -381,6 +381,8 @@ template <typename S>
 static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, int rows, hipStream_t st) {
   if (zr == 64)
     bp_dispatch<64, S>(k, geom, rows, st);
+  else if (zr == 48)
+    bp_dispatch<48, S>(k, geom, rows, st);
   else
     bp_dispatch<32, S>(k, geom, rows, st);
 }
@@ -397,7 +399,7 @@ size_t bp_chain_floats_per_view(int W, int H, int zr, int super_) {
 // buffer offsets); each chunk continues the register sums the previous one
 // stored, so the per-voxel order and the float sums are unchanged.
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
-  const int zr = a.zr == 64 ? 64 : 32;
+  const int zr = a.zr == 64 || a.zr == 48 ? a.zr : 32;
   const int rows = a.st_dt != 0 && a.rows == 4 ? 4 : 2;
   const size_t es2 = (size_t)rows * (a.st_dt == 0 ? 4u : 2u);
   BPKArgs k;
@@ -440,6 +442,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
       const unsigned cg = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
       if (zr == 64)
         bp_chain_kernel<64><<<cg, 256, 0, st>>>(k, a.chain);
+      else if (zr == 48)
+        bp_chain_kernel<48><<<cg, 256, 0, st>>>(k, a.chain);
       else
         bp_chain_kernel<32><<<cg, 256, 0, st>>>(k, a.chain);
       k.chain = a.chain;

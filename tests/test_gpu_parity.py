@@ -60,7 +60,7 @@ def test_fp_unwritten_rows(gpu):
 
 
 @pytest.mark.parametrize("geom", GEOMS)
-@pytest.mark.parametrize("zrange", [32, 64])
+@pytest.mark.parametrize("zrange", [32, 48, 64])
 @pytest.mark.parametrize("sad", [60.0, 1000.0])
 def test_bp_std(gpu, geom, zrange, sad):
     W, H, n = 40, 72, 9
