@@ -1019,7 +1019,7 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
                 int numSubsets, const int32_t* picks, const float2* sc, float sad, int geom,
                 int st) {
   const int cmax = (nProj + numSubsets - 1) / numSubsets;
-  const size_t pv = (size_t)(W + 4) * (H + 4);
+  const size_t pv = (size_t)2 * (W + 4) * (H + 3);  // row-paired copies
   const size_t es = (size_t)store_bytes(st);
   const int pst = st == MEXCT_F16 ? MEXCT_BF16 : st;  // ratio-term storage
   // whole 32-bit words so the zero fill covers every 2-byte element

@@ -221,6 +221,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       // column is the 4-row run shifted down by j1 - j0 halves.
       float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
       const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(HALO * row_bytes);
+      const uint32_t ax2 = pk_h2(ax);
       // Groups of GP voxel pairs, software-pipelined: the loads of group
       // g+1 are issued before group g is interpolated, so loads stay in
       // flight while the wave computes.
@@ -251,7 +252,13 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
           const uint32_t A1 = (uint32_t)((((uint64_t)v.y << 32) | v.x) >> sh[b][g]);
           const uint32_t B1 = (uint32_t)((((uint64_t)v.w << 32) | v.z) >> sh[b][g]);
           float c0, c1, d0, d1;
-          if constexpr (__is_same(S, _Float16)) {
+          if constexpr (__is_same(S, _Float16) && kPackedF16Lerp) {
+            acc[m0 + 2 * g] =
+                __builtin_fmaf(UU, lerp_x(pk_lerp_h(v.x, v.z, ax2), az0[b][g]), acc[m0 + 2 * g]);
+            acc[m0 + 2 * g + 1] =
+                __builtin_fmaf(UU, lerp_x(pk_lerp_h(A1, B1, ax2), az1[b][g]), acc[m0 + 2 * g + 1]);
+            continue;
+          } else if constexpr (__is_same(S, _Float16)) {
             c0 = lerp_h<0>(v.x, v.z, ax);
             c1 = lerp_h<1>(v.x, v.z, ax);
             d0 = lerp_h<0>(A1, B1, ax);
@@ -296,7 +303,11 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           float c0, c1;
-          if constexpr (__is_same(S, _Float16)) {
+          if constexpr (__is_same(S, _Float16) && kPackedF16Lerp) {
+            acc[m0 + g] =
+                __builtin_fmaf(UU, lerp_x(pk_lerp_h(q[g].x, q[g].y, pk_h2(ax)), az[g]), acc[m0 + g]);
+            continue;
+          } else if constexpr (__is_same(S, _Float16)) {
             c0 = lerp_h<0>(q[g].x, q[g].y, ax);
             c1 = lerp_h<1>(q[g].x, q[g].y, ax);
           } else {
@@ -465,16 +476,36 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
 // OSC dual backprojection + multiplicative update (CUDAmex_oscIter.cu:272-358
 // and :113-136) for one subset: num/den sums stay in registers over the
 // subset's views and the voxel is updated in place, inside the circle only.
+// The ratio terms arrive row-paired (the OSC forward projection writes them
+// so): pair (u, v) holds rows v and v+1 of column u, so the four taps of one
+// sinogram are the pairs at u and u+1, one load.
 struct OSCBPKArgs {
-  const void* pre_pad;   // storage S
+  const void* pre_pad;   // storage S, [2, W+4, H+3, count] with a zero halo
   const void* post_pad;
   float* mu_pad;
   const float2* sc;
-  int W, H, PW, first, stride, count;
-  int64_t PV, MPX, MPXY;
+  int W, H, PW, first, stride, count;  // PW: pairs per row
+  int64_t PV, MPX, MPXY;               // PV: pairs per view
   float v0, SR;
   uint32_t tiles_x, nz, n_logical;
 };
+
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef uint32_t u2a4 __attribute__((ext_vector_type(2), aligned(4)));
+// {(i0,j0), (i0,j0+1), (i0+1,j0), (i0+1,j0+1)} from the pair at p and the next
+template <typename S>
+__device__ __forceinline__ float4 osc_ld4(const S* p);
+template <>
+__device__ __forceinline__ float4 osc_ld4<float>(const float* p) {
+  const f4a8 v = *(const f4a8*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <>
+__device__ __forceinline__ float4 osc_ld4<bf16_t>(const bf16_t* p) {
+  const u2a4 u = *(const u2a4*)p;
+  return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                     __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+}
 
 constexpr int OSC_ZR = 32;
 
@@ -527,11 +558,14 @@ __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
     const float fx = floorf(xb);
     const float ax = xb - fx;
     const int i0 = (int)fminf(fmaxf(fx, -2.f), (float)a.W);
-    const size_t coff = (size_t)j * a.PV + (i0 + HALO);
+    const size_t coff = 2 * ((size_t)j * a.PV + (size_t)(i0 + HALO));
     const S* colN = (const S*)a.pre_pad + coff;
     const S* colD = (const S*)a.post_pad + coff;
-    if (GEOM == GEOM_CONE) {
-      for (int m = 0; m < z0; ++m) pz = pz + U;
+    if (GEOM == GEOM_CONE) {  // the running sum to z0, a multiple of OSC_ZR
+      for (int m = 0; m < z0; m += OSC_ZR) {
+#pragma unroll
+        for (int q = 0; q < OSC_ZR; ++q) pz = pz + U;
+      }
     } else {
       pz = pz + (float)z0;
     }
@@ -541,13 +575,11 @@ __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
       const float fz = floorf(zb);
       const float az = zb - fz;
       const int j0 = (int)fminf(fmaxf(fz, -2.f), (float)a.H);
-      const size_t r = (size_t)(j0 + HALO) * a.PW;
-      const float2 n0 = Store<S>::ld2(colN + r);
-      const float2 n1 = Store<S>::ld2(colN + r + a.PW);
-      const float2 d0 = Store<S>::ld2(colD + r);
-      const float2 d1 = Store<S>::ld2(colD + r + a.PW);
-      const float tn = lerp(lerp(n0.x, n0.y, ax), lerp(n1.x, n1.y, ax), az);
-      const float td = lerp(lerp(d0.x, d0.y, ax), lerp(d1.x, d1.y, ax), az);
+      const size_t r = 2 * (size_t)(j0 + HALO) * (size_t)a.PW;
+      const float4 n = osc_ld4<S>(colN + r);
+      const float4 d = osc_ld4<S>(colD + r);
+      const float tn = lerp(lerp(n.x, n.z, ax), lerp(n.y, n.w, ax), az);
+      const float td = lerp(lerp(d.x, d.z, ax), lerp(d.y, d.w, ax), az);
       accN[m] = accN[m] + W2 * tn;
       accD[m] = accD[m] + W2 * td;
       pz = pz + U;
@@ -585,7 +617,7 @@ hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st) {
   k.first = a.first;
   k.stride = a.stride;
   k.count = a.count;
-  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO);
+  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 3);
   k.MPX = a.W + 2 * HALO;
   k.MPXY = (int64_t)(a.W + 2 * HALO) * (a.W + 2 * HALO);
   k.v0 = float(a.H - 1) / 2;
@@ -597,10 +629,10 @@ hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st) {
   const unsigned g = padded_grid(k.n_logical);
   if (a.st_dt == 0)
     osc_bp_dispatch<float>(k, a.geom, g, st);
-  else if (a.st_dt == 1)
-    osc_bp_dispatch<_Float16>(k, a.geom, g, st);
-  else
+  else if (a.st_dt == 2)
     osc_bp_dispatch<bf16_t>(k, a.geom, g, st);
+  else
+    return hipErrorInvalidValue;  // ratio terms are f32 or bf16
   return hipGetLastError();
 }
 

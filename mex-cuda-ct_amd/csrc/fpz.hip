@@ -358,8 +358,19 @@ struct OscPad<_Float16> {
   typedef bf16_t T;
 };
 
+// Value v of detector pixel (u, row) into a row-paired copy: pair q (that
+// of (u, row)) holds it first, the pair of (u, row - 1) second.
+template <typename P>
+__device__ __forceinline__ void osc_pair_store(void* pad, size_t q, int PW, float v) {
+  P* p = (P*)pad;
+  const P s = Store<P>::from(v);
+  p[2 * q] = s;
+  p[2 * (q - (size_t)PW) + 1] = s;
+}
+
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
-// written in place, plus zero-haloed copies for the dual backprojection.
+// written in place, plus zero-haloed row-paired copies ([2, W+4, H+3, views])
+// for the dual backprojection.
 template <int GEOM, typename S, int EPI, int LAYOUT>
 __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   // Block order: (batch item, chunk of 2^vchunk_log2 views, column group,
@@ -398,8 +409,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     postval = a.post[p];
     q = (size_t)jv * a.PV + (size_t)(iy + HALO) * a.PW + (ix + HALO);
     if (iy >= a.rows) {  // unprocessed rows keep their values
-      ((P*)a.pre_pad)[q] = Store<P>::from(preval);
-      ((P*)a.post_pad)[q] = Store<P>::from(postval);
+      osc_pair_store<P>(a.pre_pad, q, a.PW, preval);
+      osc_pair_store<P>(a.post_pad, q, a.PW, postval);
       return;
     }
   }
@@ -427,8 +438,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
     const float npo = preval * expf(-dval) * dval;
     a.pre[p] = np;
     a.post[p] = npo;
-    ((P*)a.pre_pad)[q] = Store<P>::from(np);
-    ((P*)a.post_pad)[q] = Store<P>::from(npo);
+    osc_pair_store<P>(a.pre_pad, q, a.PW, np);
+    osc_pair_store<P>(a.post_pad, q, a.PW, npo);
   }
 }
 
@@ -542,8 +553,8 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   k.vol_max = a.vol_max;
   k.first = a.first;
   k.stride = a.stride;
-  k.PW = a.W + 2 * HALO;
-  k.PV = (a.W + 2 * HALO) * (a.H + 2 * HALO);
+  k.PW = a.W + 2 * HALO;        // pairs per row
+  k.PV = (a.W + 2 * HALO) * (a.H + 3);  // pairs per view: rows -2 .. H
   if (a.long_ray != 0) k.long_ray = a.long_ray;
   if (k.n_logical == 0) return hipSuccess;
   if (a.st_dt == 0)

@@ -93,7 +93,7 @@ struct OSCFPLaunch {
   const uint32_t* vol_max;  // non-null: mu_z holds mu * 2^-e (pow2_exp)
   float* pre;           // [W,H,nProj] updated in place
   float* post;          // [W,H,nProj] updated in place
-  void* pre_pad;        // [W+4,H+4,count] padded copies for the BP (zero halo)
+  void* pre_pad;        // [2,W+4,H+3,count] row-paired copies for the BP (zero halo)
   void* post_pad;
   const float2* sc;
   int geom, W, H, nProj, first, stride, count;
@@ -126,7 +126,7 @@ size_t bp_chain_floats_per_view(int W, int H, int zr, int super_);
 hipError_t launch_bp(const BPLaunch& a, hipStream_t st);
 
 struct OSCBPLaunch {
-  const void* pre_pad;    // [W+4, H+4, count] of storage st_dt
+  const void* pre_pad;    // [2, W+4, H+3, count] of storage st_dt (F32 or BF16)
   const void* post_pad;
   int st_dt;
   float* mu_pad;          // padded f32 volume, interior updated in place

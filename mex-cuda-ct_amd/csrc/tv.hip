@@ -4,7 +4,7 @@
 // Per step (CUDAmex_TVmin_3D.cu:221-234):
 //   1. tv_grad_kernel: 13-point TV gradient g on interior voxels (the
 //      border of g is 0), plus per-workgroup FP64 partial sums of g^2
-//      reduced with wave shuffles;
+//      reduced with wave shuffles (a workgroup: 64 x 4 x TVZC voxels);
 //   2. tv_scale_kernel: one workgroup sums the partials in a fixed order,
 //      |g| = (float)sqrt(sum), a = (-c)/|g| (the reference sums in float
 //      with Thrust and takes the sqrt on the host);
@@ -16,6 +16,7 @@
 namespace mexct {
 
 constexpr int TVBX = 64, TVBY = 4;
+constexpr int TVZC = 4;  // z-slices per workgroup of the reducing kernels (one partial each)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -42,10 +43,11 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(Vol3 x, float* __restrict_
                                                       double* __restrict__ partials) {
   const int ix = blockIdx.x * TVBX + threadIdx.x;
   const int iy = blockIdx.y * TVBY + threadIdx.y;
-  const int iz = blockIdx.z;
   const int Nx = x.nx, Ny = x.ny, Nz = x.nz;
   double sq = 0.0;
-  if (ix < Nx && iy < Ny) {
+  for (int k = 0; k < TVZC; ++k) {
+    const int iz = blockIdx.z * TVZC + k;
+    if (ix >= Nx || iy >= Ny || iz >= Nz) break;
     float tval = 0.f;
     if ((ix < (Nx - 1) && iy < (Ny - 1) && iz < (Nz - 1)) && (ix >= 1 && iy >= 1 && iz >= 1)) {
       const float eps = (float)1.0e-6;
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(Vol3 x, float* __restrict_
                                             (xyzp - xymzp) * (xyzp - xymzp)));
     }
     g[(size_t)ix + (size_t)Nx * ((size_t)iy + (size_t)Ny * iz)] = tval;
-    sq = (double)tval * (double)tval;
+    sq += (double)tval * (double)tval;
   }
   const double s = block_sum256(sq);
   if (threadIdx.x == 0 && threadIdx.y == 0)
@@ -87,9 +89,16 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(Vol3 x, float* __restrict_
 
 // Fixed-order sum of n partials by one 256-thread workgroup.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ partials, size_t n) {
-  double s = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += 256) s += partials[i];
-  return block_sum256(s);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four loads in flight per thread
+  size_t i = threadIdx.x;
+  for (; i + 768 < n; i += 1024) {
+    s0 += partials[i];
+    s1 += partials[i + 256];
+    s2 += partials[i + 512];
+    s3 += partials[i + 768];
+  }
+  for (; i < n; i += 256) s0 += partials[i];
+  return block_sum256((s0 + s1) + (s2 + s3));
 }
 
 __global__ __launch_bounds__(256) void tv_scale_kernel(const double* __restrict__ partials,
@@ -120,8 +129,14 @@ static dim3 tv_grid(int nx, int ny, int nz) {
   return dim3((unsigned)((nx + TVBX - 1) / TVBX), (unsigned)((ny + TVBY - 1) / TVBY), (unsigned)nz);
 }
 
+// grid of the reducing kernels: TVZC slices per workgroup
+static dim3 red_grid(int nx, int ny, int nz) {
+  return dim3((unsigned)((nx + TVBX - 1) / TVBX), (unsigned)((ny + TVBY - 1) / TVBY),
+              (unsigned)((nz + TVZC - 1) / TVZC));
+}
+
 size_t tv_partials_count(int nx, int ny, int nz) {
-  dim3 g = tv_grid(nx, ny, nz);
+  dim3 g = red_grid(nx, ny, nz);
   return (size_t)g.x * g.y * g.z;
 }
 
@@ -131,7 +146,7 @@ hipError_t launch_tvmin(Vol3 x, float* g, double* partials, float* a_dev, const 
   const dim3 block(TVBX, TVBY, 1);
   const size_t np = tv_partials_count(x.nx, x.ny, x.nz);
   for (int i = 0; i < steps; ++i) {
-    tv_grad_kernel<<<grid, block, 0, st>>>(x, g, partials);
+    tv_grad_kernel<<<red_grid(x.nx, x.ny, x.nz), block, 0, st>>>(x, g, partials);
     tv_scale_kernel<<<1, 256, 0, st>>>(partials, np, c_dev, c, a_dev);
     tv_axpy_kernel<<<grid, block, 0, st>>>(x, g, a_dev);
   }
@@ -143,15 +158,16 @@ __global__ __launch_bounds__(256) void diff_norm_kernel(Vol3 o, Vol3 n,
                                                         double* __restrict__ partials) {
   const int ix = blockIdx.x * TVBX + threadIdx.x;
   const int iy = blockIdx.y * TVBY + threadIdx.y;
-  const int iz = blockIdx.z;
   double sq = 0.0;
-  if (ix < o.nx && iy < o.ny) {
+  for (int k = 0; k < TVZC; ++k) {
+    const int iz = blockIdx.z * TVZC + k;
+    if (ix >= o.nx || iy >= o.ny || iz >= o.nz) break;
     float ov = o.p[(size_t)ix + (size_t)o.sy * iy + (size_t)o.sz * iz];
     if (ix == 0 && iy == 0 && iz == 0) ov = 0.0f;  // mu_old(1) = 0
     const float nv = n.p[(size_t)ix + (size_t)n.sy * iy + (size_t)n.sz * iz];
     float d = ov - nv;
     if (d != d) d = 0.0f;
-    sq = (double)d * (double)d;
+    sq += (double)d * (double)d;
   }
   const double s = block_sum256(sq);
   if (threadIdx.x == 0 && threadIdx.y == 0)
@@ -166,7 +182,7 @@ __global__ __launch_bounds__(256) void final_sum_kernel(const double* __restrict
 
 hipError_t launch_diff_norm(Vol3 old_v, Vol3 new_v, double* partials, double* sum,
                             hipStream_t st) {
-  const dim3 grid = tv_grid(old_v.nx, old_v.ny, old_v.nz);
+  const dim3 grid = red_grid(old_v.nx, old_v.ny, old_v.nz);
   diff_norm_kernel<<<grid, dim3(TVBX, TVBY, 1), 0, st>>>(old_v, new_v, partials);
   final_sum_kernel<<<1, 256, 0, st>>>(partials, tv_partials_count(old_v.nx, old_v.ny, old_v.nz),
                                       sum);
