@@ -280,6 +280,8 @@ int long_ray_of(const mexct_opts* o) {
 // 50.8 ms, DUAL 24.1 ms, QUAD 20.1 ms.
 int fp_layout(int64_t W, int64_t H, int st, size_t* bytes) {
   const size_t es = (size_t)store_bytes(st);
+  // the z-fastest projector's 24-bit index multiplies need (W+4)(H+4) < 2^23
+  if ((W + 4) * (H + 4) >= ((int64_t)1 << 23)) return -1;
   const size_t plain = (size_t)(W + 4) * (W + 4) * (H + 4) * es;
   const int want = st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
   const size_t wide = want == LAYOUT_DUAL ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
@@ -1034,6 +1036,9 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
   size_t lbytes = 0;
   const int layout = fp_layout(W, H, st, &lbytes);  // PLAIN at least: the state is < 2 GiB
+  if (layout < 0)
+    return fail(MEXCT_EINVAL, "OSC: (W+4)*(H+4) = %lld must stay below 2^23",
+                (long long)(W + 4) * (H + 4));
   RC(c->get(WS_MUZ, lbytes, &mz));
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
   for (int s = 0; s < numSubsets; ++s) {

@@ -100,6 +100,14 @@ __device__ __forceinline__ int cvt_flr(float x) {
   asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
   return i;
 }
+// a * b + c with a, b signed 24-bit and c a full 32-bit addend, one
+// full-rate instruction (the compiler otherwise forms mul24 + add into a
+// 64-bit v_mad_u64_u32, a quarter-rate one); b is wave-uniform
+__device__ __forceinline__ int mad_i24(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
 // median(i, -2, hi): the corner index clamped to [-2, hi]
 __device__ __forceinline__ int clamp_m2(int i, int hi) {
   int r;

@@ -49,7 +49,7 @@ struct FPZArgs {
   const uint32_t* vol_max;   // non-null: the volume holds V * 2^-e (pow2_exp)
   const float2* sc;
   int W, H, rows, NX, NY, NZ;
-  int PY, PZ;
+  int PY, PZ, PYPZ;
   uint32_t off_j, off_i, off_ij;  // byte offsets of the (j+1), (i+1), (i+1,j+1) pairs
   float u0, v0, bx, by, bz, SR, RD;
   int first, stride;  // view = first + j*stride
@@ -183,8 +183,9 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
     j = (int)__builtin_amdgcn_fmed3f(fy, -2.f, fny);
     kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
   }
-  // 24-bit multiplies: |i|, |j| <= N + 2 and PY, PZ < 2^23 (checked by the host)
-  const uint32_t e = (uint32_t)(__mul24(__mul24(i, a.PY) + j, a.PZ) + kk + base);
+  // 24-bit multiplies: |i|, |j| <= N + 2 and PZ, PY * PZ < 2^23 (checked by
+  // the host); the sums are exact in 32 bits (the layout is below 4 GiB)
+  const uint32_t e = (uint32_t)(mad_i24(i, a.PYPZ, mad_i24(j, a.PZ, kk)) + base);
   if constexpr (LAYOUT == LAYOUT_QUAD) {
     quad_ld<S>(r, e * (uint32_t)(4 * sizeof(S)), t);
   } else if constexpr (LAYOUT == LAYOUT_DUAL && __is_same(S, _Float16)) {
@@ -327,7 +328,7 @@ template <typename S, int STEP2, int LAYOUT>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
   const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
-  const int base = 2 * a.PY * a.PZ + 2 * a.PZ + 2;
+  const int base = 2 * a.PYPZ + 2 * a.PZ + 2;
   if (n <= 0) return 0.f;
   if constexpr (sizeof(S) == 2) {  // texel coordinates (see fetch_z)
     cur.x = (cur.x - a.bx) - 0.5f;
@@ -473,6 +474,7 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   const int PX = layout == LAYOUT_PLAIN ? NX + 2 * HALO : NX + 2 * HALO - 1;
   k.PY = layout == LAYOUT_QUAD ? NX + 2 * HALO - 1 : NX + 2 * HALO;
   k.PZ = NZ + 2 * HALO;
+  k.PYPZ = k.PY * k.PZ;
   const uint32_t es = (st_dt == 0 ? 4u : 2u) *
                       (layout == LAYOUT_QUAD ? 4u : layout == LAYOUT_DUAL ? 2u : 1u);
   k.vol_bytes = (uint32_t)((size_t)PX * k.PY * k.PZ * es);
