@@ -276,8 +276,12 @@ Pick choose(mexct_server* sv, Worker& w, std::vector<Req*>& group) {
   if (sv->queue.empty()) return Pick::kWait;
   Req* head = sv->queue.front();
   int64_t idle_reserved = 0;
+  int idle = 0;  // workers free to take work now, this one included
   for (auto& o : sv->workers)
-    if (!o->running) idle_reserved += o->reserved;
+    if (!o->running) {
+      idle_reserved += o->reserved;
+      ++idle;
+    }
   const int64_t others = sv->in_use - w.reserved;
   for (auto it = sv->queue.begin(); it != sv->queue.end(); ++it) {
     Req* q = *it;
@@ -297,11 +301,12 @@ Pick choose(mexct_server* sv, Worker& w, std::vector<Req*>& group) {
     // a batch holds every item's buffers at once; back-to-back runs reuse one set
     const bool batch = batchable(q->r);
     int64_t group_fp = q->footprint;
-    // share the queued requests of this key among all workers, so that a
-    // burst keeps every worker's copies and kernels overlapping
+    // share the queued requests of this key evenly among the idle workers,
+    // so that a burst keeps every one of them copying and computing (busy
+    // workers take theirs when they finish)
     int same = 1;
     for (auto jt = it; jt != sv->queue.end(); ++jt) same += same_key(**jt, *q) ? 1 : 0;
-    const int nw = (int)sv->workers.size();
+    const int nw = std::max(idle, 1);
     const int cap = std::min(sv->max_group, (same + nw - 1) / nw);
     for (auto jt = it; jt != sv->queue.end() && (int)group.size() < cap;) {
       const int64_t more = batch ? group_fp + (*jt)->footprint : group_fp;
