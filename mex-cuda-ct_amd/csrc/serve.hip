@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <memory>
 #include <mutex>
@@ -107,6 +108,12 @@ struct mexct_server {
     Buf a, b, o, x;  // staged input, second input, output, filtered sinogram
     int64_t reserved = 0;  // this worker's share of in_use
     bool running = false;
+    // pinned host chunks for device -> host copies: a copy into pageable
+    // memory runs at about a third of the DMA rate, so outputs go through
+    // these in turn, one chunk copied out on the host while the next is in
+    // flight (allocated at first use, freed when the server is destroyed)
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
   };
   int64_t n_batched = 0, n_batches = 0;  // requests run in multi-item launches, such groups
   int device = 0;
@@ -170,11 +177,83 @@ int copy_in(Worker& w, void* dst, const void* src, size_t bytes) {
   return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
 }
 
-int copy_out(Worker& w, void* dst, const void* src, size_t bytes) {
+constexpr size_t kPinChunk = (size_t)4 << 20;
+
+bool pinned_ready(Worker& w) {
+  if (w.pin[1] && w.ev[1]) return true;
+  for (int k = 0; k < 2; ++k) {
+    if (!w.pin[k] && hipHostMalloc(&w.pin[k], kPinChunk, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      w.pin[k] = nullptr;
+      return false;
+    }
+    if (!w.ev[k] && hipEventCreateWithFlags(&w.ev[k], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      w.ev[k] = nullptr;
+      return false;
+    }
+  }
+  return true;
+}
+
+void free_pinned(Worker& w) {
+  for (int k = 0; k < 2; ++k) {
+    if (w.pin[k]) (void)hipHostFree(w.pin[k]);
+    if (w.ev[k]) (void)hipEventDestroy(w.ev[k]);
+    w.pin[k] = nullptr;
+    w.ev[k] = nullptr;
+  }
+}
+
+// Device -> host copies of n (dst, src, bytes) segments on the worker's
+// stream, in order after the work already on it.  Returns when every
+// output is in host memory.
+struct Seg {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+int copy_out_segs(Worker& w, const Seg* seg, int n) {
   hipStream_t st = mexct::ctx_stream(w.ctx);
-  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipError_t e = hipSuccess;
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += seg[i].bytes;
+  if (total < ((size_t)1 << 20) || !pinned_ready(w)) {
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+      e = hipMemcpyAsync(seg[i].dst, seg[i].src, seg[i].bytes, hipMemcpyDeviceToHost, st);
+  } else {
+    // chunk c goes to pin[c & 1]; chunk c - 1 is copied out of the other
+    // buffer on the host while chunk c is in flight
+    char* prev_dst = nullptr;
+    size_t prev_len = 0;
+    int c = 0;
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+      for (size_t off = 0; off < seg[i].bytes && e == hipSuccess; off += kPinChunk, ++c) {
+        const size_t len = std::min(kPinChunk, seg[i].bytes - off);
+        e = hipMemcpyAsync(w.pin[c & 1], (const char*)seg[i].src + off, len,
+                           hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(w.ev[c & 1], st);
+        if (e == hipSuccess && prev_dst) {
+          e = hipEventSynchronize(w.ev[(c - 1) & 1]);
+          if (e == hipSuccess) memcpy(prev_dst, w.pin[(c - 1) & 1], prev_len);
+        }
+        prev_dst = (char*)seg[i].dst + off;
+        prev_len = len;
+      }
+    }
+    if (e == hipSuccess && prev_dst) {
+      e = hipEventSynchronize(w.ev[(c - 1) & 1]);
+      if (e == hipSuccess) memcpy(prev_dst, w.pin[(c - 1) & 1], prev_len);
+    }
+  }
+  const hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
   return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
+}
+
+int copy_out(Worker& w, void* dst, const void* src, size_t bytes) {
+  const Seg s{dst, src, bytes};
+  return copy_out_segs(w, &s, 1);
 }
 
 #define TRY(x)          \
@@ -261,12 +340,9 @@ int run_batch(Worker& w, const std::vector<Req*>& g) {
     TRY(mexct_bp_batch_dev(w.ctx, bp_in, MEXCT_F32, n, r.W, r.H, r.nProj, r.sad, ang, r.geom,
                            (float*)w.o.p, &o));
   }
-  hipStream_t st = mexct::ctx_stream(w.ctx);
-  hipError_t e = hipSuccess;
-  for (int i = 0; i < n && e == hipSuccess; ++i)
-    e = hipMemcpyAsync(g[i]->r.output, (char*)w.o.p + i * out_b, out_b, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  return e == hipSuccess ? MEXCT_OK : mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
+  std::vector<Seg> segs((size_t)n);
+  for (int i = 0; i < n; ++i) segs[i] = Seg{g[i]->r.output, (char*)w.o.p + i * out_b, out_b};
+  return copy_out_segs(w, segs.data(), n);
 }
 
 enum class Pick { kWait, kRun, kRelease };
@@ -497,6 +573,7 @@ int mexct_server_destroy(mexct_server* sv) {
     if (w->ctx) (void)mexct_ctx_sync(w->ctx);
     for (Buf* b : {&w->a, &w->b, &w->o, &w->x})
       if (b->p) (void)hipFree(b->p);
+    free_pinned(*w);
     mexct_ctx_destroy(w->ctx);
   }
   delete sv;
