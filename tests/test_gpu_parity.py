@@ -144,11 +144,13 @@ def _osc_inputs(W, H, n, seed):
     return mu, pre, post
 
 
+# the second shape: several 32-voxel z-ranges (the cone row start replays
+# the running sum over whole ranges) and tiles cut by the volume edge
+@pytest.mark.parametrize("W,H,n,ns", [(32, 20, 12, 4), (26, 70, 8, 2)])
 @pytest.mark.parametrize("geom", GEOMS)
-def test_osc_iter(gpu, geom):
-    W, H, n, ns = 32, 20, 12, 4
+def test_osc_iter(gpu, geom, W, H, n, ns):
     mu, pre, post = _osc_inputs(W, H, n, 61 + geom)
-    picks = np.array([2, 0, 3, 1], np.int32)
+    picks = np.array([2, 0, 3, 1] if ns == 4 else [1, 0], np.int32)
     ang = _angles(n)
     rmu, rpre, rpost = O.osc_iter(mu, pre, post, ns, picks, ang, 200.0, geom)
     mexct.CUDAmex_oscIter(mu, pre, post, ns, picks, ang, 200.0, geom)
@@ -169,8 +171,11 @@ def test_osc_empty_subset_zeroes_circle(gpu):
     assert np.array_equal(mu, rmu)
 
 
-def test_tvmin(gpu):
-    x = np.asfortranarray(_rng(71).uniform(size=(20, 18, 16)).astype(np.float32))
+# the gradient kernel reduces 64 x 4 x 4 voxels per workgroup: shapes with
+# partial workgroups along every axis, and a depth below one workgroup's 4
+@pytest.mark.parametrize("shape", [(20, 18, 16), (70, 9, 7), (33, 5, 3)])
+def test_tvmin(gpu, shape):
+    x = np.asfortranarray(_rng(71).uniform(size=shape).astype(np.float32))
     got = mexct.CUDAmex_TVmin_3D(x, 0.05)
     ref = O.tvmin(x, 0.05, 20)
     _check(got, ref, "TVmin", 1e-5, 1e-6)
