@@ -270,7 +270,9 @@ int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H
  * requests in arrival order as long as the sum of their device footprints
  * stays within hbm_budget bytes; a worker that takes a request also takes up
  * to max_group-1 queued requests with the same operator, geometry, sizes,
- * angles, SAD and storage, and runs them back to back.  Requests run the
+ * angles, SAD and storage (at most an even share of those among the idle
+ * workers) and runs an FP, BP or FDK group as one batched launch, an OSC-TV
+ * group back to back.  Requests run the
  * same operators as the entry points above, so a result does not depend on
  * what else is in flight.  Arrays are host memory in the MATLAB layout; the
  * caller keeps input/output alive and unchanged until mexct_server_wait
