@@ -637,14 +637,36 @@ hipError_t launch_osc_bp_update(const OSCBPLaunch& a, hipStream_t st) {
 }
 
 namespace {
+__device__ __forceinline__ uint32_t abs_bits(float v, uint32_t m) {
+  const float a = fabsf(v);
+  return a == a ? max(m, __float_as_uint(a)) : m;  // NaN ignored
+}
+
 // Block maxima of |x| combined with one atomicMax per block; non-negative
-// floats order like their bit patterns.
+// floats order like their bit patterns.  The body goes in 16-byte loads from
+// the first 16-byte boundary on; the few elements before it and after the
+// last whole vector are read one by one.
 __global__ __launch_bounds__(256) void absmax_kernel(const float* x, size_t n, uint32_t* out) {
   uint32_t m = 0;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-    const float v = fabsf(x[i]);
-    if (v == v) m = max(m, __float_as_uint(v));
+  const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, nth = (size_t)gridDim.x * 256;
+  const size_t head = min((size_t)(((16u - ((uintptr_t)x & 15u)) & 15u) / 4u), n);
+  if (tid < head) m = abs_bits(x[tid], m);
+  const float4* v = (const float4*)(x + head);
+  const size_t nv = (n - head) / 4;
+  size_t i = tid;
+  for (; i + 3 * nth < nv; i += 4 * nth) {  // four loads in flight
+    const float4 q0 = v[i], q1 = v[i + nth], q2 = v[i + 2 * nth], q3 = v[i + 3 * nth];
+    m = abs_bits(q0.w, abs_bits(q0.z, abs_bits(q0.y, abs_bits(q0.x, m))));
+    m = abs_bits(q1.w, abs_bits(q1.z, abs_bits(q1.y, abs_bits(q1.x, m))));
+    m = abs_bits(q2.w, abs_bits(q2.z, abs_bits(q2.y, abs_bits(q2.x, m))));
+    m = abs_bits(q3.w, abs_bits(q3.z, abs_bits(q3.y, abs_bits(q3.x, m))));
   }
+  for (; i < nv; i += nth) {
+    const float4 q = v[i];
+    m = abs_bits(q.w, abs_bits(q.z, abs_bits(q.y, abs_bits(q.x, m))));
+  }
+  const size_t t0 = head + nv * 4;
+  if (tid < n - t0) m = abs_bits(x[t0 + tid], m);
   for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
   __shared__ uint32_t wm[4];
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
@@ -655,7 +677,8 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float* x, size_t n, u
 
 hipError_t launch_absmax(const float* x, size_t n, uint32_t* out_bits, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+  // few enough workgroups that their one atomic each does not queue up
+  const unsigned g = (unsigned)std::min<size_t>((n / 4 + 255) / 256 + 1, 1024);
   absmax_kernel<<<g, 256, 0, st>>>(x, n, out_bits);
   return hipGetLastError();
 }

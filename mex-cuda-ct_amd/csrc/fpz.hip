@@ -324,7 +324,51 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
   return dval;
 }
 
-template <typename S, int STEP2, int LAYOUT>
+// The plain sample loop with the loads of the next two samples in flight
+// while one is interpolated (three register sets in rotation), for launches
+// too small to hide the gather latency with other waves (an OSC subset of a
+// few views).  The same samples are added in the same order as march_loop's;
+// fetches up to two samples past the end read in-range or range-checked
+// buffer offsets and are discarded.
+template <typename S, int STEP2, int LAYOUT, bool CLAMP>
+__device__ __forceinline__ float march_loop3(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
+                                             const R3& inc, int n, float fnx, float fny,
+                                             float fnz, int base) {
+  constexpr float step = STEP2 ? 0.5f : 1.0f;
+  float dval = 0.f;
+  Tap8<S> ta = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+  cur.x = cur.x + inc.x;
+  cur.y = cur.y + inc.y;
+  cur.z = cur.z + inc.z;
+  Tap8<S> tb = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+  int k = 0;
+  for (; k + 2 < n; k += 3) {
+    cur.x = cur.x + inc.x;
+    cur.y = cur.y + inc.y;
+    cur.z = cur.z + inc.z;
+    const Tap8<S> tc = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    __builtin_amdgcn_sched_barrier(0);
+    dval = dval + interp_z<S, LAYOUT>(ta) * step;
+    cur.x = cur.x + inc.x;
+    cur.y = cur.y + inc.y;
+    cur.z = cur.z + inc.z;
+    ta = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    __builtin_amdgcn_sched_barrier(0);
+    dval = dval + interp_z<S, LAYOUT>(tb) * step;
+    cur.x = cur.x + inc.x;
+    cur.y = cur.y + inc.y;
+    cur.z = cur.z + inc.z;
+    tb = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    __builtin_amdgcn_sched_barrier(0);
+    dval = dval + interp_z<S, LAYOUT>(tc) * step;
+  }
+  if (k < n) dval = dval + interp_z<S, LAYOUT>(ta) * step;
+  if (k + 1 < n) dval = dval + interp_z<S, LAYOUT>(tb) * step;
+  return dval;
+}
+
+// DEEP: march_loop3 for the plain loops (small launches)
+template <typename S, int STEP2, int LAYOUT, bool DEEP = false>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
   const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
@@ -341,9 +385,13 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     if (comp)
       return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
                    : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
+    if constexpr (DEEP)
+      return clamp ? march_loop3<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
+                   : march_loop3<S, STEP2, LAYOUT, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
     return clamp ? march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
                  : march_loop<S, STEP2, LAYOUT, false, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
   }
+  if constexpr (DEEP) return march_loop3<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
   return march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
 }
 
@@ -430,7 +478,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
   const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
       (void*)((const char*)a.vol + item * a.vol_stride), 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = ldexpf(march_z<S, STEP2, LAYOUT>(r, a, cur, inc, n),
+  const float dval = ldexpf(march_z<S, STEP2, LAYOUT, EPI == 1>(r, a, cur, inc, n),
                             pow2_exp(a.vol_max ? a.vol_max + item : nullptr));
   if (EPI == 0) {
     out[p] = dval;
