@@ -131,14 +131,21 @@ def main():
     import oracle as O  # checker: sample-count estimate for the FP gather rate
 
     def probe_ms(pattern, storage, reps=3):
-        ctx.probe_gather(storage, W, H, n, pattern)  # warm
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            ns = ctx.probe_gather(storage, W, H, n, pattern)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps, ns
+        """Best time of the probe over its unroll depths 1, 2, 4 and 8."""
+        best = None
+        for k in range(4):
+            pk = pattern | (k << 4)
+            ctx.probe_gather(storage, W, H, n, pk)  # warm
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                ns = ctx.probe_gather(storage, W, H, n, pk)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            if best is None or ms < best[0]:
+                best = (ms, ns)
+        return best
 
     views8 = np.linspace(0, n - 1, 8).astype(int)
     S_est = float(np.mean([O.fp(vol_np, SAD, ang[v:v + 1], 1, 2, return_samples=True)[1]

@@ -260,6 +260,9 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
  * (BP) the launch performs. */
 #define MEXCT_PROBE_FP 0
 #define MEXCT_PROBE_BP 1
+/* pattern | MEXCT_PROBE_UNROLL(k): the probe loop unrolled 2^k times (k = 0..3,
+ * up to 2^k samples' loads in flight per wave); the ceiling is the best one */
+#define MEXCT_PROBE_UNROLL(k) ((k) << 4)
 int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H, int32_t nviews,
                            int32_t pattern, int64_t* samples);
 

@@ -1456,17 +1456,17 @@ int mexct_probe_gather_dev(mexct_ctx* c, int32_t storage, int64_t W, int64_t H, 
                            int32_t pattern, int64_t* samples) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_dtype(storage, "storage"));
-  if (W < 2 || H < 1 || nviews < 1 || (pattern != MEXCT_PROBE_FP && pattern != MEXCT_PROBE_BP) ||
-      !samples)
+  if (W < 2 || H < 1 || nviews < 1 || (pattern & ~0x31) != 0 || !samples)
     return fail(MEXCT_EINVAL, "bad probe arguments");
   const size_t es = (size_t)store_bytes(storage);
-  const size_t bytes = pattern != MEXCT_PROBE_FP ? (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews
+  const int kind = pattern & 1;
+  const size_t bytes = kind != MEXCT_PROBE_FP ? (size_t)(W + 4) * (H + 3) * 2 * es * (size_t)nviews
                        : storage == MEXCT_F32  ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
                                                : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
   if (bytes >= ((size_t)1 << 32)) return fail(MEXCT_EINVAL, "probe buffer too large");
   DevGuard g(c->device);
   void *buf, *sink;
-  RC(c->get(pattern == MEXCT_PROBE_FP ? WS_VOL : WS_SINO, bytes, &buf));
+  RC(c->get(kind == MEXCT_PROBE_FP ? WS_VOL : WS_SINO, bytes, &buf));
   RC(c->get(WS_SCALARS, 4096, &sink));
   HIPCHK(launch_gather_probe(buf, storage, pattern, (int)W, (int)H, nviews, (float*)sink, samples,
                              c->stream));

@@ -324,51 +324,53 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
   return dval;
 }
 
-// The plain sample loop with the loads of the next two samples in flight
-// while one is interpolated (three register sets in rotation), for launches
-// too small to hide the gather latency with other waves (an OSC subset of a
-// few views).  The same samples are added in the same order as march_loop's;
-// fetches up to two samples past the end read in-range or range-checked
-// buffer offsets and are discarded.
-template <typename S, int STEP2, int LAYOUT, bool CLAMP>
-__device__ __forceinline__ float march_loop3(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
+// The plain sample loop with the loads of the next D samples in flight
+// while one is interpolated (D + 1 register sets in rotation): the same
+// samples added in the same order as march_loop's.  Fetches up to D samples
+// past the end read range-checked buffer offsets (or, for the plain layout,
+// clamped indices) and are discarded.
+template <typename S, int STEP2, int LAYOUT, bool CLAMP, int D>
+__device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                              const R3& inc, int n, float fnx, float fny,
                                              float fnz, int base) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
   float dval = 0.f;
-  Tap8<S> ta = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
-  cur.x = cur.x + inc.x;
-  cur.y = cur.y + inc.y;
-  cur.z = cur.z + inc.z;
-  Tap8<S> tb = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
-  int k = 0;
-  for (; k + 2 < n; k += 3) {
+  Tap8<S> t[D + 1];  // t[u] holds sample k + u at the top of each trip
+  t[0] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+#pragma unroll
+  for (int d = 1; d < D; ++d) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    const Tap8<S> tc = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
-    __builtin_amdgcn_sched_barrier(0);
-    dval = dval + interp_z<S, LAYOUT>(ta) * step;
-    cur.x = cur.x + inc.x;
-    cur.y = cur.y + inc.y;
-    cur.z = cur.z + inc.z;
-    ta = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
-    __builtin_amdgcn_sched_barrier(0);
-    dval = dval + interp_z<S, LAYOUT>(tb) * step;
-    cur.x = cur.x + inc.x;
-    cur.y = cur.y + inc.y;
-    cur.z = cur.z + inc.z;
-    tb = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
-    __builtin_amdgcn_sched_barrier(0);
-    dval = dval + interp_z<S, LAYOUT>(tc) * step;
+    t[d] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
   }
-  if (k < n) dval = dval + interp_z<S, LAYOUT>(ta) * step;
-  if (k + 1 < n) dval = dval + interp_z<S, LAYOUT>(tb) * step;
+  int k = 0;
+  for (; k + D < n; k += D + 1) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      cur.x = cur.x + inc.x;
+      cur.y = cur.y + inc.y;
+      cur.z = cur.z + inc.z;
+      t[(u + D) % (D + 1)] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+      __builtin_amdgcn_sched_barrier(0);  // sample k+u+D's loads stay ahead
+      dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (k + u < n) dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
   return dval;
 }
 
-// DEEP: march_loop3 for the plain loops (small launches)
-template <typename S, int STEP2, int LAYOUT, bool DEEP = false>
+// Samples in flight in the plain loop.  At 384^3 x 512 cone, f32: 32.3 ms
+// with 1, 31.5 with 2, 30.9 with 3, 29.9 with 5, 29.5 with 7 and 11 (119 and
+// 179 VGPRs); f16 19.8 ms with 1, 19.1-19.3 with 3 and 5.  An OSC subset's
+// projection (a few views, ~6 waves per SIMD) keeps 3 so that every wave
+// stays resident (68 VGPRs).
+template <typename S, int EPI>
+constexpr int fetch_ahead() { return EPI == 1 || sizeof(S) == 2 ? 3 : 7; }
+
+template <typename S, int STEP2, int LAYOUT, int EPI>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                          const R3& inc, int n) {
   const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
@@ -381,18 +383,19 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     // one loop per wave: compensated when any of its rays is long; without
     // the index clamps when every ray stays inside the halo
     const bool comp = __builtin_amdgcn_ballot_w64(n > a.long_ray) != 0;
-    const bool clamp = __builtin_amdgcn_ballot_w64(!in_halo(cur, inc, n, fnx, fny, fnz)) != 0;
+    // the plain layout reads its neighbours through scalar offsets, which the
+    // buffer range check does not cover: its indices are always clamped
+    const bool clamp = LAYOUT == LAYOUT_PLAIN ||
+                       __builtin_amdgcn_ballot_w64(!in_halo(cur, inc, n, fnx, fny, fnz)) != 0;
     if (comp)
       return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
                    : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
-    if constexpr (DEEP)
-      return clamp ? march_loop3<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
-                   : march_loop3<S, STEP2, LAYOUT, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
-    return clamp ? march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
-                 : march_loop<S, STEP2, LAYOUT, false, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
+    return clamp ? march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny,
+                                                                     fnz, base)
+                 : march_loopd<S, STEP2, LAYOUT, false, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny,
+                                                                      fnz, base);
   }
-  if constexpr (DEEP) return march_loop3<S, STEP2, LAYOUT, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
-  return march_loop<S, STEP2, LAYOUT, false, true>(r, a, cur, inc, n, fnx, fny, fnz, base);
+  return march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny, fnz, base);
 }
 
 // Storage of the OSC ratio terms for a volume storage S.  They span the
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
   const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
       (void*)((const char*)a.vol + item * a.vol_stride), 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = ldexpf(march_z<S, STEP2, LAYOUT, EPI == 1>(r, a, cur, inc, n),
+  const float dval = ldexpf(march_z<S, STEP2, LAYOUT, EPI>(r, a, cur, inc, n),
                             pow2_exp(a.vol_max ? a.vol_max + item : nullptr));
   if (EPI == 0) {
     out[p] = dval;

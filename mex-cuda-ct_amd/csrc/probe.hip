@@ -5,6 +5,9 @@
 // the memory pipeline's ceiling for that pattern (the "measured gather
 // roofline"):
 //
+// Each probe runs unrolled 1, 2, 4 or 8 times (up to that many samples'
+// loads in flight per wave); the measured ceiling is the best of them.
+//
 //   FP: f32, the x-pair (DUAL) z-fastest volume, per sample two 16-byte
 //       loads (rows j and j+1); f16/bf16, the corner-quad volume, one
 //       16-byte load per sample.  The 64 lanes of a wave hold 64
@@ -37,7 +40,7 @@ __device__ __forceinline__ float probe_ld(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // FP pattern.  Element (x, y, z) at ((x+2)*QY + y+2)*QZ + z+2, E bytes each.
 // QUAD: one 16-byte load per sample; DUAL: two 16-byte loads, rows y and
 // y+1 (QZ*E apart).
-template <int QUAD>
+template <int QUAD, int U>
 __global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t bytes,
                                                        uint32_t E, int QY, int QZ, int ncols,
                                                        int nz, int nsteps, float* sink) {
@@ -49,6 +52,8 @@ __global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t
   const uint32_t step = (uint32_t)QY * (uint32_t)QZ * E;
   const uint32_t oj = (uint32_t)QZ * E;
   float s = 0.f;
+  // unrolled U times: up to U samples' loads in flight per wave
+#pragma unroll U
   for (int k = 0; k < nsteps; ++k) {
     s = s + probe_ld<16>(r, voff);
     if (!QUAD) s = s + probe_ld<16>(r, voff + oj);
@@ -58,7 +63,7 @@ __global__ __launch_bounds__(256) void fp_probe_kernel(const void* vol, uint32_t
 }
 
 // BP pattern.  Pair element (u, row) of view v at v*VB + (row+2)*RB + (u+2)*E2.
-template <int LB>
+template <int LB, int U>
 __global__ __launch_bounds__(256) void bp_probe_kernel(const void* sino, uint32_t bytes,
                                                        uint32_t E2, uint32_t RB, uint32_t VB,
                                                        int W, int H, int nviews, int nsteps,
@@ -71,6 +76,7 @@ __global__ __launch_bounds__(256) void bp_probe_kernel(const void* sino, uint32_
       __builtin_amdgcn_make_buffer_rsrc((void*)sino, 0, (int)bytes, 0x00020000);
   const uint32_t base = (uint32_t)view * VB + 2u * RB + (uint32_t)(u + 2) * E2;
   float s = 0.f;
+#pragma unroll U
   for (int k = 0; k < nsteps; ++k) {
     s = s + probe_ld<LB>(r, base + (uint32_t)row * RB);
     row = row + 1 == H ? 0 : row + 1;
@@ -78,10 +84,11 @@ __global__ __launch_bounds__(256) void bp_probe_kernel(const void* sino, uint32_
   if (s == 1234.5f) sink[threadIdx.x] = s;
 }
 
-hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, int H, int nviews,
-                               float* sink, int64_t* samples, hipStream_t st) {
+template <int U>
+static void probe_launch(const void* buf, int st_dt, int kind, int W, int H, int nviews,
+                         float* sink, int64_t* samples, hipStream_t st) {
   const uint32_t es = st_dt == 0 ? 4u : 2u;
-  if (pattern == 0) {
+  if (kind == 0) {
     const int quad = st_dt != 0;
     const int QX = W + 2 * HALO - 1;
     const int QY = W + 2 * HALO - (quad ? 1 : 0), QZ = H + 2 * HALO;
@@ -89,9 +96,9 @@ hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, i
     const uint32_t bytes = (uint32_t)((size_t)QX * QY * QZ * E);
     const dim3 g((unsigned)((W + 3) / 4), (unsigned)((H + 63) / 64 * nviews));
     if (quad)
-      fp_probe_kernel<1><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
+      fp_probe_kernel<1, U><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
     else
-      fp_probe_kernel<0><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
+      fp_probe_kernel<0, U><<<g, 256, 0, st>>>(buf, bytes, E, QY, QZ, W, H, W, sink);
     *samples = (int64_t)g.x * 4 * (int64_t)g.y * 64 * W;
   } else {
     const uint32_t E2 = 2u * es, RB = (uint32_t)(W + 2 * HALO) * E2;
@@ -101,10 +108,22 @@ hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, i
     const unsigned groups = (unsigned)((size_t)W * W / ((size_t)4 * 64 * gx));
     const dim3 g(gx, (unsigned)nviews * (groups > 0 ? groups : 1));
     if (st_dt == 0)
-      bp_probe_kernel<16><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
+      bp_probe_kernel<16, U><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
     else
-      bp_probe_kernel<8><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
+      bp_probe_kernel<8, U><<<g, 256, 0, st>>>(buf, bytes, E2, RB, VB, W, H, nviews, H, sink);
     *samples = (int64_t)g.x * (int64_t)g.y * 256 * H;
+  }
+}
+
+// pattern bit 0: FP (0) or BP (1) loads; bits 4-5: loop unrolled 1, 2, 4 or 8 times
+hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, int H, int nviews,
+                               float* sink, int64_t* samples, hipStream_t st) {
+  const int kind = pattern & 1;
+  switch ((pattern >> 4) & 3) {
+    case 0: probe_launch<1>(buf, st_dt, kind, W, H, nviews, sink, samples, st); break;
+    case 1: probe_launch<2>(buf, st_dt, kind, W, H, nviews, sink, samples, st); break;
+    case 2: probe_launch<4>(buf, st_dt, kind, W, H, nviews, sink, samples, st); break;
+    default: probe_launch<8>(buf, st_dt, kind, W, H, nviews, sink, samples, st); break;
   }
   return hipGetLastError();
 }

@@ -99,8 +99,9 @@ class Context:
 
     def probe_gather(self, storage, W, H, nviews, pattern=0):
         """Gather microbenchmark launch (see mexct_probe_gather_dev): pattern
-        0 is the forward projector's loads, 1 the backprojector's; returns the
-        sample (update) count of the launch."""
+        0 is the forward projector's loads, 1 the backprojector's, plus k << 4
+        for the loop unrolled 2^k times (k = 0..3); returns the sample
+        (update) count of the launch."""
         n = C.c_int64()
         self._call(lib().mexct_probe_gather_dev, self._h, int(storage), int(W), int(H),
                    int(nviews), int(pattern), C.byref(n))

@@ -79,3 +79,19 @@ def test_tv_flat_volume_is_nan_like_reference(gpu):
     got = mexct.CUDAmex_TVmin_3D(x, 0.1, steps=1)
     ref = O.tvmin(x, 0.1, 1)
     assert np.array_equal(np.isnan(got), np.isnan(ref))
+
+
+def test_gather_probe_unroll_variants(gpu):
+    # the measured gather ceiling runs each probe unrolled 1, 2, 4 and 8 times:
+    # every variant performs the same loads; other pattern bits are rejected
+    from mexct import device
+
+    ctx = device.Context(0)
+    for storage in (mexct.F32, mexct.F16):
+        for kind in (0, 1):
+            counts = {ctx.probe_gather(storage, 40, 36, 6, kind | (k << 4)) for k in range(4)}
+            assert len(counts) == 1 and counts.pop() > 0
+    with pytest.raises(mexct.MexctError):
+        ctx.probe_gather(mexct.F32, 40, 36, 6, 0x40)
+    ctx.sync()
+    ctx.close()
