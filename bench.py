@@ -25,7 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "mex-cuda-ct_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker (cpu_baseline, FP64 sample)
 
 HBM_PEAK_GBS = 8000.0
-FP_TRAFFIC_BYTES = 184.5e9  # measured, cone 384^3 x 512 f32 FP launch
+FP_TRAFFIC_BYTES = 134.5e9  # measured, cone 384^3 x 512 f32 FP launch
 
 
 def log(*a):
