@@ -225,6 +225,13 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
 template <typename S, int LAYOUT>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
+  if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD && kPackedF16Lerp) {
+    // x lerps of both y rows at once in packed f16, then y and z in float
+    const uint32_t ax2 = pk_h2(t.ax);
+    const float c0 = lerp_x(pk_lerp_h(t.w0, t.w1, ax2), t.ay);  // {c00, c10} at z
+    const float c1 = lerp_x(pk_lerp_h(t.w2, t.w3, ax2), t.ay);  // at z+1
+    return lerp(c0, c1, t.az);
+  }
   if constexpr (__is_same(S, _Float16)) {
     if (LAYOUT == LAYOUT_DUAL) {
       c00 = lerp_x(t.w0, t.ax);
