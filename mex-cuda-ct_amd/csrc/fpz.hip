@@ -74,21 +74,17 @@ __device__ __forceinline__ float2 buf_ld2<float>(__amdgpu_buffer_rsrc_t r, uint3
   f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
   return make_float2(v.x, v.y);
 }
-template <>
-__device__ __forceinline__ float2 buf_ld2<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                                  uint32_t soff) {
-  const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
-}
 
-// Corner pairs and weights of one sample (f32 and bf16 taps, widened).
+// Corner pairs and weights of one sample (f32 taps).
 template <typename S>
 struct Tap8 {
   float2 p00, p01, p10, p11;
   float ax, ay, az;
 };
-// f16 taps stay packed, two halves per dword: the first lerp level reads
-// them through mixed-precision FMAs.
+// 2-byte taps stay packed, two halves per dword, until the interpolation
+// (so that the loads of the samples ahead stay in flight): the f16 first lerp
+// level reads them through packed or mixed-precision FMAs, bf16 halves are
+// widened by a shift or a mask.
 //   plain layout: w0..w3 = (z, z+1) pairs of (i,j), (i,j+1), (i+1,j), (i+1,j+1)
 //   quad layout:  w0 = {(i,j), (i,j+1)} and w1 = {(i+1,j), (i+1,j+1)} at z,
 //                 w2, w3 the same at z+1
@@ -96,6 +92,11 @@ struct Tap8 {
 //                 the same for row j+1
 template <>
 struct Tap8<_Float16> {
+  uint32_t w0, w1, w2, w3;
+  float ax, ay, az;
+};
+template <>
+struct Tap8<bf16_t> {
   uint32_t w0, w1, w2, w3;
   float ax, ay, az;
 };
@@ -132,10 +133,10 @@ __device__ __forceinline__ void quad_ld<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32
   typedef uint32_t u4v __attribute__((ext_vector_type(4)));
   const u4v q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
   // q.x = {(i,j), (i,j+1)}, q.y = {(i+1,j), (i+1,j+1)} at k; q.z, q.w at k+1
-  t.p00 = make_float2(__uint_as_float(q.x << 16), __uint_as_float(q.z << 16));
-  t.p01 = make_float2(__uint_as_float(q.x & 0xffff0000u), __uint_as_float(q.z & 0xffff0000u));
-  t.p10 = make_float2(__uint_as_float(q.y << 16), __uint_as_float(q.w << 16));
-  t.p11 = make_float2(__uint_as_float(q.y & 0xffff0000u), __uint_as_float(q.w & 0xffff0000u));
+  t.w0 = q.x;
+  t.w1 = q.y;
+  t.w2 = q.z;
+  t.w3 = q.w;
 }
 
 // Four taps {(i,j,k), (i+1,j,k), (i,j,k+1), (i+1,j,k+1)} of one f32 DUAL row.
@@ -207,7 +208,7 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
     t.p11 = make_float2(q1.y, q1.w);
   } else {
     const uint32_t voff = e * (uint32_t)sizeof(S);
-    if constexpr (__is_same(S, _Float16)) {
+    if constexpr (sizeof(S) == 2) {
       t.w0 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
       t.w1 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, a.off_j, 0);
       t.w2 = __builtin_amdgcn_raw_buffer_load_b32(r, voff, a.off_i, 0);
@@ -222,17 +223,31 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
   return t;
 }
 
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
 template <typename S, int LAYOUT>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
-  if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD && kPackedF16Lerp) {
+  if constexpr (__is_same(S, bf16_t)) {
+    if constexpr (LAYOUT == LAYOUT_QUAD) {  // w0 {(i,j), (i,j+1)}, w1 {(i+1,j), (i+1,j+1)}; w2, w3 at z+1
+      c00 = lerp(bf_lo(t.w0), bf_lo(t.w1), t.ax);
+      c10 = lerp(bf_hi(t.w0), bf_hi(t.w1), t.ax);
+      c01 = lerp(bf_lo(t.w2), bf_lo(t.w3), t.ax);
+      c11 = lerp(bf_hi(t.w2), bf_hi(t.w3), t.ax);
+    } else {  // w0..w3: (z, z+1) of (i,j), (i,j+1), (i+1,j), (i+1,j+1)
+      c00 = lerp(bf_lo(t.w0), bf_lo(t.w2), t.ax);
+      c10 = lerp(bf_lo(t.w1), bf_lo(t.w3), t.ax);
+      c01 = lerp(bf_hi(t.w0), bf_hi(t.w2), t.ax);
+      c11 = lerp(bf_hi(t.w1), bf_hi(t.w3), t.ax);
+    }
+  } else if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD && kPackedF16Lerp) {
     // x lerps of both y rows at once in packed f16, then y and z in float
     const uint32_t ax2 = pk_h2(t.ax);
     const float c0 = lerp_x(pk_lerp_h(t.w0, t.w1, ax2), t.ay);  // {c00, c10} at z
     const float c1 = lerp_x(pk_lerp_h(t.w2, t.w3, ax2), t.ay);  // at z+1
     return lerp(c0, c1, t.az);
-  }
-  if constexpr (__is_same(S, _Float16)) {
+  } else if constexpr (__is_same(S, _Float16)) {
     if (LAYOUT == LAYOUT_DUAL) {
       c00 = lerp_x(t.w0, t.ax);
       c01 = lerp_x(t.w1, t.ax);
