@@ -507,7 +507,12 @@ __device__ __forceinline__ float4 osc_ld4<bf16_t>(const bf16_t* p) {
                      __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
 }
 
-constexpr int OSC_ZR = 32;
+// z-voxels per thread.  A subset's launch is small (a few views), so more
+// and lighter waves win: at 256^3 x 60, 10 subsets, OSC-TV took 120 ms with
+// 32 (89 VGPRs, 5 waves per SIMD), 112 ms with 16 and 110 ms with 8 (the
+// cone row start replays more of the running sum per voxel, but every wave
+// stays resident).
+constexpr int OSC_ZR = 8;
 
 template <int GEOM, typename S>
 __global__ __launch_bounds__(64) void osc_bp_update_kernel(OSCBPKArgs a) {
