@@ -144,8 +144,9 @@ def _osc_inputs(W, H, n, seed):
     return mu, pre, post
 
 
-# the second shape: several 32-voxel z-ranges (the cone row start replays
-# the running sum over whole ranges) and tiles cut by the volume edge
+# the second shape: many z-ranges of the backprojection (the cone row start
+# replays the running sum over whole ranges), a last range cut short, and
+# tiles cut by the volume edge
 @pytest.mark.parametrize("W,H,n,ns", [(32, 20, 12, 4), (26, 70, 8, 2)])
 @pytest.mark.parametrize("geom", GEOMS)
 def test_osc_iter(gpu, geom, W, H, n, ns):
