@@ -387,10 +387,11 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
 // Samples in flight in the plain loop.  At 384^3 x 512 cone, f32: 32.3 ms
 // with 1, 31.5 with 2, 30.9 with 3, 29.9 with 5, 29.5 with 7 and 11 (119 and
 // 179 VGPRs); f16 19.8 ms with 1, 19.1-19.3 with 3 and 5.  An OSC subset's
-// projection (a few views, ~6 waves per SIMD) keeps 3 so that every wave
-// stays resident (68 VGPRs).
+// projection (6 views at 256^2 x 256: 6 waves per SIMD) keeps 4, the most
+// that lets all 6 stay resident (80 VGPRs f32): 307 us per subset with 2,
+// 279 with 3, ~240 with 4.
 template <typename S, int EPI>
-constexpr int fetch_ahead() { return EPI == 1 || sizeof(S) == 2 ? 3 : 7; }
+constexpr int fetch_ahead() { return EPI == 1 ? 4 : sizeof(S) == 2 ? 3 : 7; }
 
 template <typename S, int STEP2, int LAYOUT, int EPI>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
