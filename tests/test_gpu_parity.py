@@ -182,6 +182,19 @@ def test_tvmin(gpu, shape):
     _check(got, ref, "TVmin", 1e-5, 1e-6)
 
 
+# The steps of one call keep their state (gradient, norm, step constant) on
+# the device: the result must be the bits of the same steps taken one call at
+# a time, for odd and even counts.
+@pytest.mark.parametrize("steps", [2, 3, 6])
+def test_tvmin_steps_equal_single_steps(gpu, steps):
+    x = np.asfortranarray(_rng(72).uniform(size=(70, 9, 7)).astype(np.float32))
+    fused = mexct.CUDAmex_TVmin_3D(x, 0.05, steps=steps)
+    one = x
+    for _ in range(steps):
+        one = mexct.CUDAmex_TVmin_3D(one, 0.05, steps=1)
+    assert np.array_equal(fused, one)
+
+
 @pytest.mark.parametrize("geom", ["par3d", "fan3d", "cone3d"])
 @pytest.mark.parametrize("window", ["ramp", "hamming", "shepp-logan"])
 def test_prefilter(gpu, geom, window):
