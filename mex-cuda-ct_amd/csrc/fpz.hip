@@ -276,8 +276,9 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
 }
 
 // One ray through the z-fastest volume, software-pipelined: the loads of
-// sample k+1 are issued before sample k is interpolated.  A fetch past the
-// last sample reads a clamped, in-range corner and is discarded.
+// the samples ahead (k+1 in march_loop, k+1 .. k+D in march_loopd) are
+// issued before sample k is interpolated.  Fetches past the last sample read
+// clamped or range-checked offsets and are discarded.
 // Whether every sample cur + k inc (k < n, texel coordinates, running sum)
 // has floor in [-2, N] on each axis, the range the 2-texel halo serves.  The
 // samples lie on the segment between the first and the last, each off by
@@ -296,11 +297,12 @@ __device__ __forceinline__ bool in_halo(const R3& c, const R3& inc, int n, float
          fminf(c.z, ez) - d >= -1.5f && fmaxf(c.z, ez) + d <= fnz + 0.5f;
 }
 
-// The sample loop.  COMP (2-byte storage, long rays): each pair of samples
-// is added to the ray sum with Kahan's compensated summation, so the sum's
-// rounding error stays near one ulp however many samples the ray has
-// instead of growing with their count.  Otherwise the samples are added one
-// by one in ray order, as the reference does.
+// The sample loop of the compensated waves (COMP: 2-byte storage, long
+// rays): each pair of samples is added to the ray sum with Kahan's
+// compensated summation, so the sum's rounding error stays near one ulp
+// however many samples the ray has instead of growing with their count.
+// Without COMP the samples are added one by one in ray order, as the
+// reference does (march_loopd is that loop with more samples ahead).
 template <typename S, int STEP2, int LAYOUT, bool COMP, bool CLAMP>
 __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                             const R3& inc, int n, float fnx, float fny, float fnz,
