@@ -83,6 +83,27 @@ def lib():
     return _lib
 
 
+class use_library:
+    """Context manager: every wrapper below calls the oracle built at `path`
+    (same exports) instead of liboracle.so.  The tests use it to run a pin
+    against a copy of the oracle with a deliberate mistake planted in it."""
+
+    def __init__(self, path):
+        self.path = path
+        self.saved = None
+
+    def __enter__(self):
+        global _lib, _LIB_PATH
+        self.saved = (_lib, _LIB_PATH)
+        _lib, _LIB_PATH = None, self.path
+        return self
+
+    def __exit__(self, *exc):
+        global _lib, _LIB_PATH
+        _lib, _LIB_PATH = self.saved
+        return False
+
+
 def _f32(a):
     return np.asfortranarray(a, dtype=np.float32)
 

@@ -240,6 +240,30 @@ def test_g3d_bp_axis_ray_weights():
     assert col.sum() == pytest.approx(p * 0.5 * (2 * N - 1), rel=1e-6)
 
 
+def test_g3d_bp_quarter_voxel_z_profile():
+    # One ray along -x at y on a voxel centre (interpYY = 1) and z a quarter
+    # voxel above a centre: kL = floor(z - bz) = 3, kU = 4, interpZ =
+    # z - bz - kL + 0.5 = 0.75, interpZZ = 0.25.  It enters at x = M/2
+    # exactly (SAD-free endpoints 12 and -4, alpha 1/2) and samples
+    # X = x - bx = 8, 7.5, ..., 0.5.  A sample adds p*0.5*interpXX*interpZZ
+    # to (iL, jL, kL) and p*0.5*interpXX*interpZ to (iL, jL, kU)
+    # (CUDAmex_general3D_BP.cu:147-170), and is skipped when ceil(X) >= M
+    # (X = 8 and 7.5).  interpXX = X - floor(X) + 0.5 per sample gives the
+    # cell totals S = [1.0, 1.5 x 6, 0.5] along x.
+    M, N, p = 8, 8, 3.0
+    jL, kL = 5, 3
+    y = jL + 0.5 - M / 2.0
+    z = kL + 0.25 - N / 2.0
+    pts = np.zeros((1, 1, 6), np.float32, order="F")
+    pts[0, 0] = [12.0, y, z, -4.0, y, z]
+    out = O.g3d_bp(np.full((1, 1, 1), p, np.float32, order="F"), np.zeros(1, np.float32), pts, M, N)
+    S = np.array([1.0, 1.5, 1.5, 1.5, 1.5, 1.5, 1.5, 0.5])
+    want = np.zeros((M, M, N))
+    want[:, jL, kL] = p * 0.5 * 0.25 * S
+    want[:, jL, kL + 1] = p * 0.5 * 0.75 * S
+    np.testing.assert_array_equal(out, want.astype(np.float32))
+
+
 # ------------------------------------------------------------------ OSC
 def _osc_setup(W=16, H=8, n=8, seed=17):
     r = rng(seed)
@@ -284,6 +308,175 @@ def test_osc_outside_circle_untouched_and_views_once():
     assert (pr2 != pre).all() and (po2 != post).all()  # every view rewritten once
 
 
+_libm = None
+
+
+def _expf(x):
+    """glibc expf, element by element (the oracle's exp of a float)."""
+    global _libm
+    if _libm is None:
+        import ctypes
+
+        _libm = ctypes.CDLL("libm.so.6")
+        _libm.expf.restype = ctypes.c_float
+        _libm.expf.argtypes = [ctypes.c_float]
+    x = np.asarray(x, np.float32)
+    return np.array([_libm.expf(float(v)) for v in x.ravel()], np.float32).reshape(x.shape)
+
+
+def _std_geometry_pts(geom, W, H, sad):
+    """The standard geometries' rays as general-3D pts [W,H,6] (beta = 0):
+    source SAD along +x (on the axis for fan and cone, beside each detector
+    pixel for parallel), flat detector SAD behind the axis, pixel (ix, iy)
+    at u = ix - u0, v = iy - v0 times the magnification 2 across the fan
+    (fan: u only; cone: u and v), u0 = (W-1)/2, v0 = (H-1)/2.  float32
+    arithmetic in the order of the geometry definition
+    (CUDAmex_oscIter.cu:150-190, CUDAmex_FP.cu:64-77,162-175,259-272)."""
+    f = np.float32
+    SR = RD = f(sad)
+    u0, v0 = f(W - 1) / f(2), f(H - 1) / f(2)
+    ix = np.arange(W, dtype=f)[:, None] * np.ones((1, H), f)
+    iy = np.ones((W, 1), f) * np.arange(H, dtype=f)[None, :]
+    zero = np.zeros((W, H), f)
+    if geom == 0:
+        u, v = ix - u0, iy - v0
+        cols = (SR + zero, u, v, -RD + zero, u, v)
+    elif geom == 1:
+        u, v = (ix - u0) * (SR + RD) / SR, iy - v0
+        cols = (SR + zero, zero, v, -RD + zero, u, v)
+    else:
+        u, v = (ix - u0) * (SR + RD) / SR, (iy - v0) * (SR + RD) / SR
+        cols = (SR + zero, zero, zero, -RD + zero, u, v)
+    return np.asfortranarray(np.stack(cols, axis=2).astype(f))
+
+
+def _osc_project(vol, pre, post, ang, sad, geom):
+    """One CUDAmex_oscIter subset over every view; returns (pre', post')."""
+    _, pr, po = O.osc_iter(vol, pre, post, 1, np.array([0], np.int32), ang, sad, geom)
+    return pr, po
+
+
+@pytest.mark.parametrize("geom", [0, 1, 2])
+def test_osc_projection_is_border_step_half_march(geom):
+    # The OSC projection marches every geometry with step 0.5 and border
+    # (zero) addressing (CUDAmex_oscIter.cu:244,419-425,460-462).  The
+    # general-3D projector does exactly that for arbitrary rays, so with the
+    # standard rays given as pts (same float operations after the rotation)
+    # its ray sums are the OSC line integrals bit for bit.  The volume has
+    # non-zero faces, so clamp addressing would differ; cone at step 1.0
+    # would differ everywhere.
+    W, H, n, sad = 16, 10, 5, 40.0
+    r = rng(30 + geom)
+    vol = f32(r.uniform(0.0, 0.05, size=(W, W, H)))
+    ang = r.uniform(0, 2 * np.pi, n).astype(np.float32)
+    pre = f32(r.uniform(1.0, 3.0, size=(W, H, n)))
+    post = f32(r.uniform(0.0, 0.5, size=(W, H, n)))
+    l = O.g3d_fp(vol, ang, _std_geometry_pts(geom, W, H, sad), n)
+    e = _expf(-l)
+    pr, po = _osc_project(vol, pre, post, ang, sad, geom)
+    # the ratio terms of CUDAmex_oscIter.cu:266-268
+    np.testing.assert_array_equal(pr, f32(pre * e - post))
+    np.testing.assert_array_equal(po, f32(pre * e * l))
+
+
+@pytest.mark.parametrize("geom", [0, 1])
+def test_osc_projection_par_fan_equal_fp_on_zero_faces(geom):
+    # par/fan CUDAmex_FP also steps 0.5; on a volume whose faces are zero
+    # clamp and border fetch the same taps, so the OSC line integral is the
+    # CUDAmex_FP ray sum bit for bit
+    W, H, n, sad = 16, 10, 5, 40.0
+    r = rng(33 + geom)
+    vol = np.zeros((W, W, H), np.float32, order="F")
+    vol[1:-1, 1:-1, 1:-1] = r.uniform(0.0, 0.05, size=(W - 2, W - 2, H - 2))
+    ang = r.uniform(0, 2 * np.pi, n).astype(np.float32)
+    pre = f32(np.ones((W, H, n)))
+    pr, po = _osc_project(vol, pre, np.zeros_like(pre), ang, sad, geom)
+    l = O.fp(vol, sad, ang, n, geom)
+    np.testing.assert_array_equal(pr, _expf(-l))
+    np.testing.assert_array_equal(po, f32(_expf(-l) * l))
+
+
+def test_osc_cone_axis_ray_closed_form():
+    # Cone, beta = 0, W and H odd, SAD a power of two: the central pixel's
+    # ray runs along -x through texel centres in y and z and enters at
+    # x = W/2 exactly.  With mu = 1/64 everywhere it takes 2W samples at
+    # tex x = W, W-1/2, ..., 1/2; the first lerps T[W-1] with the border 0,
+    # every other one sees 1/64: l = (W - 1/4)/64 exactly.  Clamp would
+    # give W/64, step 1.0 (CUDAmex_FP's cone step) (W - 1/2)/64.
+    W, H, sad = 15, 9, 512.0
+    vol = f32(np.full((W, W, H), 1.0 / 64))
+    ang = np.zeros(1, np.float32)
+    pre = f32(np.ones((W, H, 1)))
+    pr, po = _osc_project(vol, pre, np.zeros_like(pre), ang, sad, 2)
+    l = np.float32((W - 0.25) / 64)
+    assert pr[W // 2, H // 2, 0] == _expf(-l)
+    assert po[W // 2, H // 2, 0] == np.float32(_expf(-l) * l)
+
+
+def _embed1(P):
+    """P [W,H,n] inside a zero frame one texel wide: [W+2,H+2,n]."""
+    W, H, n = P.shape
+    Q = np.zeros((W + 2, H + 2, n), np.float32, order="F")
+    Q[1:-1, 1:-1] = P
+    return Q
+
+
+@pytest.mark.parametrize("geom", [0, 1, 2])
+def test_osc_update_equals_bp_of_ratio_sinograms(geom):
+    # num/den are the backprojections of pre'/post' with border addressing
+    # and the weights of CUDAmex_oscIter.cu:320-355 (par U = UU = 1; fan
+    # U = 1 for the row step, UU = SR/(SR-s) for the column and weight;
+    # cone U = UU), which are CUDAmex_BP's geometries.  Border addressing of
+    # P equals clamp addressing of P inside a one-texel zero frame, and the
+    # (W+2, H+2) problem puts the same voxel at the same detector
+    # coordinate + 1 (hw and v0 both grow by 1), so CUDAmex_BP of the
+    # framed sinograms, cropped, is num/den.  The update is then
+    # x <- max(x + x*num/den, 0) inside the circle, x unchanged outside
+    # (CUDAmex_oscIter.cu:113-136).  The ratio sinograms have non-zero
+    # edges and the fan/cone taps reach past them, so clamp in place of
+    # border fails here, and so does a fan row step of UU.
+    W, H, n, sad = 20, 10, 6, 30.0
+    r = rng(40 + geom)
+    mu = f32(r.uniform(0.001, 0.02, size=(W, W, H)))
+    pre = f32(r.uniform(2.0, 3.0, size=(W, H, n)))
+    post = f32(r.uniform(0.0, 1.0, size=(W, H, n)))
+    ang = r.uniform(0, 2 * np.pi, n).astype(np.float32)
+    mu2, pr, po = O.osc_iter(mu, pre, post, 1, np.array([0], np.int32), ang, sad, geom)
+    num = O.bp(_embed1(pr), np.float32(sad), ang, geom)[1:-1, 1:-1, 1:-1].astype(np.float64)
+    den = O.bp(_embed1(po), np.float32(sad), ang, geom)[1:-1, 1:-1, 1:-1].astype(np.float64)
+    inside = _circle(W)
+    m = mu.astype(np.float64)[inside]
+    want = np.maximum(m + m * num[inside] / den[inside], 0.0)
+    np.testing.assert_allclose(mu2[inside], want, rtol=2e-6, atol=0)
+    np.testing.assert_array_equal(mu2[~inside], mu[~inside])
+
+
+@pytest.mark.parametrize("geom", [0, 1, 2])
+def test_osc_subsets_compose_in_pick_order(geom):
+    # subset s takes views picks[s], picks[s] + S, picks[s] + 2S, ... and
+    # the subsets run in pick order, each projecting the volume the previous
+    # one left (CUDAmex_oscIter.cu:505-575): one call equals S calls of one
+    # subset each on those views' sub-sinograms, bit for bit
+    W, H, n, S, sad = 16, 8, 9, 3, 50.0
+    r = rng(50 + geom)
+    mu = f32(r.uniform(0.001, 0.02, size=(W, W, H)))
+    pre = f32(r.uniform(2.0, 3.0, size=(W, H, n)))
+    post = f32(r.uniform(0.0, 1.0, size=(W, H, n)))
+    ang = r.uniform(0, 2 * np.pi, n).astype(np.float32)
+    picks = np.array([2, 0, 1], np.int32)
+    m_all, p_all, q_all = O.osc_iter(mu, pre, post, S, picks, ang, sad, geom)
+    m, P, Q = mu, pre.copy(order="F"), post.copy(order="F")
+    for pick in picks:
+        views = np.arange(pick, n, S)
+        m, p1, q1 = O.osc_iter(m, pre[:, :, views], post[:, :, views], 1, np.array([0], np.int32),
+                               ang[views], sad, geom)
+        P[:, :, views] = p1
+        Q[:, :, views] = q1
+    np.testing.assert_array_equal(m_all, m)
+    np.testing.assert_array_equal(p_all, P)
+    np.testing.assert_array_equal(q_all, Q)
+
+
 def test_osc_f32_close_to_f64_twin():
     W, H, n = 16, 8, 8
     mu, pre, ang = _osc_setup(W, H, n)
@@ -291,6 +484,55 @@ def test_osc_f32_close_to_f64_twin():
     a, _, _ = O.osc_iter(mu, pre, post, 2, np.array([0, 1], np.int32), ang, 100.0, 2)
     b, _, _ = O.osc_iter(mu, pre, post, 2, np.array([0, 1], np.int32), ang, 100.0, 2, f64=True)
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
+
+
+def _osc_tv_by_parts(pre_in, post_in, ang, sad, geom, K, tv, s0, s1, p, tv_steps, first_picks):
+    """OSC_TV_recon.m:33-73 composed from the separately pinned oracle parts
+    (subset schedule, CUDAmex_oscIter, CUDAmex_TVmin_3D) with the driver's
+    own steps written out in numpy: mu starts at single(1e-10); pre/post are
+    raised to 1e-6; from the second iteration on each call gets the raised
+    data with element 1 zeroed (:41,62); TV runs only for k > 1 with
+    c = TV * ||mu_old - mu||, mu_old's element 1 zeroed and NaN differences
+    dropped (:45,64-68); NaN and Inf become 0 (:69-70)."""
+    W, H, n = pre_in.shape
+    mu = np.full((W, W, H), np.float32(1e-10), np.float32, order="F")
+    eps = np.float32(1e-6)
+    pre = f32(np.where(pre_in.astype(np.float64) < 1e-6, eps, pre_in))
+    post = f32(np.where(post_in.astype(np.float64) < 1e-6, eps, post_in))
+    pre2, post2 = pre.copy(order="F"), post.copy(order="F")
+    pre2[0, 0, 0] = 0.0
+    post2[0, 0, 0] = 0.0
+    for k in range(1, K + 1):
+        mu_old = mu.copy(order="F")
+        mu_old[0, 0, 0] = 0.0
+        ns = O.calc_num_subsets(s0, s1, p, K, k)
+        picks = O.calc_subset_order(ns, int(first_picks[k - 1]))
+        mu, _, _ = O.osc_iter(mu, pre, post, ns, picks, ang, sad, geom)
+        pre, post = pre2.copy(order="F"), post2.copy(order="F")
+        if k > 1:
+            d = (mu_old - mu).astype(np.float64)
+            d[np.isnan(d)] = 0.0
+            d_a = np.float32(np.sqrt(np.cumsum((d * d).ravel(order="F"))[-1]))
+            mu = O.tvmin(mu, np.float32(tv) * d_a, tv_steps)
+            mu[~np.isfinite(mu)] = 0.0
+    return mu
+
+
+@pytest.mark.parametrize("geom", [1, 2])
+def test_osc_tv_recon_is_composition_of_parts(geom):
+    # The corner pixel of view 0 is far above the rest so that feeding it
+    # back (not zeroing element 1 from the second iteration on) moves the
+    # result; three iterations so that TV runs twice.
+    W, H, n, sad = 16, 8, 12, 30.0
+    r = rng(70 + geom)
+    pre = f32(r.uniform(1e3, 2e3, size=(W, H, n)))
+    post = f32(pre * r.uniform(0.4, 0.9, size=pre.shape))
+    pre[0, 0, 0], post[0, 0, 0] = 5e4, 1e4
+    post[3, 2, 1] = 0.0  # raised to 1e-6
+    ang = (np.arange(n) * 2 * np.pi / n).astype(np.float32)
+    fp = np.array([2, 1, 3], np.int32)
+    args = (ang, sad, geom, 3, 0.1, 4, 2, 1.0, 5, fp)
+    np.testing.assert_array_equal(O.osc_tv_recon(pre, post, *args), _osc_tv_by_parts(pre, post, *args))
 
 
 def test_osc_tv_recon_f32_close_to_f64_twin():
@@ -320,6 +562,47 @@ def test_tv_gradient_hand_computed_peak():
     assert g[4, 2, 2] == pytest.approx(float(-one), rel=1e-7)
     assert g[1, 2, 2] == 0.0 and g[5, 2, 2] == 0.0
     assert not g[0].any() and not g[:, 0].any() and not g[:, :, 0].any()  # border
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_tv_gradient_step_edge_each_axis(axis):
+    # x = 1 on and beyond plane p of `axis`, 0 before it: the first voxel of
+    # the plane has only its backward difference along `axis` (g = 1/(1+eps)),
+    # the voxel before it only its forward term along `axis` (g = -1/(1+eps))
+    shape = [7, 6, 5]
+    x = np.zeros(shape, np.float32, order="F")
+    idx = [slice(None)] * 3
+    idx[axis] = slice(3, None)
+    x[tuple(idx)] = 1.0
+    g = O.tv_gradient(x)
+    one = float(np.float32(1.0) / (np.float32(1e-6) + np.float32(1.0)))
+    at = lambda k: tuple(k if a == axis else 2 for a in range(3))  # noqa: E731
+    assert g[at(3)] == pytest.approx(one, rel=1e-7)
+    assert g[at(2)] == pytest.approx(-one, rel=1e-7)
+    assert g[at(1)] == 0.0 and g[at(4)] == 0.0
+
+
+def test_tv_gradient_is_gradient_of_tv_functional():
+    # The gradient of CUDAmex_TVmin_3D.cu:101-158 is d/dx of the isotropic
+    # TV with backward differences, TV(x) = sum_{v >= 1 on every axis}
+    # sqrt(|x_v - x_{v-e1}|^2 + |x_v - x_{v-e2}|^2 + |x_v - x_{v-e3}|^2),
+    # up to eps = 1e-6 in each denominator.  Differentiating that functional
+    # (torch autograd, float64) checks every one of the 13 stencil taps,
+    # including the six diagonal ones (xmypz, xpymz, ...) that only enter
+    # the neighbours' norms.
+    torch = pytest.importorskip("torch")
+    r = rng(60)
+    n = (9, 8, 7)
+    base = np.cumsum(r.uniform(0.2, 1.0, size=n), axis=0)  # no flat spots
+    x = f32(base + r.uniform(0.0, 0.6, size=n))
+    t = torch.tensor(x.astype(np.float64), requires_grad=True)
+    d0 = t[1:, 1:, 1:] - t[:-1, 1:, 1:]
+    d1 = t[1:, 1:, 1:] - t[1:, :-1, 1:]
+    d2 = t[1:, 1:, 1:] - t[1:, 1:, :-1]
+    torch.sqrt(d0 * d0 + d1 * d1 + d2 * d2).sum().backward()
+    want = t.grad.numpy()[1:-1, 1:-1, 1:-1]
+    got = O.tv_gradient(x).astype(np.float64)[1:-1, 1:-1, 1:-1]
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * np.abs(want).max())
 
 
 def test_tv_constant_volume_gives_nan():
