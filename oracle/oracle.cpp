@@ -12,7 +12,10 @@
 // sampler is the public CUDA texture-fetch formula (unnormalised
 // coordinates, x_B = x - 0.5, i = floor(x_B), a = x_B - i) with exact
 // FP32 weights, lerp(a, b, t) = fmaf(t, b - a, a), applied x, then y,
-// then z.  Clamp uses the edge texel, border uses 0.
+// then z.  Clamp uses the edge texel, border uses 0.  A second weight
+// mode (orc_set_texture_weight_bits(8)) rounds each fraction to 8 bits, the
+// texture unit's documented weight format, to measure how far exact
+// weights sit from what NVIDIA hardware computes.
 #include "oracle.h"
 
 #include <algorithm>
@@ -29,6 +32,9 @@
 namespace {
 
 int g_threads = 0;
+// Fraction bits of the texture weights: 0 exact FP32, else rounded to
+// multiples of 2^-bits (see weight_of).
+int g_weight_bits = 0;
 
 int nthreads() {
   if (g_threads > 0) return g_threads;
@@ -97,6 +103,18 @@ inline R fetch2(const E* T, int nx, int ny, int i, int j, int mode) {
   return R(T[(size_t)i + (size_t)nx * (size_t)j]);
 }
 
+// Filter weight of fraction a.  The public CUDA Programming Guide
+// (texture fetching appendix, linear filtering) stores it in 9-bit fixed
+// point with 8 fractional bits; with g_weight_bits = 8 the fraction is
+// rounded to the nearest multiple of 2^-8 (ties to even).  The rounding
+// rule of the hardware is not documented; nearest is the model here.
+template <typename R>
+inline R weight_of(R a) {
+  if (g_weight_bits <= 0) return a;
+  const R s = (R)(1 << g_weight_bits);
+  return std::nearbyint(a * s) / s;
+}
+
 // tex3D(reconTex, x, y, z) with cudaFilterModeLinear, unnormalised
 // coordinates (CUDAmex_FP.cu:405-409, CUDAmex_general3D_FP.cu:217-221,
 // CUDAmex_oscIter.cu:458-462).
@@ -105,7 +123,7 @@ R tex3(const E* T, int nx, int ny, int nz, R x, R y, R z, int mode) {
   const R h = R(0.5);
   R xb = x - h, yb = y - h, zb = z - h;
   R fx = std::floor(xb), fy = std::floor(yb), fz = std::floor(zb);
-  R ax = xb - fx, ay = yb - fy, az = zb - fz;
+  R ax = weight_of(xb - fx), ay = weight_of(yb - fy), az = weight_of(zb - fz);
   int i = to_index(fx), j = to_index(fy), k = to_index(fz);
   R t000 = fetch3<R, E>(T, nx, ny, nz, i, j, k, mode);
   R t100 = fetch3<R, E>(T, nx, ny, nz, i + 1, j, k, mode);
@@ -130,7 +148,7 @@ R tex2(const E* T, int nx, int ny, R x, R y, int mode) {
   const R h = R(0.5);
   R xb = x - h, yb = y - h;
   R fx = std::floor(xb), fy = std::floor(yb);
-  R ax = xb - fx, ay = yb - fy;
+  R ax = weight_of(xb - fx), ay = weight_of(yb - fy);
   int i = to_index(fx), j = to_index(fy);
   R t00 = fetch2<R, E>(T, nx, ny, i, j, mode);
   R t10 = fetch2<R, E>(T, nx, ny, i + 1, j, mode);
@@ -857,6 +875,8 @@ extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
 int orc_get_threads(void) { return nthreads(); }
+void orc_set_texture_weight_bits(int bits) { g_weight_bits = bits > 0 ? std::min(bits, 20) : 0; }
+int orc_get_texture_weight_bits(void) { return g_weight_bits; }
 
 float orc_tex3(const float* T, int nx, int ny, int nz, float x, float y, float z, int mode) {
   return tex3<float, float>(T, nx, ny, nz, x, y, z, mode);

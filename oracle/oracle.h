@@ -33,7 +33,15 @@ extern "C" {
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
-/* Exact FP32 texture fetch restatement (for pin tests of the sampler). */
+/* Texture weight mode of every sampler call below: 0 (default) exact FP32
+ * fractions; 8 rounds each fraction to a multiple of 2^-8, the 9-bit
+ * fixed-point weight (8 fraction bits) of the CUDA Programming Guide's
+ * texture-fetching appendix, to bound the distance between exact weights
+ * and NVIDIA's texture unit.  Process-wide; not for concurrent callers. */
+void orc_set_texture_weight_bits(int bits);
+int orc_get_texture_weight_bits(void);
+
+/* FP32 texture fetch restatement (for pin tests of the sampler). */
 float orc_tex3(const float* T, int nx, int ny, int nz, float x, float y, float z, int mode);
 float orc_tex2(const float* T, int nx, int ny, float x, float y, int mode);
 

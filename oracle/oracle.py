@@ -44,6 +44,8 @@ def lib():
         sig = {
             "orc_set_threads": (None, [i]),
             "orc_get_threads": (i, []),
+            "orc_set_texture_weight_bits": (None, [i]),
+            "orc_get_texture_weight_bits": (i, []),
             "orc_tex3": (f, [_f32p, i, i, i, f, f, f, i]),
             "orc_tex2": (f, [_f32p, i, i, f, f, i]),
             "orc_fp": (i64, [_f32p, i, i, f, _f32p, i, i, _f32p]),
@@ -118,6 +120,25 @@ def set_threads(n: int):
 
 def threads() -> int:
     return lib().orc_get_threads()
+
+
+class texture_weight_bits:
+    """Context manager: every sampler call inside rounds its filter
+    fractions to `bits` fraction bits (8: the texture unit's documented
+    9-bit fixed-point weights; 0: exact FP32 weights, the default)."""
+
+    def __init__(self, bits):
+        self.bits = int(bits)
+        self.saved = 0
+
+    def __enter__(self):
+        self.saved = lib().orc_get_texture_weight_bits()
+        lib().orc_set_texture_weight_bits(self.bits)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_texture_weight_bits(self.saved)
+        return False
 
 
 def tex3(T, x, y, z, mode=CLAMP):

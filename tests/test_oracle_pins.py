@@ -70,6 +70,43 @@ def test_tex3_address_modes_outside():
     assert O.tex3(T, 0.0, 1.5, 2.5, O.CLAMP) == T[0, 1, 2]
 
 
+def test_texture_weight_bits_mode():
+    # 8 fraction bits: the filter fraction rounds to the nearest multiple of
+    # 1/256; texel centres (fraction 0) and the exact mode are unchanged
+    T = f32(np.arange(10, dtype=np.float32).reshape(10, 1))
+    V = f32(rng(7).uniform(size=(6, 5, 4)))
+    for x in (2.3, 4.5017, 7.998, 1.0):
+        frac = (x - 0.5) - np.floor(x - 0.5)
+        want = np.floor(x - 0.5) + np.round(frac * 256) / 256
+        with O.texture_weight_bits(8):
+            assert O.tex2(T, x, 0.5) == pytest.approx(want, abs=1e-6)
+        assert O.tex2(T, x, 0.5) == pytest.approx(x - 0.5, abs=1e-6)
+    with O.texture_weight_bits(8):
+        q = O.tex3(V, 2.3, 1.7, 2.9, O.CLAMP)
+        c = O.tex3(V, 2.5, 1.5, 3.5, O.CLAMP)
+    assert O.tex3(V, 2.5, 1.5, 3.5, O.CLAMP) == c == V[2, 1, 3]
+    assert q == pytest.approx(_numpy_trilinear(V, 2.3 - 0.3 + 77 / 256, 1.7 - 0.2 + 51 / 256,
+                                               2.9 - 0.4 + 102 / 256, O.CLAMP), abs=1e-6)
+    assert O.threads() > 0 and O.lib().orc_get_texture_weight_bits() == 0
+
+
+def test_texture_weight_gap_small_parallel():
+    # exact against 8-bit weights on a small parallel FP/BP: the gap is far
+    # below the FP16 storage error and within the FP32 parity bar's order
+    import mexct.phantoms as ph
+
+    vol = f32(ph.ellipsoid_phantom(32, 32, 1, 3, 12.0))
+    ang = ph.angles_uniform(24, np.pi)
+    a = O.fp(vol, 1000.0, ang, 24, 0)
+    with O.texture_weight_bits(8):
+        b = O.fp(vol, 1000.0, ang, 24, 0)
+        bb = O.bp(a, 1000.0, ang, 0)
+    ba = O.bp(a, 1000.0, ang, 0)
+    for x, y in ((b, a), (bb, ba)):
+        d = np.abs(x.astype(np.float64) - y)
+        assert 0 < d.max() / np.abs(y).max() < 1e-3
+
+
 def test_tex2_linear_function_reproduced():
     a, b, c = 0.75, -1.25, 3.0
     u, v = np.meshgrid(np.arange(9), np.arange(7), indexing="ij")
