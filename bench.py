@@ -25,7 +25,41 @@ sys.path.insert(0, os.path.join(ROOT, "mex-cuda-ct_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker (cpu_baseline, FP64 sample)
 
 HBM_PEAK_GBS = 8000.0
-FP_TRAFFIC_BYTES = 134.5e9  # measured, cone 384^3 x 512 f32 FP launch
+VC_PEAK_GBS = 256 * 64 * 2.4  # vector L1 caches: 64 B/clock per CU, 256 CUs, 2.4 GHz
+# HBM-side bytes of one cone 384^3 x 512 f32 FP launch from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (copy-calibrated), valid for the kernel sources
+# whose hash is stamped beside it; a changed source reports null until the
+# counters are taken again.
+FP_TRAFFIC = {"bytes": 135.69e9, "sources_sha16": "f648fc64a07a155b"}
+FP_KERNEL_SOURCES = ("mex-cuda-ct_amd/csrc/fpz.hip", "mex-cuda-ct_amd/csrc/ray.h",
+                     "mex-cuda-ct_amd/csrc/common.h")
+
+
+def kernel_sources_sha16():
+    import hashlib
+    h = hashlib.sha256()
+    for f in FP_KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def host_cpu():
+    """(cores this process may run on, the host's logical CPUs, CPU model)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, os.cpu_count() or avail, model
 
 
 def log(*a):
@@ -124,28 +158,32 @@ def main():
     filt32 = filt.clone()
     rec32 = rec.clone()
 
-    # Measured gather roofline: the probe issues the projectors' own load
-    # instructions with ideal coherence (FP: quad-volume loads per sample,
-    # BP: one row-paired load per update).  Rates are compared as texels
-    # gathered per second: 8 per FP sample, 4 per BP update.
+    # Measured gather roofline: each operator against its own loads-only
+    # form (the same launches, block order, addresses and loads, with the
+    # taps XOR-folded instead of interpolated), which it cannot outrun.
+    # Rates are texels gathered per second: 8 per FP sample, 4 per BP
+    # update, storage bytes each; the vector caches' peak (64 B per clock
+    # per CU, 256 CUs at 2.4 GHz) is reported beside.
     import oracle as O  # checker: sample-count estimate for the FP gather rate
 
-    def probe_ms(pattern, storage, reps=3):
-        """Best time of the probe over its unroll depths 1, 2, 4 and 8."""
-        best = None
-        for k in range(4):
-            pk = pattern | (k << 4)
-            ctx.probe_gather(storage, W, H, n, pk)  # warm
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                ns = ctx.probe_gather(storage, W, H, n, pk)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / reps
-            if best is None or ms < best[0]:
-                best = (ms, ns)
-        return best
+    scratch = {"fp": torch.empty_like(sino), "bp": torch.empty_like(rec)}
+
+    def loads_ms(kind, storage, v, reps=3):
+        """Mean time of the loads-only FP (on volume v) or BP (on the filtered
+        sinogram), measured like the operators: events around the call."""
+        out = scratch[kind]
+        if kind == "fp":
+            call = lambda: ctx.fp_loads(v, SAD, ang, n, 2, out, storage=storage)  # noqa: E731
+        else:
+            call = lambda: ctx.bp_loads(filt32, SAD, ang, 2, out, storage=storage)  # noqa: E731
+        call()  # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
 
     views8 = np.linspace(0, n - 1, 8).astype(int)
     S_est = float(np.mean([O.fp(vol_np, SAD, ang[v:v + 1], 1, 2, return_samples=True)[1]
@@ -154,27 +192,28 @@ def main():
     gx, gy = np.meshgrid(np.arange(W), np.arange(W), indexing="ij")
     U_bp = float(((gx - hw) ** 2 + (gy - hw) ** 2 < hw * hw).sum()) * H * n
 
-    def gather_fracs(storage, t_fp, t_bp):
+    def gather_fracs(storage, t_fp, t_bp, v):
         es = 4 if storage == mexct.F32 else 2
-        pf_ms, pf_n = probe_ms(0, storage)
-        pb_ms, pb_n = probe_ms(1, storage)
-        fp_a = S_est * 8 * es / (t_fp * 1e-3) / 1e9
-        bp_a = U_bp * 4 * es / (t_bp * 1e-3) / 1e9
-        fp_p = pf_n * 8 * es / (pf_ms * 1e-3) / 1e9
-        bp_p = pb_n * 4 * es / (pb_ms * 1e-3) / 1e9
-        return {
-            "fp": {"achieved_GBs": round(fp_a, 1), "measured_peak_GBs": round(fp_p, 1),
-                   "frac": round(fp_a / fp_p, 4), "probe_ms": round(pf_ms, 3)},
-            "bp": {"achieved_GBs": round(bp_a, 1), "measured_peak_GBs": round(bp_p, 1),
-                   "frac": round(bp_a / bp_p, 4), "probe_ms": round(pb_ms, 3)},
-        }
+        lf = loads_ms("fp", storage, v)
+        lb = loads_ms("bp", storage, v)
+        out = {}
+        for k, t, tl, nbytes in (("fp", t_fp, lf, S_est * 8 * es), ("bp", t_bp, lb, U_bp * 4 * es)):
+            a = nbytes / (t * 1e-3) / 1e9
+            p = nbytes / (tl * 1e-3) / 1e9
+            out[k] = {"achieved_GBs": round(a, 1), "measured_peak_GBs": round(p, 1),
+                      "frac": round(a / p, 4), "loads_only_ms": round(tl, 3),
+                      "frac_of_vector_cache_peak": round(a / VC_PEAK_GBS, 4)}
+        return out
 
-    gather = gather_fracs(mexct.F32, fp_ms, bp_ms)
+    gather = gather_fracs(mexct.F32, fp_ms, bp_ms, vol)
     gather.update({"fp_samples": S_est,
                    "fp_samples_note": "ray-march samples, mean of 8 views x views (oracle count)",
                    "bp_updates_in_circle": U_bp,
                    "unit": "GB/s of gathered texels (8 per FP sample, 4 per BP update, "
-                           "storage bytes each)"})
+                           "storage bytes each)",
+                   "ceiling": "measured_peak = the operator's loads-only form (same launch, "
+                              "every load, taps XOR-folded instead of interpolated)",
+                   "vector_cache_peak_GBs": VC_PEAK_GBS})
     log(f"[bench] gather roofline: FP {gather['fp']['achieved_GBs']:.0f}/"
         f"{gather['fp']['measured_peak_GBs']:.0f} GB/s ({gather['fp']['frac']:.2f}), "
         f"BP {gather['bp']['achieved_GBs']:.0f}/{gather['bp']['measured_peak_GBs']:.0f} GB/s "
@@ -187,11 +226,18 @@ def main():
     # traffic: HBM-side bytes per FP launch from rocprofv3 --pmc FETCH_SIZE and
     # WRITE_SIZE on this workload, scaled by what they read for a 1 GiB device
     # copy (FETCH x2.0, WRITE x1.0); rerun the counters when the kernel changes
+    sha = kernel_sources_sha16()
+    fresh = sha == FP_TRAFFIC["sources_sha16"] and (W, n) == (384, 512)
     roofline = {"bound": "hbm", "kernel": "fpz_kernel<cone,f32,x-pair volume>",
                 "achieved": round(fp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(fp_gbs / HBM_PEAK_GBS, 4),
-                "traffic": FP_TRAFFIC_BYTES if (W, n) == (384, 512) else None,
-                "traffic_note": "rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated",
+                "traffic": FP_TRAFFIC["bytes"] if fresh else None,
+                "traffic_note": ("rocprofv3 FETCH_SIZE+WRITE_SIZE per launch, copy-calibrated, "
+                                 f"taken on kernel sources {FP_TRAFFIC['sources_sha16']}"
+                                 if fresh else
+                                 f"kernel sources {sha} differ from the counter run's "
+                                 f"{FP_TRAFFIC['sources_sha16']} (or another workload): "
+                                 "no measured traffic"),
                 "algorithmic_bytes_per_launch": fp_bytes}
 
     f16 = None
@@ -226,7 +272,7 @@ def main():
         rec32n = rec32.cpu().numpy()[:, cy, cx].T
         bb16 = float(np.linalg.norm(rec16 - refb) / np.linalg.norm(refb))
         bb32 = float(np.linalg.norm(rec32n - refb) / np.linalg.norm(refb))
-        gr16 = gather_fracs(mexct.F16, f16_fp_ms, f16_bp_ms)
+        gr16 = gather_fracs(mexct.F16, f16_fp_ms, f16_bp_ms, vol16)
         log(f"[bench] fp16 gather roofline: FP {gr16['fp']['frac']:.2f}, BP {gr16['bp']['frac']:.2f}")
         f16 = {"fp_ms": round(f16_fp_ms, 3), "bp_ms": round(f16_bp_ms, 3), "gather_roofline": gr16,
                "fp_gups": round(U / (f16_fp_ms * 1e-3) / 1e9, 1),
@@ -243,11 +289,9 @@ def main():
     if not args.no_cpu:
         import oracle as O
 
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
-        cores = max(1, min(cores, 16))  # the host share of one GPU
+        avail, logical, model = host_cpu()
+        # at most 16 threads: the host share of one GPU on the box
+        cores = max(1, min(avail, 16))
         O.set_threads(cores)
         kv = 64  # views of the FP sample (about 5 s on 16 threads)
         log(f"[bench] cpu baseline: oracle FP on {kv} views, BP on 1024 columns, {cores} threads")
@@ -266,6 +310,8 @@ def main():
         cbp = float(ncol) * H * n / tb / 1e9
         cpu = {"value": round(2.0 / (1.0 / cfp + 1.0 / cbp), 4), "unit": "GUPS", "cores": cores,
                "kind": "port", "fp_gups": round(cfp, 4), "bp_gups": round(cbp, 4),
+               "host_cpus_available": avail, "host_cpus_logical": logical, "cpu_model": model,
+               "threads_note": "capped at 16 threads, the host share of one GPU",
                "sample": f"FP of {kv} of {n} views ({tf:.1f} s); BP of {ncol} random voxel "
                          f"columns x {H} z x {n} views ({tb:.1f} s); FP32 oracle"}
         log(f"[bench] cpu: FP {cfp:.3f} GUPS, BP {cbp:.3f} GUPS")
@@ -368,7 +414,11 @@ def other_workloads(ctx, stream, log):
     res["helix_g3d_320x360"] = {
         "fp_ms": dev_ms(lambda: ctx.g3d_fp(vt, z, pt, nv, g)),
         "fp16_fp_ms": dev_ms(lambda: ctx.g3d_fp(vt, z, pt, nv, g, storage=mexct.F16)),
-        "bp_ms": dev_ms(lambda: ctx.g3d_bp(g, z, pt, side, side, rv), reps=1)}
+        "bp_ms": dev_ms(lambda: ctx.g3d_bp(g, z, pt, side, side, rv), reps=3),
+        "bp_deterministic_ms": dev_ms(lambda: ctx.g3d_bp(g, z, pt, side, side, rv, mode=1), reps=1),
+        "bp_atomics_ms": dev_ms(lambda: ctx.g3d_bp(g, z, pt, side, side, rv, mode=2), reps=1),
+        "bp_note": "bp_ms: voxel bricks in LDS (default); deterministic: the same with 64-bit "
+                   "fixed-point sums; atomics: the ray-driven float-atomic scatter"}
     del vt, pt, g, rv
     log("[bench] other workloads: OSC-TV 256^3 x 60")
     W, n = 256, 60
@@ -392,32 +442,40 @@ def other_workloads(ctx, stream, log):
 
 
 def same_geometry_burst(Server, phantoms, n=16, slots=2, max_group=8):
-    """Bursts of n same-geometry requests (parallel 128^3, 180 views, f32, host
-    arrays) queued at once: servers that batch up to max_group of them per
-    launch against servers that run them one by one (max_group 1).  Each
-    server first runs a burst untimed, which sizes its buffers and tunes the
-    shapes; the second burst is timed."""
+    """Bursts of n same-geometry requests (parallel 128^3, 180 views, f32)
+    queued at once: servers that batch up to max_group of them per launch
+    against servers that run them one by one (max_group 1), with host arrays
+    (copied in and out) and with device-resident arrays (torch tensors, no
+    host copies).  Each server first runs a burst untimed, which sizes its
+    buffers and tunes the shapes; the second burst is timed."""
+    import torch
+
     vol = phantoms.par_phantom_128(seed=1)
     ang = phantoms.angles_uniform(180, np.pi)
     sino = np.asfortranarray(np.full((128, 128, 180), 0.5, np.float32))
+    dvol = torch.from_numpy(np.ascontiguousarray(vol.T)).cuda()
+    dsino = torch.from_numpy(np.ascontiguousarray(sino.T)).cuda()
     out = {"workload": f"bursts of {n} requests, parallel 128^3, 180 views, f32, queued at once; "
-                       "the second burst on a warm server is timed",
+                       "the second burst on a warm server is timed; host arrays, or resident "
+                       "device arrays",
            "slots": slots}
     for kind in ("fp", "bp"):
-        for mg in (1, max_group):
-            with Server(hbm_budget=16 << 30, slots=slots, max_group=mg) as s:
-                sub = ((lambda: s.submit_fp(vol, 1000.0, ang, 180, 0)) if kind == "fp"
-                       else (lambda: s.submit_bp(sino, 1000.0, ang, 0)))
-                for t in [sub() for _ in range(n)]:
-                    s.result(t)
-                b0 = s.stats()["batched"]
-                t0 = time.perf_counter()
-                for t in [sub() for _ in range(n)]:
-                    s.result(t)
-                wall = time.perf_counter() - t0
-                st = s.stats()
-            out[f"{kind}_max_group_{mg}"] = {"throughput_req_s": round(n / wall, 1),
-                                             "batched": st["batched"] - b0}
+        for resident in (False, True):
+            v, p = (dvol, dsino) if resident else (vol, sino)
+            for mg in (1, max_group):
+                with Server(hbm_budget=16 << 30, slots=slots, max_group=mg) as s:
+                    sub = ((lambda: s.submit_fp(v, 1000.0, ang, 180, 0)) if kind == "fp"
+                           else (lambda: s.submit_bp(p, 1000.0, ang, 0)))
+                    for t in [sub() for _ in range(n)]:
+                        s.result(t)
+                    b0 = s.stats()["batched"]
+                    t0 = time.perf_counter()
+                    for t in [sub() for _ in range(n)]:
+                        s.result(t)
+                    wall = time.perf_counter() - t0
+                    st = s.stats()
+                key = f"{kind}_{'resident_' if resident else ''}max_group_{mg}"
+                out[key] = {"throughput_req_s": round(n / wall, 1), "batched": st["batched"] - b0}
     return out
 
 
@@ -502,9 +560,9 @@ def serving(n_req, rate):
 
     budget = 96 << 30
 
-    def served(open_loop):
+    def served(open_loop, max_group=4):
         lats, queues = [None] * n_req, [None] * n_req
-        with Server(hbm_budget=budget, slots=4, max_group=4) as s:
+        with Server(hbm_budget=budget, slots=4, max_group=max_group) as s:
             tickets = [None] * n_req
             ready = threading.Semaphore(0)
 
@@ -540,6 +598,10 @@ def serving(n_req, rate):
     lat_open, wall_open, st_open = served(True)
     log(f"[serve] open loop: p50 {np.percentile(lat_open, 50) * 1e3:.1f} ms "
         f"p99 {np.percentile(lat_open, 99) * 1e3:.1f} ms")
+    log("[serve] open-loop run without batching (max_group 1)")
+    lat_nob, wall_nob, st_nob = served(True, max_group=1)
+    log(f"[serve] open loop, no batching: p50 {np.percentile(lat_nob, 50) * 1e3:.1f} ms "
+        f"p99 {np.percentile(lat_nob, 99) * 1e3:.1f} ms")
     log("[serve] same-geometry burst")
     burst = same_geometry_burst(Server, phantoms)
     log(f"[serve] burst: {burst}")
@@ -569,7 +631,15 @@ def serving(n_req, rate):
         "open_loop": {"p50_ms": pct(lat_open, 50), "p99_ms": pct(lat_open, 99),
                       "mean_ms": round(float(lat_open.mean()) * 1e3, 2),
                       "throughput_req_s": round(n_req / wall_open, 3),
-                      "peak_reserved_gb": round(st_open["peak_reserved_bytes"] / 1e9, 2)},
+                      "peak_reserved_gb": round(st_open["peak_reserved_bytes"] / 1e9, 2),
+                      "batched": st_open["batched"]},
+        "open_loop_no_batching": {"max_group": 1, "p50_ms": pct(lat_nob, 50),
+                                  "p99_ms": pct(lat_nob, 99),
+                                  "mean_ms": round(float(lat_nob.mean()) * 1e3, 2),
+                                  "throughput_req_s": round(n_req / wall_nob, 3),
+                                  "note": "every request of this stream has its own SAD, so no "
+                                          "two share a batch key; the same-geometry bursts "
+                                          "measure batching"},
         "saturated": {"throughput_req_s": round(n_req / wall_sat, 3),
                       "p50_ms": pct(lat_sat, 50), "p99_ms": pct(lat_sat, 99),
                       "groups": st_sat["groups"], "grouped": st_sat["grouped"],

@@ -56,6 +56,8 @@ typedef struct mexct_opts {
   int32_t zrange;   /* BP z-voxels per thread: 32, 48 or 64; 0 -> chosen by H    */
   int32_t long_ray; /* F16/BF16 FP: rays of more samples than this are summed with  */
                     /* compensated (Kahan) summation; 0 -> 1024, < 0 -> never       */
+  int32_t g3d_bp;   /* general-3D BP: MEXCT_G3D_BRICKS (0, default),                */
+                    /* MEXCT_G3D_DETERMINISTIC or MEXCT_G3D_ATOMICS (see below)      */
 } mexct_opts;
 
 typedef struct mexct_ctx mexct_ctx;
@@ -146,9 +148,19 @@ int mexct_g3d_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t
 /* ------------------------------------------------------------------ general-3D BP
  * bp = CUDAmex_general3D_BP(P, angles, pts, M, N)
  * replaces projection_kernels/CUDAmex_general3D_BP.cu:194-282 (kernel :61-188).
- * Ray-driven scatter with float atomics into [M,M,N]; two corners per
- * 0.5-voxel step with the reference's weights.  Summation order is not
- * fixed (as the reference's atomicAdd). */
+ * Every 0.5-voxel sample of every ray adds the reference's two weighted
+ * corners to [M,M,N].  opts->g3d_bp selects how the sums are formed:
+ *   MEXCT_G3D_BRICKS: per 16x16x32 voxel brick in LDS, written once (float
+ *     sums; their order is not fixed, as the reference's atomicAdd order);
+ *   MEXCT_G3D_DETERMINISTIC: the same with 64-bit fixed-point sums, exact
+ *     in any order, so every run gives the same bits (up to 2^23 samples per
+ *     voxel; the fixed-point step is max|P| * 2^-39 or finer);
+ *   MEXCT_G3D_ATOMICS: the ray-driven scatter with float atomics.
+ * Volumes whose bricks exceed the binning pass's LDS (about 800^3) take
+ * the atomics (MEXCT_ENOTSUP for the deterministic mode). */
+#define MEXCT_G3D_BRICKS 0
+#define MEXCT_G3D_DETERMINISTIC 1
+#define MEXCT_G3D_ATOMICS 2
 int mexct_g3d_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, const float* angles,
                  int64_t n_angles, const float* pts, int32_t pts_per_view, int64_t M, int64_t N,
                  float* out, const mexct_opts* opts);
@@ -246,6 +258,25 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
                            int32_t sFinal, double p, const int32_t* first_picks, float* mu_out,
                            const mexct_opts* opts);
 
+/* Checkpoint / resume of the driver above (no counterpart in the
+ * reference, whose only state between iterations is mu, OSC_TV_recon.m:44-71):
+ * runs iterations k_begin..k_end (1 <= k_begin <= k_end <= nIter) of the
+ * nIter-iteration schedule on mu [W,W,H] in place.  On entry mu holds the
+ * volume after iteration k_begin - 1 (ignored for k_begin = 1, which starts
+ * from 1e-10); on return the volume after k_end.  Splitting 1..nIter into
+ * consecutive ranges gives the bits of one mexct_osc_tv_recon call. */
+int mexct_osc_tv_iterations(const float* pre, const float* post, int64_t W, int64_t H,
+                            int64_t nProj, const float* angles, int64_t n_angles, float sad,
+                            int32_t geom, int32_t nIter, float tv, int32_t sInit, int32_t sFinal,
+                            double p, const int32_t* first_picks, int32_t k_begin, int32_t k_end,
+                            float* mu, const mexct_opts* opts);
+int mexct_osc_tv_iterations_dev(mexct_ctx* ctx, const float* pre, const float* post, int64_t W,
+                                int64_t H, int64_t nProj, const float* angles, float sad,
+                                int32_t geom, int32_t nIter, float tv, int32_t sInit,
+                                int32_t sFinal, double p, const int32_t* first_picks,
+                                int32_t k_begin, int32_t k_end, float* mu,
+                                const mexct_opts* opts);
+
 /* ------------------------------------------------------------------ measurement
  * No reference counterpart: the gather-throughput microbenchmark behind the
  * projectors' "measured gather roofline".  Launches, on the context's stream,
@@ -258,6 +289,22 @@ int mexct_osc_tv_recon_dev(mexct_ctx* ctx, const float* pre, const float* post, 
  * the context's workspace (its contents are whatever the last projection
  * left there).  *samples receives the number of samples (FP) or updates
  * (BP) the launch performs. */
+/* The forward projector's own gather ceiling: mexct_fp_dev's halo pass and
+ * projector launch with the same block order, per-lane addresses and loads,
+ * but each ray's taps XOR-folded instead of interpolated and summed; out
+ * receives the folds (not a projection).  The projector cannot outrun it.
+ * par/fan/cone while the volume fits the z-fastest layouts
+ * (MEXCT_ENOTSUP otherwise). */
+int mexct_fp_loads_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
+                       float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+                       const mexct_opts* opts);
+/* The backprojector's own gather ceiling, as mexct_fp_loads_dev: mexct_bp_dev's
+ * copy and launches (cone; MEXCT_ENOTSUP otherwise) with every load but
+ * each loaded element XOR-folded instead of interpolated; out receives the
+ * folds. */
+int mexct_bp_loads_dev(mexct_ctx* ctx, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
+                       int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+                       const mexct_opts* opts);
 #define MEXCT_PROBE_FP 0
 #define MEXCT_PROBE_BP 1
 /* pattern | MEXCT_PROBE_UNROLL(k): the probe loop unrolled 2^k times (k = 0..3,
@@ -277,8 +324,11 @@ int mexct_probe_gather_dev(mexct_ctx* ctx, int32_t storage, int64_t W, int64_t H
  * workers) and runs an FP, BP or FDK group as one batched launch, an OSC-TV
  * group back to back.  Requests run the
  * same operators as the entry points above, so a result does not depend on
- * what else is in flight.  Arrays are host memory in the MATLAB layout; the
- * caller keeps input/output alive and unchanged until mexct_server_wait
+ * what else is in flight.  Arrays are host memory in the MATLAB layout, or,
+ * with resident = 1, device memory of the server's device (checked at
+ * submit: each array must lie in a device allocation of that device that
+ * holds the request's bytes from that address on);
+ * the caller keeps input/output alive and unchanged until mexct_server_wait
  * returns for the ticket.
  *   FP:    input vol [W,W,H]          -> output sino [W,H,nProj]
  *   BP:    input sino [W,H,nProj]     -> output vol [W,W,H]
@@ -302,12 +352,18 @@ typedef struct mexct_request {
   const float* input2;
   float* output;
   const int32_t* first_picks; /* OSCTV: osc_iters entries */
+  int32_t resident;           /* 1: input, input2 and output are device arrays */
 } mexct_request;
 
 typedef struct mexct_server mexct_server;
 
 int mexct_server_create(mexct_server** sv, int device, int64_t hbm_budget, int32_t slots,
                         int32_t max_group);
+/* Device bytes a request is reserved against the budget: its staging
+ * arrays plus the workspace the entry point it runs takes (an upper bound
+ * of what a worker holds while running it alone); MEXCT_EINVAL (negative)
+ * for a malformed request. */
+int64_t mexct_server_footprint(const mexct_request* req);
 /* Queues a request; MEXCT_ENOMEM if its footprint alone exceeds the budget. */
 int mexct_server_submit(mexct_server* sv, const mexct_request* req, uint64_t* ticket);
 /* Blocks until the request has finished; *status is its return code (on

@@ -23,8 +23,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmexct.so")
-SOURCES = ["pad.hip", "fp.hip", "fpz.hip", "bp.hip", "tv.hip", "fdk.hip", "probe.hip", "api.hip",
-           "serve.hip"]
+SOURCES = ["pad.hip", "fp.hip", "fpz.hip", "bp.hip", "g3dbp.hip", "tv.hip", "fdk.hip", "probe.hip",
+           "api.hip", "serve.hip"]
 HEADERS = ["common.h", "internal.h", "ray.h"]
 ARCH = os.environ.get("MEXCT_ARCH", "gfx950")
 

@@ -12,6 +12,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -89,6 +90,10 @@ enum Slot {
   WS_MAXBITS,  // max|x| bits for the FP16 copies: [1] volume, [2] sinogram
   WS_PARKER,   // Parker window: column terms, view betas, then [nProj][W] weights
   WS_FILTERED, // FBP: the filtered sinogram handed to the backprojector
+  WS_G3D_CK,   // general-3D BP bricks: ray checkpoints
+  WS_G3D_MASK, // (brick, tile) bits
+  WS_G3D_HALO, // halo planes (float, or 64-bit fixed point)
+  WS_G3D_ACC,  // deterministic mode: 64-bit fixed-point volume
   WS_COUNT
 };
 
@@ -275,28 +280,36 @@ int long_ray_of(const mexct_opts* o) {
 
 // z-fastest FP volume layout for storage st (see fpz.hip): DUAL for f32,
 // QUAD for 2-byte storage while the copy stays below 4 GiB (32-bit buffer
-// offsets), else PLAIN below 2 GiB, else -1 (the x-fastest projector).
-// Measured at 384^3 x 512 cone: f32 PLAIN 47.6 ms, DUAL 32.3 ms; f16 PLAIN
-// 50.8 ms, DUAL 24.1 ms, QUAD 20.1 ms.
+// offsets); past that the same copies read through 64-bit addresses
+// (DUAL64 / QUAD64, below 2^32 elements); else -1 (the x-fastest
+// projector).  Measured at 384^3 x 512 cone: f32 PLAIN 47.6 ms, DUAL
+// 32.3 ms; f16 PLAIN 50.8 ms, DUAL 24.1 ms, QUAD 20.1 ms.
+// MEXCT_FP_LAYOUT=wide64 takes the 64-bit forms below 4 GiB as well, and
+// MEXCT_FP_LAYOUT=plain the one-value-per-element copy (below 2 GiB), so
+// that every address path can be compared with the default on one volume.
 int fp_layout(int64_t W, int64_t H, int st, size_t* bytes) {
   const size_t es = (size_t)store_bytes(st);
   // the z-fastest projector's 24-bit index multiplies need (W+4)(H+4) < 2^23
   if ((W + 4) * (H + 4) >= ((int64_t)1 << 23)) return -1;
-  const size_t plain = (size_t)(W + 4) * (W + 4) * (H + 4) * es;
-  const int want = st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
-  const size_t wide = want == LAYOUT_DUAL ? (size_t)(W + 3) * (W + 4) * (H + 4) * 2 * es
-                                          : (size_t)(W + 3) * (W + 3) * (H + 4) * 4 * es;
-  if (want != LAYOUT_PLAIN && wide < ((size_t)1 << 32)) {
-    *bytes = wide;
-    return want;
+  const char* force = getenv("MEXCT_FP_LAYOUT");
+  if (force && !strcmp(force, "plain")) {
+    *bytes = (size_t)(W + 4) * (W + 4) * (H + 4) * es;
+    return *bytes < ((size_t)1 << 31) ? LAYOUT_PLAIN : -1;
   }
-  *bytes = plain;
-  return plain < ((size_t)1 << 31) ? LAYOUT_PLAIN : -1;
+  const bool force64 = force && !strcmp(force, "wide64");
+  const int want = st == MEXCT_F32 ? LAYOUT_DUAL : LAYOUT_QUAD;
+  const size_t elems = want == LAYOUT_DUAL ? (size_t)(W + 3) * (W + 4) * (H + 4)
+                                           : (size_t)(W + 3) * (W + 3) * (H + 4);
+  *bytes = elems * (want == LAYOUT_DUAL ? 2 : 4) * es;
+  if (*bytes < ((size_t)1 << 32) && !force64) return want;
+  if (elems < ((size_t)1 << 32)) return want == LAYOUT_DUAL ? LAYOUT_DUAL64 : LAYOUT_QUAD64;
+  return -1;
 }
 
 hipError_t build_fp_volume(int layout, const void* in, int in_dt, int64_t sy, int64_t sz,
                            void* out, int st, int64_t W, int64_t H, int mode, hipStream_t s,
                            const uint32_t* scale) {
+  layout = base_layout(layout);  // the 64-bit layouts hold the same copies
   if (layout == LAYOUT_DUAL)
     return launch_pad3d(in, in_dt, sy, sz, out, st, (int)W, (int)W, (int)H, mode, s, scale);
   if (layout == LAYOUT_QUAD)
@@ -499,6 +512,61 @@ void conv_taps(const std::vector<double>& full, int W, std::vector<double>& g) {
 namespace mexct {
 hipStream_t ctx_stream(const mexct_ctx* c) { return c->stream; }
 int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+// Mirrors the slots fp_run, bp_run, prefilter_run and mexct_osc_tv_recon_dev
+// take from the context (mexct_ctx::get rounds each up to 2 MiB).
+size_t op_workspace_bytes(int op, int geom, int st, int64_t W, int64_t H, int64_t nProj, float sad,
+                          int items, int min_subsets) {
+  auto slot = [](size_t b) {
+    if (b == 0) b = 256;
+    return (b + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+  };
+  const size_t es = (size_t)store_bytes(st);
+  const size_t n = (size_t)std::max<int64_t>(nProj, 1);
+  const size_t nitems = (size_t)std::max(items, 1);
+  const size_t common = slot(4 * (64 + 2 * kMaxItems)) + std::max<size_t>(n, 1024) * sizeof(float2);
+  size_t b = common;
+  if (op == MEXCT_OP_FP) {
+    size_t lbytes = 0;
+    const int layout = fp_layout(W, H, st, &lbytes);
+    b += layout >= 0 ? slot(lbytes * nitems) : slot((size_t)(W + 4) * (W + 4) * (H + 4) * es);
+    return b;
+  }
+  if (op == MEXCT_OP_BP || op == MEXCT_OP_FDK) {
+    const double hw = (double)(W / 2 - 1);
+    const int rows = st != MEXCT_F32 && (geom != 2 || (double)sad > 2.02 * hw) ? 4 : 2;
+    b += slot((size_t)(W + 4) * (H + 3) * rows * es * n * nitems);
+    if (geom == 2 && st == MEXCT_F32) {
+      size_t cfv = 0;  // the largest chain of any tuning candidate
+      for (int sup : {2, 4, 8})
+        for (int zr : {32, 48, 64})
+          cfv = std::max(cfv, bp_chain_floats_per_view((int)W, (int)H, zr, sup));
+      if (cfv > 0) {
+        const size_t cv = std::min<size_t>(n, std::max<size_t>(1, ((size_t)1 << 28) / cfv));
+        b += slot(cv * cfv * 4);
+      }
+    }
+    if (op == MEXCT_OP_FDK)
+      b += slot(sizeof(double) * (size_t)(2 * W - 1)) +
+           (geom != GEOM_PAR ? slot(sizeof(float) * (size_t)W * H) : 0) + slot(sizeof(float) * n);
+    return b;
+  }
+  // OSC-TV: raised and zeroed pre/post, their working copies, the padded
+  // state and its previous value, the plain volume, the TV gradient, the
+  // reduction partials and scalars, the ratio-term copies of the largest
+  // subset and the z-fastest projected volume
+  const size_t psz = (size_t)W * H * n * 4, vsz = (size_t)W * W * H * 4;
+  const size_t padsz = (size_t)(W + 4) * (W + 4) * (H + 4) * 4;
+  const size_t cmax = (n + (size_t)std::max(min_subsets, 1) - 1) / (size_t)std::max(min_subsets, 1);
+  const size_t pv = (size_t)2 * (W + 4) * (H + 3);
+  const size_t pbytes = (pv * cmax * es + 3) / 4 * 4;
+  size_t lbytes = 0;
+  (void)fp_layout(W, H, st, &lbytes);
+  b += 6 * slot(psz) + 2 * slot(padsz) + 2 * slot(vsz) +
+       slot(tv_partials_count((int)W, (int)W, (int)H) * 8) + slot(256) + 2 * slot(pbytes) +
+       slot(lbytes);
+  return b;
+}
 }  // namespace mexct
 
 // =============================================================== C ABI
@@ -543,7 +611,7 @@ namespace {
 // elements), outputs one after another: one launch for all of them.
 int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t W, int64_t H,
            float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
-           const mexct_opts* opts) {
+           const mexct_opts* opts, bool loads_only = false) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims W=%lld H=%lld nProj=%d", (long long)W, (long long)H, nProj);
@@ -561,6 +629,8 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
   const int layout = fp_layout(W, H, st, &lbytes);
   const float2* sc;
   const size_t vel = (size_t)W * W * H;  // elements of one input volume
+  if (loads_only && (layout < 0 || wide64(layout)))
+    return fail(MEXCT_ENOTSUP, "no loads-only form for a volume copy past 4 GiB");
   if (layout >= 0) {
     // z-fastest haloed volumes, one wave per 64 detector rows of a column
     RC(c->get(WS_VOL, lbytes * items, &pad));
@@ -589,11 +659,12 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
     z.items = items;
     z.vol_stride = lbytes;
     z.out_stride = (uint64_t)W * H * nProj;
+    z.loads_only = loads_only;
     const TuneKey key{MEXCT_OP_FP, geom, st, W, H, nProj};
     TuneVal tv;
     if (tune_find(key, &tv)) {
       z.vchunk_log2 = tv.a;
-    } else if (want_tune(c, key)) {
+    } else if (!loads_only && want_tune(c, key)) {
       static const int cand[] = {2, 3, 4, 5};  // 4 .. 32 views per chunk
       const int best = tune_pick(c, 4, 2, [&](int i) {
         FPZLaunch t = z;
@@ -611,6 +682,7 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
     return MEXCT_OK;
   }
   if (items != 1) return fail(MEXCT_ENOTSUP, "volume too large to batch");
+  if (loads_only) return fail(MEXCT_ENOTSUP, "no loads-only form for the x-fastest projector");
   RC(c->get(WS_VOL, pbytes, &pad));
   HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
   RC(c->sincos_table(angles, nProj, &sc));
@@ -640,6 +712,12 @@ int mexct_fp_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, in
                  float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
                  const mexct_opts* opts) {
   return fp_run(c, vol, vol_dtype, 1, W, H, sad, angles, nProj, geom, out, opts);
+}
+
+int mexct_fp_loads_dev(mexct_ctx* c, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
+                       float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
+                       const mexct_opts* opts) {
+  return fp_run(c, vol, vol_dtype, 1, W, H, sad, angles, nProj, geom, out, opts, true);
 }
 
 int mexct_fp_batch_dev(mexct_ctx* c, const void* vols, int32_t vol_dtype, int32_t count,
@@ -676,7 +754,7 @@ namespace {
 // views for all of them.
 int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_t W, int64_t H,
            int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
-           const mexct_opts* opts) {
+           const mexct_opts* opts, bool loads_only = false) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
@@ -686,6 +764,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
   RC(check_padded(W, H, nProj, "sinogram"));
+  if (loads_only && geom != GEOM_CONE) return fail(MEXCT_ENOTSUP, "the loads-only BP is cone only");
   // row-grouped haloed copy: (W+4) x (H+3) elements of two values per view,
   // four for 2-byte storage when the rows of consecutive z-voxels step by
   // less than 2 (par, fan; cone with SAD > 2.02 hw, hw = W/2-1, so that
@@ -731,10 +810,11 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   a.items = items;
   a.sino_stride = sino_stride;
   a.out_stride = (uint64_t)W * W * H;
+  a.loads_only = loads_only;
   const TuneKey key{MEXCT_OP_BP, geom, st, W, H, nProj};
   TuneVal tv;
   const bool found = tune_find(key, &tv);
-  const bool tune = !found && nProj > 0 && want_tune(c, key);
+  const bool tune = !found && !loads_only && nProj > 0 && want_tune(c, key);
   // candidates: super-tile side 2, 4, 8; the z-range too for FP32 storage
   // unless the caller fixed it
   const bool zr_free =
@@ -747,8 +827,10 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
       cand.push_back({sup, a.zr});
   }
   if (found) {
+    // a z-range the caller fixed, or one in texel space (2-byte storage,
+    // whose row start depends on it), is never replaced by a tuned one
     a.super_ = tv.a;
-    a.zr = tv.b;
+    if (zr_free) a.zr = tv.b;
   }
   // chain scratch for the largest candidate (super-tile 8, z-range 32)
   const bool chain = geom == 2 && st == MEXCT_F32;
@@ -784,7 +866,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
     tv.b = cand[best].second;
     tune_store(key, tv);
     a.super_ = tv.a;
-    a.zr = tv.b;
+    if (zr_free) a.zr = tv.b;
   }
   HIPCHK(launch_bp(a, c->stream));
   return MEXCT_OK;
@@ -795,6 +877,12 @@ int mexct_bp_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, 
                  int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
                  const mexct_opts* opts) {
   return bp_run(c, sino, sino_dtype, 1, W, H, nProj, sad, angles, geom, out, opts);
+}
+
+int mexct_bp_loads_dev(mexct_ctx* c, const void* sino, int32_t sino_dtype, int64_t W, int64_t H,
+                       int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
+                       const mexct_opts* opts) {
+  return bp_run(c, sino, sino_dtype, 1, W, H, nProj, sad, angles, geom, out, opts, true);
 }
 
 int mexct_bp_batch_dev(mexct_ctx* c, const void* sinos, int32_t sino_dtype, int32_t count,
@@ -952,13 +1040,55 @@ int mexct_g3d_fp(const float* vol, int64_t side, int64_t nSlices, const float* a
 int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int64_t nProj,
                      const float* angles, const float* pts_dev, int32_t ppv, int64_t M, int64_t N,
                      float* out, const mexct_opts* opts) {
-  (void)opts;
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   if (W < 1 || H < 1 || nProj < 0 || M < 1 || N < 1) return fail(MEXCT_EINVAL, "bad dims");
   if (!out || (nProj > 0 && (!sino || !angles || !pts_dev))) return fail(MEXCT_EINVAL, "null array argument");
+  const int mode = opts ? opts->g3d_bp : MEXCT_G3D_BRICKS;
+  if (mode < MEXCT_G3D_BRICKS || mode > MEXCT_G3D_ATOMICS)
+    return fail(MEXCT_EINVAL, "unknown general-3D BP mode %d", mode);
+  if ((double)M * M * N >= 2147483648.0 || (double)W * H * nProj >= 4294967296.0)
+    return fail(MEXCT_EINVAL, "general-3D BP too large (M=%lld N=%lld)", (long long)M, (long long)N);
   DevGuard g(c->device);
   if (nProj == 0) {
     HIPCHK(launch_fill(out, 0.f, (size_t)M * M * N, c->stream));
+    return MEXCT_OK;
+  }
+  const G3DBrickPlan plan = g3d_brick_plan((int)W, (int)H, (int)M, (int)N, (int)nProj);
+  if (mode == MEXCT_G3D_DETERMINISTIC && !plan.ok)
+    return fail(MEXCT_ENOTSUP, "deterministic general-3D BP: %u bricks exceed the binning LDS",
+                plan.nbricks);
+  if (mode != MEXCT_G3D_ATOMICS && plan.ok) {
+    const bool det = mode == MEXCT_G3D_DETERMINISTIC;
+    G3DBrickLaunch L;
+    L.plan = plan;
+    L.sino = sino;
+    L.pts = pts_dev;
+    L.pts_view_stride = ppv ? (int64_t)6 * W * H : 0;
+    RC(c->sincos_table(angles, (int)nProj, &L.sc));
+    L.W = (int)W;
+    L.H = (int)H;
+    L.nviews = (int)nProj;
+    L.M = (int)M;
+    L.N = (int)N;
+    void *ck, *mk, *hl, *acc = nullptr;
+    RC(c->get(WS_G3D_CK, plan.ck_bytes, &ck));
+    RC(c->get(WS_G3D_MASK, plan.mask_bytes, &mk));
+    RC(c->get(WS_G3D_HALO, plan.halo_elems * (det ? 8 : 4), &hl));
+    if (det) RC(c->get(WS_G3D_ACC, (size_t)M * M * N * 8, &acc));
+    L.mask = (uint32_t*)mk;
+    L.ck = (float*)ck;
+    L.out = out;
+    L.deterministic = det;
+    L.halo = det ? nullptr : (float*)hl;
+    L.halo64 = det ? (int64_t*)hl : nullptr;
+    L.out64 = (int64_t*)acc;
+    L.pmax = nullptr;
+    if (det) {
+      uint32_t* pm;
+      RC(max_bits(c, sino, (size_t)W * H * nProj, 2, &pm));
+      L.pmax = pm;
+    }
+    HIPCHK(launch_g3d_bp_bricks(L, c->stream));
     return MEXCT_OK;
   }
   void* acc;
@@ -1035,9 +1165,11 @@ int osc_subsets(mexct_ctx* c, float* mu_pad, int W, int H, float* pre, float* po
   void* mz;
   const int64_t MPX = W + 4, MPXY = (int64_t)(W + 4) * (W + 4);
   size_t lbytes = 0;
-  const int layout = fp_layout(W, H, st, &lbytes);  // PLAIN at least: the state is < 2 GiB
+  const int layout = fp_layout(W, H, st, &lbytes);
   if (layout < 0)
-    return fail(MEXCT_EINVAL, "OSC: (W+4)*(H+4) = %lld must stay below 2^23",
+    return fail(MEXCT_EINVAL,
+                "OSC: the projected volume copy needs (W+4)*(H+4) = %lld below 2^23 and fewer "
+                "than 2^32 elements",
                 (long long)(W + 4) * (H + 4));
   RC(c->get(WS_MUZ, lbytes, &mz));
   const float* interior = mu_pad + 2 + 2 * MPX + 2 * MPXY;
@@ -1207,18 +1339,15 @@ bool parker_on(double q) { return q >= 0.0; }  // negative or NaN: the 5-argumen
 // on the device.  alpha runs along the detector columns (the fan direction);
 // the reference takes size(sino,2) entries and expands with repmat (:32,:73),
 // the same window when W == H.
-int parker_window(mexct_ctx* c, int64_t W, int64_t nProj, int64_t n_all, int geom, double sad,
-                  const float* angles, double q, const float** win) {
-  const size_t col_b = (size_t)W * sizeof(float4);
-  const size_t beta_b = ((size_t)nProj * sizeof(float) + 255) & ~(size_t)255;
-  void* base;
-  RC(c->get(WS_PARKER, col_b + beta_b + (size_t)W * nProj * sizeof(float), &base));
-  float* w = (float*)((char*)base + col_b + beta_b);
-  *win = w;
-  const bool same = c->pk_W == W && c->pk_n == nProj && c->pk_geom == geom && c->pk_sad == sad &&
-                    c->pk_q == q && c->pk_angles.size() == (size_t)n_all &&
-                    memcmp(c->pk_angles.data(), angles, (size_t)n_all * sizeof(float)) == 0;
-  if (same) return MEXCT_OK;
+size_t parker_col_bytes(int64_t W) { return (size_t)W * sizeof(float4); }
+size_t parker_beta_bytes(int64_t nProj) { return ((size_t)nProj * sizeof(float) + 255) & ~(size_t)255; }
+
+// Host part of the window: the per-column terms and per-view betas, laid
+// out as parker_col_bytes(W) of float4 then parker_beta_bytes(nProj) of
+// float in h, and the scan constants k.
+void parker_tables(int64_t W, int64_t nProj, int64_t n_all, int geom, double sad,
+                   const float* angles, double q, std::vector<char>& h, ParkerConsts& k) {
+  const size_t col_b = parker_col_bytes(W), beta_b = parker_beta_bytes(nProj);
   std::vector<double> alpha((size_t)W, 0.0);
   if (geom != GEOM_PAR)
     for (int64_t u = 0; u < W; ++u) alpha[u] = atan((-(double)W / 2 + 0.5 + (double)u) / sad);
@@ -1229,18 +1358,33 @@ int parker_window(mexct_ctx* c, int64_t W, int64_t nProj, int64_t n_all, int geo
     for (double& a : alpha) a = -a;
   const double delta = fabs(alpha[W - 1] - alpha[0]) / 2;
   const float extent = fabsf(angles[n_all - 1] - angles[0]);
-  ParkerConsts k;
   k.pi_f = (float)M_PI;
   k.d2 = (float)(2 * delta);
   k.eps = std::max(extent - (float)(M_PI + 2 * delta), 0.0f);
   const float qf = (float)q;
-  std::vector<char> h(col_b + beta_b, 0);
+  h.assign(col_b + beta_b, 0);
   float4* col = (float4*)h.data();
   for (int64_t u = 0; u < W; ++u)
     col[u] = make_float4((float)(2 * alpha[u]), qf * ((float)(2 * delta - 2 * alpha[u]) + k.eps),
                          qf * ((float)(2 * delta + 2 * alpha[u]) + k.eps), 0.f);
   float* beta = (float*)(h.data() + col_b);
   for (int64_t i = 0; i < nProj; ++i) beta[i] = fabsf(angles[i] - angles[0]);
+}
+
+int parker_window(mexct_ctx* c, int64_t W, int64_t nProj, int64_t n_all, int geom, double sad,
+                  const float* angles, double q, const float** win) {
+  const size_t col_b = parker_col_bytes(W), beta_b = parker_beta_bytes(nProj);
+  void* base;
+  RC(c->get(WS_PARKER, col_b + beta_b + (size_t)W * nProj * sizeof(float), &base));
+  float* w = (float*)((char*)base + col_b + beta_b);
+  *win = w;
+  const bool same = c->pk_W == W && c->pk_n == nProj && c->pk_geom == geom && c->pk_sad == sad &&
+                    c->pk_q == q && c->pk_angles.size() == (size_t)n_all &&
+                    memcmp(c->pk_angles.data(), angles, (size_t)n_all * sizeof(float)) == 0;
+  if (same) return MEXCT_OK;
+  std::vector<char> h;
+  ParkerConsts k;
+  parker_tables(W, nProj, n_all, geom, sad, angles, q, h, k);
   c->pk_W = -1;  // invalid until the kernel is queued
   RC(c->upload(base, h.data(), h.size()));
   HIPCHK(launch_parker((const float4*)base, (const float*)((char*)base + col_b), k, (int)W,
@@ -1252,6 +1396,27 @@ int parker_window(mexct_ctx* c, int64_t W, int64_t nProj, int64_t n_all, int geo
   c->pk_sad = sad;
   c->pk_q = q;
   return MEXCT_OK;
+}
+
+// Cosine weights Distance = SAD ./ sqrt(SAD^2 + U.^2 [+ V.^2]) with
+// U = (1..W) - (W-1)/2 (preFilterSinogram.m:40-42, :54-57), rounded to
+// single, as w[u + W*v] (fan and cone).
+void cosine_weights(int64_t W, int64_t H, int geom, double sad, std::vector<float>& w) {
+  w.resize((size_t)W * H);
+  for (int64_t v = 0; v < H; ++v)
+    for (int64_t u = 0; u < W; ++u) {
+      const double U = (double)(u + 1) - (double)(W - 1) / 2.0;
+      const double V = (double)(v + 1) - (double)(H - 1) / 2.0;
+      const double D = geom == GEOM_CONE ? sad / sqrt(sad * sad + U * U + V * V)
+                                         : sad / sqrt(sad * sad + U * U);
+      w[(size_t)u + (size_t)W * v] = (float)D;
+    }
+}
+
+// (angles(i) - angles(i-1)) in single; view 1 takes the next step (:80-84).
+void angle_steps(int64_t nProj, const float* angles, std::vector<float>& dA) {
+  dA.resize((size_t)nProj);
+  for (int64_t i = 0; i < nProj; ++i) dA[i] = i > 0 ? angles[i] - angles[i - 1] : angles[1] - angles[0];
 }
 
 int check_prefilter_args(int32_t geom, int64_t W, int64_t H, int64_t nProj, int32_t window) {
@@ -1285,17 +1450,8 @@ int prefilter_run(mexct_ctx* c, FilterIn in, int64_t W, int64_t H, int64_t nProj
     void* wp;
     RC(c->get(WS_WGT, sizeof(float) * W * H, &wp));
     if (c->w_W != W || c->w_H != H || c->w_geom != geom || c->w_sad != sad) {
-      // Distance = SAD ./ sqrt(SAD^2 + U.^2 [+ V.^2]), U = (1..W) - (W-1)/2
-      // (preFilterSinogram.m:40-42, :54-57), rounded to single.
-      std::vector<float> w((size_t)W * H);
-      for (int64_t v = 0; v < H; ++v)
-        for (int64_t u = 0; u < W; ++u) {
-          const double U = (double)(u + 1) - (double)(W - 1) / 2.0;
-          const double V = (double)(v + 1) - (double)(H - 1) / 2.0;
-          const double D = geom == GEOM_CONE ? sad / sqrt(sad * sad + U * U + V * V)
-                                             : sad / sqrt(sad * sad + U * U);
-          w[(size_t)u + (size_t)W * v] = (float)D;
-        }
+      std::vector<float> w;
+      cosine_weights(W, H, geom, sad, w);
       RC(c->upload(wp, w.data(), w.size() * sizeof(float)));
       c->w_W = (int)W;
       c->w_H = (int)H;
@@ -1306,9 +1462,8 @@ int prefilter_run(mexct_ctx* c, FilterIn in, int64_t W, int64_t H, int64_t nProj
   }
   in.pw = nullptr;
   if (parker_on(parker_q)) RC(parker_window(c, W, nProj, n_all, geom, sad, angles, parker_q, &in.pw));
-  // (angles(i) - angles(i-1)) in single, view 1 takes the next step (:80-84)
-  std::vector<float> dA((size_t)nProj);
-  for (int64_t i = 0; i < nProj; ++i) dA[i] = i > 0 ? angles[i] - angles[i - 1] : angles[1] - angles[0];
+  std::vector<float> dA;
+  angle_steps(nProj, angles, dA);
   const bool have = c->ws[WS_DA].p && c->dA_key.size() == dA.size() &&
                     memcmp(c->dA_key.data(), dA.data(), dA.size() * sizeof(float)) == 0;
   RC(c->get(WS_DA, sizeof(float) * nProj, &dp));
@@ -1522,16 +1677,32 @@ __global__ void clamp_min_kernel(const float* __restrict__ in, float* __restrict
 }
 }  // namespace
 
-int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, int64_t W,
-                           int64_t H, int64_t nProj, const float* angles, float sad, int32_t geom,
-                           int32_t nIter, float tv, int32_t sInit, int32_t sFinal, double p,
-                           const int32_t* first_picks, float* mu_out, const mexct_opts* opts) {
-  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+namespace {
+int osc_tv_args(int32_t geom, int64_t W, int64_t H, int64_t nProj, int32_t nIter, int32_t k_begin,
+                 int32_t k_end) {
   RC(check_geom(geom));
   if (W < 2 || H < 1 || nProj < 1) return fail(MEXCT_EINVAL, "bad dims");
   if (nIter < 2) return fail(MEXCT_EINVAL, "nIter must be >= 2 (calcNumSubsets divides by (nIter-1)^p)");
-  if (!pre || !post || !angles || !first_picks || !mu_out) return fail(MEXCT_EINVAL, "null array argument");
+  if (k_begin < 1 || k_end < k_begin || k_end > nIter)
+    return fail(MEXCT_EINVAL, "iterations %d..%d outside 1..%d", k_begin, k_end, nIter);
   RC(check_padded(W, W, H, "volume"));
+  return MEXCT_OK;
+}
+
+// Iterations k_begin..k_end of the nIter-iteration OSC_TV_recon.m loop on
+// the device.  The state between iterations is mu alone (pre/post are
+// re-derived from the inputs each iteration, the subset order from
+// first_picks), so mu_in = the volume after iteration k_begin - 1 resumes
+// the run exactly; k_begin = 1 starts from 1e-10 and ignores mu_in.
+int osc_tv_run(mexct_ctx* c, const float* pre, const float* post, int64_t W, int64_t H,
+               int64_t nProj, const float* angles, float sad, int32_t geom, int32_t nIter, float tv,
+               int32_t sInit, int32_t sFinal, double p, const int32_t* first_picks,
+               int32_t k_begin, int32_t k_end, const float* mu_in, float* mu_out,
+               const mexct_opts* opts) {
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  RC(osc_tv_args(geom, W, H, nProj, nIter, k_begin, k_end));
+  if (!pre || !post || !angles || !first_picks || !mu_out || (k_begin > 1 && !mu_in))
+    return fail(MEXCT_EINVAL, "null array argument");
   const int st_dt = storage_of(opts);
   RC(check_dtype(st_dt, "storage"));
   DevGuard g(c->device);
@@ -1554,9 +1725,13 @@ int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, in
   clamp_min_kernel<<<cg, 256, 0, st>>>(pre, (float*)pre0, (float*)pre2, psz);
   clamp_min_kernel<<<cg, 256, 0, st>>>(post, (float*)post0, (float*)post2, psz);
   HIPCHK(hipGetLastError());
-  // mu = ones(W,W,H,'single') .* 1e-10
-  HIPCHK(launch_fill((float*)plain, (float)(1.0 * 1e-10), vsz, st));
-  HIPCHK(launch_pad3(plain, MEXCT_F32, mp, MEXCT_F32, (int)W, (int)W, (int)H, ADDR_BORDER, st));
+  if (k_begin == 1) {
+    // mu = ones(W,W,H,'single') .* 1e-10
+    HIPCHK(launch_fill((float*)plain, (float)(1.0 * 1e-10), vsz, st));
+    HIPCHK(launch_pad3(plain, MEXCT_F32, mp, MEXCT_F32, (int)W, (int)W, (int)H, ADDR_BORDER, st));
+  } else {
+    HIPCHK(launch_pad3(mu_in, MEXCT_F32, mp, MEXCT_F32, (int)W, (int)W, (int)H, ADDR_BORDER, st));
+  }
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   const int64_t MPX = W + 4, MPXY = (W + 4) * (W + 4);
@@ -1569,7 +1744,7 @@ int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, in
   const int steps = tv_steps_of(opts);
   void* gp;
   RC(c->get(WS_TVG, vsz * 4, &gp));
-  for (int k = 1; k <= nIter; ++k) {
+  for (int k = k_begin; k <= k_end; ++k) {
     HIPCHK(hipMemcpyAsync(mo, mp, padsz * 4, hipMemcpyDeviceToDevice, st));
     const int n = mexct_calc_num_subsets(sInit, sFinal, p, nIter, k);
     if (n < 1) return fail(MEXCT_EINVAL, "iteration %d has %d subsets", k, n);
@@ -1591,6 +1766,51 @@ int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, in
     }
   }
   HIPCHK(launch_unpad3((const float*)mp, mu_out, (int)W, (int)W, (int)H, st));
+  return MEXCT_OK;
+}
+}  // namespace
+
+int mexct_osc_tv_recon_dev(mexct_ctx* c, const float* pre, const float* post, int64_t W,
+                           int64_t H, int64_t nProj, const float* angles, float sad, int32_t geom,
+                           int32_t nIter, float tv, int32_t sInit, int32_t sFinal, double p,
+                           const int32_t* first_picks, float* mu_out, const mexct_opts* opts) {
+  return osc_tv_run(c, pre, post, W, H, nProj, angles, sad, geom, nIter, tv, sInit, sFinal, p,
+                    first_picks, 1, std::max<int32_t>(nIter, 1), nullptr, mu_out, opts);
+}
+
+int mexct_osc_tv_iterations_dev(mexct_ctx* c, const float* pre, const float* post, int64_t W,
+                                int64_t H, int64_t nProj, const float* angles, float sad,
+                                int32_t geom, int32_t nIter, float tv, int32_t sInit,
+                                int32_t sFinal, double p, const int32_t* first_picks,
+                                int32_t k_begin, int32_t k_end, float* mu, const mexct_opts* opts) {
+  return osc_tv_run(c, pre, post, W, H, nProj, angles, sad, geom, nIter, tv, sInit, sFinal, p,
+                    first_picks, k_begin, k_end, mu, mu, opts);
+}
+
+int mexct_osc_tv_iterations(const float* pre, const float* post, int64_t W, int64_t H,
+                            int64_t nProj, const float* angles, int64_t n_angles, float sad,
+                            int32_t geom, int32_t nIter, float tv, int32_t sInit, int32_t sFinal,
+                            double p, const int32_t* first_picks, int32_t k_begin, int32_t k_end,
+                            float* mu, const mexct_opts* opts) {
+  if (n_angles < nProj) return fail(MEXCT_EINVAL, "nProj exceeds the number of angles");
+  RC(osc_tv_args(geom, W, H, nProj, nIter, k_begin, k_end));
+  if (!pre || !post || !mu || !angles || !first_picks) return fail(MEXCT_EINVAL, "null array argument");
+  mexct_ctx* c;
+  RC(default_ctx(&c));
+  DevGuard g(c->device);
+  const size_t pb = (size_t)W * H * nProj * 4, vb = (size_t)W * W * H * 4;
+  void *dpre, *dpost, *dmu;
+  RC(c->get(WS_HOST_A, pb, &dpre));
+  RC(c->get(WS_HOST_B, pb, &dpost));
+  RC(c->get(WS_HOST_OUT, vb, &dmu));
+  HIPCHK(hipMemcpyAsync(dpre, pre, pb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dpost, post, pb, hipMemcpyHostToDevice, c->stream));
+  if (k_begin > 1) HIPCHK(hipMemcpyAsync(dmu, mu, vb, hipMemcpyHostToDevice, c->stream));
+  RC(mexct_osc_tv_iterations_dev(c, (const float*)dpre, (const float*)dpost, W, H, nProj, angles,
+                                 sad, geom, nIter, tv, sInit, sFinal, p, first_picks, k_begin,
+                                 k_end, (float*)dmu, opts));
+  HIPCHK(hipMemcpyAsync(mu, dmu, vb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return MEXCT_OK;
 }
 

@@ -76,6 +76,7 @@ struct BPKArgs {
   uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
   uint64_t sino_stride;  // bytes between the items' haloed sinograms
   uint64_t out_stride;   // floats between the items' volumes
+  int loads_only;        // launch the loads-only form
 };
 
 // Voxel column (ix, iy) of lane `lane` of tile `tile`: 8x8 tiles ordered in
@@ -152,7 +153,17 @@ __global__ __launch_bounds__(256) void bp_chain_kernel(BPKArgs a, float* chain) 
 
 // SMALL: a view's haloed slice is below 2^24 bytes, so lane offsets are
 // exact in float and cost one FMA.
-template <int GEOM, int ZR, typename S, bool SMALL, int RQ>
+// Bits of one loaded tap group, for the loads-only form.
+__device__ __forceinline__ uint32_t load_bits(const float4& q) {
+  return __float_as_uint(q.x) ^ __float_as_uint(q.y) ^ __float_as_uint(q.z) ^ __float_as_uint(q.w);
+}
+__device__ __forceinline__ uint32_t load_bits(const u2v& q) { return q.x ^ q.y; }
+__device__ __forceinline__ uint32_t load_bits(const u4v& q) { return q.x ^ q.y ^ q.z ^ q.w; }
+
+// LOADS: the loads-only form (the backprojector's gather ceiling): the same
+// blocks, offsets and loads, each load's bits XOR-folded instead of
+// interpolated; every voxel of the range receives the fold.
+template <int GEOM, int ZR, typename S, bool SMALL, int RQ, bool LOADS = false>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const uint32_t b0 = xcd_remap(blockIdx.x, gridDim.x);
   if (b0 >= a.n_logical) return;
@@ -185,6 +196,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   float acc[ZR];
 #pragma unroll
   for (int m = 0; m < ZR; ++m) acc[m] = 0.f;
+  uint32_t fold = 0;
   const int sexp = pow2_exp(a.sino_max ? a.sino_max + item : nullptr);
   if (a.accumulate) {
     const float* q = o;
@@ -249,6 +261,10 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
         for (int g = 0; g < GP; ++g) {
           // q.x, q.y: rows j0..j0+3 of column i0; q.z, q.w: of column i0+1
           const u4v v = q[b][g];
+          if constexpr (LOADS) {
+            fold ^= load_bits(v);
+            continue;
+          }
           const uint32_t A1 = (uint32_t)((((uint64_t)v.y << 32) | v.x) >> sh[b][g]);
           const uint32_t B1 = (uint32_t)((((uint64_t)v.w << 32) | v.z) >> sh[b][g]);
           float c0, c1, d0, d1;
@@ -303,6 +319,10 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           float c0, c1;
+          if constexpr (LOADS) {
+            fold ^= load_bits(q[g]);
+            continue;
+          }
           if constexpr (__is_same(S, _Float16) && kPackedF16Lerp) {
             acc[m0 + g] =
                 __builtin_fmaf(UU, lerp_x(pk_lerp_h(q[g].x, q[g].y, pk_h2(ax)), az[g]), acc[m0 + g]);
@@ -346,6 +366,10 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         float c0, c1;  // rows j0 and j0+1, lerped between columns i0 and i0+1
+        if constexpr (LOADS) {
+          fold ^= load_bits(q[g]);
+          continue;
+        }
         if constexpr (__is_same(S, _Float16)) {
           c0 = lerp_h<0>(q[g].x, q[g].y, ax);
           c1 = lerp_h<1>(q[g].x, q[g].y, ax);
@@ -360,7 +384,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   float* q = o;
 #pragma unroll
   for (int m = 0; m < ZR; ++m) {
-    if (m < zc) *q = ldexpf(acc[m], sexp);
+    if (m < zc) *q = LOADS ? __uint_as_float(fold) : ldexpf(acc[m], sexp);
     q += WW;
   }
 }
@@ -370,6 +394,10 @@ static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u;
 template <int ZR, typename S, bool SMALL, int RQ>
 static void bp_dispatch_g(const BPKArgs& k, int geom, hipStream_t st) {
   const unsigned g = padded_grid(k.n_logical);
+  if (k.loads_only) {  // cone only (the host checks)
+    bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ, true><<<g, 64, 0, st>>>(k);
+    return;
+  }
   switch (geom) {
     case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
     case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
@@ -433,6 +461,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.ncols = tpad * tpad * 64u;
   k.sino_stride = a.sino_stride;
   k.out_stride = a.out_stride;
+  k.loads_only = a.loads_only ? 1 : 0;
+  if (a.loads_only && a.geom != GEOM_CONE) return hipErrorInvalidValue;
   if (k.n_logical == 0) return hipSuccess;
   const size_t view_bytes = (size_t)k.PV * es2;
   int chunk = (int)std::max<size_t>(1, (((size_t)1 << 31) - 1) / view_bytes);
@@ -701,6 +731,10 @@ struct G3DBPKArgs {
   int W, H, M, N;
   float bx, by, bz;
   uint32_t tiles_x, tiles_per_view, n_logical;
+  int n_min;       // only rays of more samples
+  int xfast;       // out is the [M,M,N] x-fastest volume
+  int64_t* out64;  // fixed-point sums (x-fastest) instead of out
+  const uint32_t* pmax;
 };
 
 __device__ __forceinline__ int floor_index(float v) {
@@ -785,8 +819,20 @@ __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
     const float q = ceilf(distance / stepsize);
     n = q < 16777216.f ? (int)q : 16777216;
   }
+  if (n <= a.n_min) return;
   const float pval = a.sino[(size_t)view * npts + (size_t)iy * a.W + ix];
   const unsigned M = (unsigned)a.M, N = (unsigned)a.N;
+  const size_t hstep = a.xfast ? (size_t)a.M * a.M : 1;  // from corner kL to kL + 1
+  const float scale = a.out64 ? g3d_fixed_scale(a.pmax) : 1.f;  // the bricks' fixed point
+  auto emit = [&](size_t at, float lo, float hi) {
+    if (a.out64) {
+      atomicAdd((unsigned long long*)(a.out64 + at), g3d_fixed(lo * scale));
+      if (hi != 0.f) atomicAdd((unsigned long long*)(a.out64 + at + hstep), g3d_fixed(hi * scale));
+    } else {
+      atomicAdd(a.out + at, lo);
+      if (hi != 0.f) atomicAdd(a.out + at + hstep, hi);
+    }
+  };
   // Consecutive 0.5-voxel steps often land in the same cell: their weights
   // are summed in registers and scattered once (the scatter order is not
   // fixed anyway, as with the reference's atomicAdd).  pend is the index of
@@ -810,15 +856,13 @@ __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
         lo = lo + hi;
         hi = 0.f;
       }
-      const size_t idx = ((size_t)iL * a.M + (size_t)jL) * N + (size_t)kL;
+      const size_t idx = a.xfast ? ((size_t)kL * a.M + (size_t)jL) * a.M + (size_t)iL
+                                 : ((size_t)iL * a.M + (size_t)jL) * N + (size_t)kL;
       if (idx == pend) {
         wlo = wlo + lo;
         whi = whi + hi;
       } else {
-        if (pend != ~(size_t)0) {
-          atomicAdd(a.out + pend, wlo);
-          if (whi != 0.f) atomicAdd(a.out + pend + 1, whi);
-        }
+        if (pend != ~(size_t)0) emit(pend, wlo, whi);
         pend = idx;
         wlo = lo;
         whi = hi;
@@ -828,10 +872,7 @@ __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
     cur.y = cur.y + sY;
     cur.z = cur.z + sZ;
   }
-  if (pend != ~(size_t)0) {
-    atomicAdd(a.out + pend, wlo);
-    if (whi != 0.f) atomicAdd(a.out + pend + 1, whi);
-  }
+  if (pend != ~(size_t)0) emit(pend, wlo, whi);
 }
 
 hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st) {
@@ -851,6 +892,10 @@ hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st) {
   k.tiles_x = (uint32_t)((a.W + 3) / 4);
   k.tiles_per_view = k.tiles_x * (uint32_t)((a.H + 63) / 64);
   k.n_logical = k.tiles_per_view * (uint32_t)a.nviews;
+  k.n_min = a.n_min;
+  k.xfast = a.xfast ? 1 : 0;
+  k.out64 = a.out64;
+  k.pmax = a.pmax;
   if (k.n_logical == 0) return hipSuccess;
   g3d_bp_kernel<<<padded_grid(k.n_logical), 256, 0, st>>>(k);
   return hipGetLastError();

@@ -171,6 +171,21 @@ __device__ __forceinline__ uint32_t pk_lerp_h(uint32_t A, uint32_t B, uint32_t t
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Fixed point of the general-3D BP's deterministic sums.  A sample corner is
+// at most |P| * 0.5 * 1.5^3 < 2^(e+1) for max|P| < 2^e (pmax: the bits of
+// max|P|), and a run of samples merged in one cell holds at most 4 of them
+// (a unit cell is crossed in under 4 half-voxel steps), so with scale
+// 2^(27-e) a merged corner rounds to an int32, and 2^32 of them sum in an
+// int64 without overflow.
+__device__ __forceinline__ float g3d_fixed_scale(const uint32_t* pmax) {
+  int e = 0;
+  (void)frexpf(__uint_as_float(*pmax), &e);
+  return ldexpf(1.f, 27 - e);
+}
+__device__ __forceinline__ unsigned long long g3d_fixed(float v) {  // v: a scaled corner
+  return (unsigned long long)(long long)(int)rintf(v);
+}
+
 // float threshold equivalent to the reference's `fabsf(d) >= 1e-5`
 // (a float compared with the double literal): the least float >= 1e-5.
 __device__ __forceinline__ float parallel_threshold() { return __uint_as_float(0x3727c5adu); }

@@ -154,10 +154,13 @@ __device__ __forceinline__ float4 dual_ld(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // CLAMP false (2-byte storage only): the caller has shown that every sample
 // lies where the 2-texel halo serves it, floor(t) in [-2, N], so the clamp
 // would change nothing.
+typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
 template <typename S, int LAYOUT, bool CLAMP = true>
 __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a,
                                            const R3& cur, float fnx, float fny, float fnz,
-                                           int base) {
+                                           int base, const char* vb) {
   Tap8<S> t;
   int i, j, kk;
   if constexpr (sizeof(S) == 2) {
@@ -185,9 +188,24 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
     kk = (int)__builtin_amdgcn_fmed3f(fz, -2.f, fnz);
   }
   // 24-bit multiplies: |i|, |j| <= N + 2 and PZ, PY * PZ < 2^23 (checked by
-  // the host); the sums are exact in 32 bits (the layout is below 4 GiB)
-  const uint32_t e = (uint32_t)(mad_i24(i, a.PYPZ, mad_i24(j, a.PZ, kk)) + base);
-  if constexpr (LAYOUT == LAYOUT_QUAD) {
+  // the host).  The element index is below 2^32 (checked by the host), so
+  // the sums, taken modulo 2^32, are exact.
+  const uint32_t e = (uint32_t)mad_i24(i, a.PYPZ, mad_i24(j, a.PZ, kk)) + (uint32_t)base;
+  if constexpr (LAYOUT == LAYOUT_DUAL64) {  // f32, elements of 8 bytes (e < 2^32)
+    const char* p = vb + ((uint64_t)e << 3);
+    const f4a4 q0 = *(const f4a4*)p;              // row j
+    const f4a4 q1 = *(const f4a4*)(p + a.off_j);  // row j+1
+    t.p00 = make_float2(q0.x, q0.z);
+    t.p10 = make_float2(q0.y, q0.w);
+    t.p01 = make_float2(q1.x, q1.z);
+    t.p11 = make_float2(q1.y, q1.w);
+  } else if constexpr (LAYOUT == LAYOUT_QUAD64) {  // 2-byte storage, elements of 8 bytes
+    const u4a4 q = *(const u4a4*)(vb + ((uint64_t)e << 3));
+    t.w0 = q.x;
+    t.w1 = q.y;
+    t.w2 = q.z;
+    t.w3 = q.w;
+  } else if constexpr (LAYOUT == LAYOUT_QUAD) {
     quad_ld<S>(r, e * (uint32_t)(4 * sizeof(S)), t);
   } else if constexpr (LAYOUT == LAYOUT_DUAL && __is_same(S, _Float16)) {
     typedef uint32_t u2v __attribute__((ext_vector_type(2)));
@@ -226,8 +244,9 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT_>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
+  constexpr int LAYOUT = base_layout(LAYOUT_);
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
   if constexpr (__is_same(S, bf16_t)) {
     if constexpr (LAYOUT == LAYOUT_QUAD) {  // w0 {(i,j), (i,j+1)}, w1 {(i+1,j), (i+1,j+1)}; w2, w3 at z+1
@@ -306,25 +325,25 @@ __device__ __forceinline__ bool in_halo(const R3& c, const R3& inc, int n, float
 template <typename S, int STEP2, int LAYOUT, bool COMP, bool CLAMP>
 __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                             const R3& inc, int n, float fnx, float fny, float fnz,
-                                            int base) {
+                                            int base, const char* vb) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
   float dval = 0.f, comp = 0.f;
   // two register sets in ping-pong: while one sample is interpolated the
   // next sample's loads are in flight
-  Tap8<S> t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+  Tap8<S> t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
   int k = 0;
   for (; k + 1 < n; k += 2) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    const Tap8<S> t1 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    const Tap8<S> t1 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
     __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
     float x = interp_z<S, LAYOUT>(t0) * step;
     if constexpr (!COMP) dval = dval + x;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (COMP) {
       x = x + interp_z<S, LAYOUT>(t1) * step;
@@ -348,25 +367,41 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
   return dval;
 }
 
+// The bits of every tap of a sample folded by XOR: the loads-only form of
+// the projector keeps each load (all of its dwords are used) and drops the
+// interpolation.
+__device__ __forceinline__ uint32_t tap_bits(const Tap8<float>& t) {
+  return __float_as_uint(t.p00.x) ^ __float_as_uint(t.p00.y) ^ __float_as_uint(t.p01.x) ^
+         __float_as_uint(t.p01.y) ^ __float_as_uint(t.p10.x) ^ __float_as_uint(t.p10.y) ^
+         __float_as_uint(t.p11.x) ^ __float_as_uint(t.p11.y);
+}
+template <typename S>
+__device__ __forceinline__ uint32_t tap_bits(const Tap8<S>& t) {
+  return t.w0 ^ t.w1 ^ t.w2 ^ t.w3;
+}
+
 // The plain sample loop with the loads of the next D samples in flight
 // while one is interpolated (D + 1 register sets in rotation): the same
 // samples added in the same order as march_loop's.  Fetches up to D samples
 // past the end read range-checked buffer offsets (or, for the plain layout,
-// clamped indices) and are discarded.
-template <typename S, int STEP2, int LAYOUT, bool CLAMP, int D>
+// clamped indices) and are discarded.  LOADS: the loads-only form (the
+// projector's gather ceiling): every address and load of the real loop, the
+// taps XOR-folded instead of interpolated; returns the fold's bits.
+template <typename S, int STEP2, int LAYOUT, bool CLAMP, int D, bool LOADS = false>
 __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                              const R3& inc, int n, float fnx, float fny,
-                                             float fnz, int base) {
+                                             float fnz, int base, const char* vb) {
   constexpr float step = STEP2 ? 0.5f : 1.0f;
   float dval = 0.f;
+  uint32_t fold = 0;
   Tap8<S> t[D + 1];  // t[u] holds sample k + u at the top of each trip
-  t[0] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+  t[0] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
 #pragma unroll
   for (int d = 1; d < D; ++d) {
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
     cur.z = cur.z + inc.z;
-    t[d] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+    t[d] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
   }
   int k = 0;
   for (; k + D < n; k += D + 1) {
@@ -375,15 +410,23 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
       cur.x = cur.x + inc.x;
       cur.y = cur.y + inc.y;
       cur.z = cur.z + inc.z;
-      t[(u + D) % (D + 1)] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base);
+      t[(u + D) % (D + 1)] = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
       __builtin_amdgcn_sched_barrier(0);  // sample k+u+D's loads stay ahead
-      dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
+      if constexpr (LOADS)
+        fold ^= tap_bits(t[u]);
+      else
+        dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
     }
   }
 #pragma unroll
   for (int u = 0; u < D; ++u)
-    if (k + u < n) dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
-  return dval;
+    if (k + u < n) {
+      if constexpr (LOADS)
+        fold ^= tap_bits(t[u]);
+      else
+        dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
+    }
+  return LOADS ? __uint_as_float(fold) : dval;
 }
 
 // Samples in flight in the plain loop.  At 384^3 x 512 cone, f32: 32.3 ms
@@ -395,9 +438,12 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
 template <typename S, int EPI>
 constexpr int fetch_ahead() { return EPI == 1 ? 4 : sizeof(S) == 2 ? 3 : 7; }
 
+// EPI: 0 store the ray sum, 1 the OSC ratio terms, 2 the loads-only form.
+constexpr bool loads_only(int epi) { return epi == 2; }
+
 template <typename S, int STEP2, int LAYOUT, int EPI>
 __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
-                                         const R3& inc, int n) {
+                                         const R3& inc, int n, const char* vb) {
   const float fnx = (float)a.NX, fny = (float)a.NY, fnz = (float)a.NZ;
   const int base = 2 * a.PYPZ + 2 * a.PZ + 2;
   if (n <= 0) return 0.f;
@@ -409,18 +455,21 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     // the index clamps when every ray stays inside the halo
     const bool comp = __builtin_amdgcn_ballot_w64(n > a.long_ray) != 0;
     // the plain layout reads its neighbours through scalar offsets, which the
-    // buffer range check does not cover: its indices are always clamped
-    const bool clamp = LAYOUT == LAYOUT_PLAIN ||
+    // buffer range check does not cover, and the 64-bit layouts have no range
+    // check at all (loads run past the ray's end): their indices are always
+    // clamped
+    const bool clamp = LAYOUT == LAYOUT_PLAIN || wide64(LAYOUT) ||
                        __builtin_amdgcn_ballot_w64(!in_halo(cur, inc, n, fnx, fny, fnz)) != 0;
-    if (comp)
-      return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base)
-                   : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base);
-    return clamp ? march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny,
-                                                                     fnz, base)
-                 : march_loopd<S, STEP2, LAYOUT, false, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny,
-                                                                      fnz, base);
+    if (comp && !loads_only(EPI))
+      return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base, vb)
+                   : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base, vb);
+    return clamp ? march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>(), loads_only(EPI)>(
+                       r, a, cur, inc, n, fnx, fny, fnz, base, vb)
+                 : march_loopd<S, STEP2, LAYOUT, false, fetch_ahead<S, EPI>(), loads_only(EPI)>(
+                       r, a, cur, inc, n, fnx, fny, fnz, base, vb);
   }
-  return march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>()>(r, a, cur, inc, n, fnx, fny, fnz, base);
+  return march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>(), loads_only(EPI)>(
+      r, a, cur, inc, n, fnx, fny, fnz, base, vb);
 }
 
 // Storage of the OSC ratio terms for a volume storage S.  They span the
@@ -447,7 +496,8 @@ __device__ __forceinline__ void osc_pair_store(void* pad, size_t q, int PW, floa
 
 // EPI 0: store the ray sum.  EPI 1: OSC ratio terms of CUDAmex_oscIter.cu:262-268
 // written in place, plus zero-haloed row-paired copies ([2, W+4, H+3, views])
-// for the dual backprojection.
+// for the dual backprojection.  EPI 2: the loads-only form, storing the
+// fold of each ray's taps (a measurement, not a projection).
 template <int GEOM, typename S, int EPI, int LAYOUT>
 __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   // Block order: (batch item, chunk of 2^vchunk_log2 views, column group,
@@ -476,7 +526,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   float preval = 0.f, postval = 0.f;
   size_t q = 0;
   typedef typename OscPad<S>::T P;
-  if (EPI == 0) {
+  if (EPI != 1) {
     if (iy >= a.rows) {  // the reference's grid never writes these rows
       out[p] = 0.f;
       return;
@@ -502,13 +552,24 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   // CUDAmex_FP marches cone rays with step 1.0 (CUDAmex_FP.cu:231), par and
   // fan with 0.5; CUDAmex_oscIter uses 0.5 for every geometry (:244).
   constexpr int STEP2 = (EPI == 1 || GEOM != GEOM_CONE) ? 1 : 0;
+  if constexpr (loads_only(EPI)) {
+    R3 cur, inc;
+    const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
+    const char* vb = (const char*)a.vol + item * a.vol_stride;
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)vb, 0, (int)a.vol_bytes, 0x00020000);
+    out[p] = march_z<S, STEP2, LAYOUT, EPI>(r, a, cur, inc, n, vb);
+    return;
+  }
   R3 cur, inc;
   const int n = clip_ray(st, en, a.bx, a.by, a.bz, STEP2 ? 0.5f : 1.0f, cur, inc);
-  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)((const char*)a.vol + item * a.vol_stride), 0, (int)a.vol_bytes, 0x00020000);  // < 4 GiB
-  const float dval = ldexpf(march_z<S, STEP2, LAYOUT, EPI>(r, a, cur, inc, n),
+  const char* vb = (const char*)a.vol + item * a.vol_stride;
+  // below 4 GiB (the 64-bit layouts read through vb instead)
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)vb, 0, (int)a.vol_bytes, 0x00020000);
+  const float dval = ldexpf(march_z<S, STEP2, LAYOUT, EPI>(r, a, cur, inc, n, vb),
                             pow2_exp(a.vol_max ? a.vol_max + item : nullptr));
-  if (EPI == 0) {
+  if (EPI != 1) {
     out[p] = dval;
   } else {
     const float np = (preval * expf(-dval) - postval);
@@ -531,8 +592,9 @@ static void set_view_chunk(FPZArgs& k, uint32_t log2, uint32_t items) {
 
 // W x H detector, NX x NX x NZ volume (the same sizes in the standard
 // geometries; general-3D rays have their own detector).
-static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* sc, int W, int H,
+static FPZArgs make_zargs(const void* vol, int st_dt, int layout_, const float2* sc, int W, int H,
                           int nviews, float sad, int NX = 0, int NZ = 0) {
+  const int layout = base_layout(layout_);
   if (NX == 0) NX = W;
   if (NZ == 0) NZ = H;
   FPZArgs k{};
@@ -553,7 +615,7 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout, const float2* 
   k.PYPZ = k.PY * k.PZ;
   const uint32_t es = (st_dt == 0 ? 4u : 2u) *
                       (layout == LAYOUT_QUAD ? 4u : layout == LAYOUT_DUAL ? 2u : 1u);
-  k.vol_bytes = (uint32_t)((size_t)PX * k.PY * k.PZ * es);
+  k.vol_bytes = wide64(layout_) ? 0u : (uint32_t)((size_t)PX * k.PY * k.PZ * es);
   k.off_j = (uint32_t)k.PZ * es;
   k.off_i = (uint32_t)k.PY * (uint32_t)k.PZ * es;
   k.off_ij = k.off_i + k.off_j;
@@ -587,16 +649,30 @@ static void fpz_dispatch_l(const FPZArgs& k, int geom, hipStream_t st) {
   }
 }
 
-// Layouts built: PLAIN for every storage, DUAL for f32, QUAD for 2-byte.
+// Layouts built: PLAIN for every storage, DUAL and DUAL64 for f32, QUAD and
+// QUAD64 for 2-byte storage (the loads-only form, EPI 2, below).
 template <int EPI, typename S>
 static void fpz_dispatch(const FPZArgs& k, int geom, int layout, hipStream_t st) {
   if constexpr (sizeof(S) == 2) {
     if (layout == LAYOUT_QUAD) return fpz_dispatch_l<EPI, S, LAYOUT_QUAD>(k, geom, st);
+    if (layout == LAYOUT_QUAD64) return fpz_dispatch_l<EPI, S, LAYOUT_QUAD64>(k, geom, st);
   }
   if constexpr (!__is_same(S, bf16_t)) {
     if (layout == LAYOUT_DUAL) return fpz_dispatch_l<EPI, S, LAYOUT_DUAL>(k, geom, st);
   }
+  if constexpr (__is_same(S, float))
+    if (layout == LAYOUT_DUAL64) return fpz_dispatch_l<EPI, S, LAYOUT_DUAL64>(k, geom, st);
   fpz_dispatch_l<EPI, S, LAYOUT_PLAIN>(k, geom, st);
+}
+
+// The loads-only form exists for the layouts the projector picks below
+// 4 GiB: DUAL for f32, QUAD for 2-byte storage.
+template <typename S>
+static hipError_t fpz_loads_dispatch(const FPZArgs& k, int geom, int layout, hipStream_t st) {
+  constexpr int L = sizeof(S) == 2 ? LAYOUT_QUAD : LAYOUT_DUAL;
+  if (layout != L || geom == GEOM_G3D) return hipErrorInvalidValue;
+  fpz_dispatch_l<2, S, L>(k, geom, st);
+  return hipSuccess;
 }
 
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
@@ -613,6 +689,12 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   k.out_stride = a.out_stride;
   if ((uint64_t)k.item_blocks * (uint64_t)items >= (1ull << 31)) return hipErrorInvalidValue;
   if (k.n_logical == 0) return hipSuccess;
+  if (a.loads_only) {
+    const hipError_t e = a.st_dt == 0   ? fpz_loads_dispatch<float>(k, a.geom, a.layout, st)
+                         : a.st_dt == 1 ? fpz_loads_dispatch<_Float16>(k, a.geom, a.layout, st)
+                                        : fpz_loads_dispatch<bf16_t>(k, a.geom, a.layout, st);
+    return e != hipSuccess ? e : hipGetLastError();
+  }
   if (a.st_dt == 0)
     fpz_dispatch<0, float>(k, a.geom, a.layout, st);
   else if (a.st_dt == 1)

@@ -1,0 +1,506 @@
+// g3dbp.hip -- general-3D backprojection (CUDAmex_general3D_BP.cu:61-188)
+// by voxel bricks, for gfx950.
+//
+// The reference scatters each 0.5-voxel sample of each ray into two voxels,
+// (iL, jL, kL) and (iL, jL, kL + 1), with atomicAdd.  A voxel of the helical
+// 320^3 x 360 workload receives about 290 samples, under one ray per view:
+// the rays of one view barely overlap, so staging one view's rays in LDS
+// merges nothing, and one float atomic per sample corner costs ~90% of the
+// ray-driven kernel's time.  Here the volume is cut into bricks of
+// 16 x 16 x 32 voxels; a brick is one workgroup that adds, in LDS, every
+// sample whose cell (iL, jL, kL) lies in it, from every ray of every view,
+// and writes each of its voxels once with a plain store.  Per chunk of views:
+//
+//   bin    one thread per ray marches the reference's running sum once,
+//          keeps the position every kCk samples (checkpoints) and marks, for
+//          its 8x8 tile of detector pixels, each brick that owns one of its
+//          samples (one bit per (brick, tile));
+//   brick  for each tile marked for it, each ray of the tile bounds the
+//          samples the brick can own from its exact line (widened by the
+//          running sum's drift), resumes the running sum at the checkpoint
+//          before them and adds the samples it owns, with the reference's
+//          two-corner weights.  The corner kL + 1 of the brick's top plane
+//          belongs to the brick above and goes to a halo plane;
+//   halo   after the last chunk each halo plane is added to its plane.
+//
+// A brick's LDS sums are float atomics (their order is not fixed, as the
+// reference's atomicAdd order is not) or, in the deterministic mode,
+// 64-bit fixed-point integers whose sum is exact in any order, so the
+// result has the same bits on every run.  Rays of more samples than the
+// volume's diagonal admits cannot cross it; they go through the ray-driven
+// atomic kernel (launch_g3d_bp), which checks every sample as the reference
+// does.
+#include <stdlib.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+#include "internal.h"
+#include "ray.h"
+
+namespace mexct {
+
+constexpr int BX = 16, BY = 16, BZ = 32;  // brick (voxels)
+constexpr int BZH = BZ + 1;               // LDS z extent: the brick plus its halo plane
+constexpr int kTileSide = 8;              // tiles of 8 x 8 detector pixels: one wave
+
+struct G3DBrickArgs {
+  const float* sino;  // [W,H,nProj]
+  const float* pts;
+  int64_t pts_view_stride;
+  const float2* sc;
+  int W, H, M, N;
+  int v0;  // first view of the chunk
+  float bx, by, bz;
+  int n_cap;  // rays of more samples are left to the atomic kernel
+  int S;      // checkpoint slots per ray
+  uint32_t tiles_x, tiles_per_view, ntiles;
+  uint32_t nbx, nby, nbz, nbricks, tw;
+  uint32_t* mask;    // [nbricks][tw] bits over the chunk's tiles
+  float* ck;         // checkpoints: x, y, z planes of ck_plane floats
+  size_t ck_plane;
+  float* rays;       // per ray of the chunk: inc x, y, z and the sample count (bits)
+  size_t ray_plane;  // planes of ray_plane entries, one per (tile, lane)
+  float* out;        // [M,M,N] (float mode)
+  float* halo;       // [nbz][M][M]: corners kL + 1 of each brick's top plane
+  int64_t* out64;    // deterministic mode: fixed-point accumulators
+  int64_t* halo64;
+  const uint32_t* pmax;  // deterministic mode: bits of max |P|
+  int accumulate;        // a later chunk: add to out/halo
+};
+
+// Ray of chunk tile t, lane `lane`: detector pixel and view, and whether it
+// exists.
+struct TileRay {
+  int ix, iy, view;
+  bool ok;
+};
+__device__ __forceinline__ TileRay tile_ray(const G3DBrickArgs& a, uint32_t t, uint32_t lane) {
+  TileRay r;
+  const uint32_t vl = t / a.tiles_per_view, rem = t - vl * a.tiles_per_view;
+  const uint32_t ty = rem / a.tiles_x, tx = rem - ty * a.tiles_x;
+  r.ix = (int)(tx * kTileSide + (lane & 7u));
+  r.iy = (int)(ty * kTileSide + (lane >> 3));
+  r.view = a.v0 + (int)vl;
+  r.ok = r.ix < a.W && r.iy < a.H;
+  return r;
+}
+
+__device__ __forceinline__ int ray_setup(const G3DBrickArgs& a, const TileRay& r, R3& cur,
+                                         R3& inc) {
+  const float2 scv = a.sc[r.view];
+  R3 st, en;
+  g3d_ray(a.pts + (size_t)r.view * a.pts_view_stride + (size_t)r.iy * a.W + r.ix,
+          (size_t)a.W * a.H, scv.x, scv.y, st, en);
+  // slab clip and sample count, as CUDAmex_general3D_BP.cu:84-143
+  return clip_ray(st, en, a.bx, a.by, a.bz, 0.5f, cur, inc);
+}
+
+// ------------------------------------------------------------------ bin
+// One wave per tile.  LDS: one bit per brick per wave.
+__global__ __launch_bounds__(256) void g3d_bin_kernel(G3DBrickArgs a) {
+  extern __shared__ uint32_t smask[];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t t = blockIdx.x * 4u + wave;
+  const uint32_t mw = (a.nbricks + 31u) >> 5;
+  uint32_t* m = smask + (size_t)wave * mw;
+  const float fm = (float)(a.M - 1), fn1 = (float)(a.N - 1);
+  for (uint32_t i = lane; i < mw; i += 64u) m[i] = 0u;
+  __syncthreads();
+  if (t < a.ntiles) {
+    const TileRay r = tile_ray(a, t, lane);
+    if (r.ok) {
+      R3 cur, inc;
+      const int n = ray_setup(a, r, cur, inc);
+      // the brick pass reads the ray's set-up from here (its first
+      // position is checkpoint 0)
+      const size_t ri = (size_t)t * 64u + lane;
+      a.rays[ri] = inc.x;
+      a.rays[a.ray_plane + ri] = inc.y;
+      a.rays[2 * a.ray_plane + ri] = inc.z;
+      a.rays[3 * a.ray_plane + ri] = __int_as_float(n);
+      if (n > 0 && n <= a.n_cap) {
+        const size_t cbase = (size_t)t * a.S * 64u + lane;
+        uint32_t last = ~0u;
+        for (int k = 0; k < n; ++k) {
+          if ((k & (kCkSamples - 1)) == 0) {
+            const size_t ci = cbase + (size_t)(k / kCkSamples) * 64u;
+            a.ck[ci] = cur.x;
+            a.ck[a.ck_plane + ci] = cur.y;
+            a.ck[2 * a.ck_plane + ci] = cur.z;
+          }
+          // the reference's index test (CUDAmex_general3D_BP.cu:148-156),
+          // floor(f) >= 0 and ceil(f) < size on each axis, is
+          // 0 <= f <= size - 1 on the exact coordinates
+          const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
+          if (fx >= 0.f && fx <= fm && fy >= 0.f && fy <= fm && fz >= 0.f && fz <= fn1) {
+            const uint32_t b = (uint32_t)cvt_flr(fx) / BX +
+                               a.nbx * ((uint32_t)cvt_flr(fy) / BY + a.nby * ((uint32_t)cvt_flr(fz) / BZ));
+            if (b != last) {
+              atomicOr(&m[b >> 5], 1u << (b & 31u));
+              last = b;
+            }
+          }
+          cur.x = cur.x + inc.x;
+          cur.y = cur.y + inc.y;
+          cur.z = cur.z + inc.z;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (t < a.ntiles) {
+    const uint32_t bit = 1u << (t & 31u);
+    const size_t col = t >> 5;
+    for (uint32_t i = lane; i < mw; i += 64u) {
+      uint32_t w = m[i];
+      while (w) {
+        const uint32_t b = i * 32u + (uint32_t)__builtin_ctz(w);
+        w &= w - 1u;
+        atomicOr(&a.mask[(size_t)b * a.tw + col], bit);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ brick
+// Range [klo, khi] of the samples k of the line X0 + k s (one axis) that
+// can fall in [lo, hi) once the running sum's drift d is allowed for.  rs is
+// 1/s to within an ulp; the caller widens the range by a sample on each side.
+__device__ __forceinline__ void axis_range(float X0, float s, float rs, float lo, float hi,
+                                           float d, float& klo, float& khi) {
+  if (s > 0.f) {
+    klo = fmaxf(klo, (lo - d - X0) * rs);
+    khi = fminf(khi, (hi + d - X0) * rs);
+  } else if (s < 0.f) {
+    klo = fmaxf(klo, (hi + d - X0) * rs);
+    khi = fminf(khi, (lo - d - X0) * rs);
+  } else if (!(X0 >= lo - d && X0 < hi + d)) {
+    khi = -1.f;  // the axis never moves and stays outside
+  }
+}
+
+// Exclusive float bound of a brick axis [lo, lo + B) intersected with the
+// reference's index test on that axis (ceil(f) < n, i.e. f <= n - 1): the
+// float just above n - 1 when the brick reaches past it.
+__device__ __forceinline__ float axis_hi(int lo, int B, int n) {
+  return lo + B <= n - 1 ? (float)(lo + B) : __uint_as_float(__float_as_uint((float)(n - 1)) + 1u);
+}
+
+template <bool DET>
+struct BrickAcc {
+  typedef float T;
+};
+template <>
+struct BrickAcc<true> {
+  typedef unsigned long long T;
+};
+
+// Adds one (register-merged) cell's two corners to the brick's LDS sums
+// (hi is 0 when both corners are one voxel: adding it changes nothing).
+template <bool DET>
+__device__ __forceinline__ void lds_add(typename BrickAcc<DET>::T* lds, uint32_t idx, float lo,
+                                        float hi, float scale) {
+  if constexpr (DET) {
+    atomicAdd(&lds[idx], g3d_fixed(lo * scale));
+    atomicAdd(&lds[idx + 1], g3d_fixed(hi * scale));
+  } else {
+    atomicAdd(&lds[idx], lo);
+    atomicAdd(&lds[idx + 1], hi);
+  }
+}
+
+template <bool DET>
+__global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
+  typedef typename BrickAcc<DET>::T Acc;
+  __shared__ Acc lds[BX * BY * BZH];
+  const uint32_t brick = xcd_remap(blockIdx.x, gridDim.x);
+  if (brick >= a.nbricks) return;
+  const uint32_t bxi = brick % a.nbx, byi = (brick / a.nbx) % a.nby, bzi = brick / (a.nbx * a.nby);
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(BX * BY * BZH); i += 256u) lds[i] = (Acc)0;
+  __syncthreads();
+  const float scale = DET ? g3d_fixed_scale(a.pmax) : 1.f;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const float lox = (float)(bxi * BX), loy = (float)(byi * BY), loz = (float)(bzi * BZ);
+  // a sample is added here when its cell (iL, jL, kL) passes the reference's
+  // index test (floor >= 0 and ceil < size on each axis) and lies in the
+  // brick: on the exact coordinates f these are lo <= f < hi per axis
+  const float hx = axis_hi((int)(bxi * BX), BX, a.M), hy = axis_hi((int)(byi * BY), BY, a.M),
+              hz = axis_hi((int)(bzi * BZ), BZ, a.N);
+  // LDS index of cell (i, j, k): (i - lox) * BY * BZH + (j - loy) * BZH + (k - loz)
+  const uint32_t base = (bxi * BX * BY + byi * BY) * BZH + bzi * BZ;
+  const uint32_t* row = a.mask + (size_t)brick * a.tw;
+  uint32_t ord = 0;  // marked tiles seen; wave w takes every 4th from w
+  for (uint32_t q = 0; q < a.tw; ++q) {
+    uint32_t w = row[q];
+    while (w) {
+      const uint32_t t = q * 32u + (uint32_t)__builtin_ctz(w);
+      w &= w - 1u;
+      if ((ord++ & 3u) != wave) continue;
+      const TileRay r = tile_ray(a, t, lane);
+      if (!r.ok) continue;
+      // the ray as the bin pass set it up: increments, sample count and
+      // first position (checkpoint 0)
+      const size_t ri = (size_t)t * 64u + lane;
+      const int n = __float_as_int(a.rays[3 * a.ray_plane + ri]);
+      if (n <= 0 || n > a.n_cap) continue;
+      R3 cur, inc;
+      inc.x = a.rays[ri];
+      inc.y = a.rays[a.ray_plane + ri];
+      inc.z = a.rays[2 * a.ray_plane + ri];
+      const size_t c0 = (size_t)t * a.S * 64u + lane;
+      cur.x = a.ck[c0];
+      cur.y = a.ck[a.ck_plane + c0];
+      cur.z = a.ck[2 * a.ck_plane + c0];
+      // samples the brick can own: the exact line X0 + k s inside the brick
+      // on every axis, widened by the running sum's drift (at most a half
+      // ulp of the largest coordinate per step)
+      const float X0 = cur.x - a.bx, Y0 = cur.y - a.by, Z0 = cur.z - a.bz;
+      const float fn = (float)n;
+      const float big = fmaxf(fmaxf(fabsf(cur.x) + fabsf(inc.x) * fn, fabsf(cur.y) + fabsf(inc.y) * fn),
+                              fabsf(cur.z) + fabsf(inc.z) * fn) + 1.f;
+      const float d = fn * big * 1.2e-7f + 1e-3f;
+      float klo = 0.f, khi = fn - 1.f;
+      axis_range(X0, inc.x, __builtin_amdgcn_rcpf(inc.x), lox, lox + (float)BX, d, klo, khi);
+      axis_range(Y0, inc.y, __builtin_amdgcn_rcpf(inc.y), loy, loy + (float)BY, d, klo, khi);
+      axis_range(Z0, inc.z, __builtin_amdgcn_rcpf(inc.z), loz, loz + (float)BZ, d, klo, khi);
+      if (!(klo <= khi)) continue;
+      const int k0 = max(0, (int)floorf(klo) - 1);
+      const int k1 = min(n - 1, (int)ceilf(khi) + 1);
+      // resume the running sum at the checkpoint before k0
+      const int c = k0 / kCkSamples;
+      const size_t ci = ((size_t)t * a.S + (size_t)c) * 64u + lane;
+      cur.x = a.ck[ci];
+      cur.y = a.ck[a.ck_plane + ci];
+      cur.z = a.ck[2 * a.ck_plane + ci];
+      for (int k = c * kCkSamples; k < k0; ++k) {
+        cur.x = cur.x + inc.x;
+        cur.y = cur.y + inc.y;
+        cur.z = cur.z + inc.z;
+      }
+      const float pval = a.sino[(size_t)r.view * a.W * a.H + (size_t)r.iy * a.W + r.ix];
+      const float w0 = pval * 0.5f;  // P * stepsize
+      // consecutive samples in one cell are summed in registers first
+      uint32_t pend = ~0u;
+      float wlo = 0.f, whi = 0.f;
+      for (int k = k0; k <= k1; ++k) {
+        const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
+        const bool in = fx >= lox && fx < hx && fy >= loy && fy < hy && fz >= loz && fz < hz;
+        // CUDAmex_general3D_BP.cu:158-170: f - floor(f) + 0.5 per axis
+        // (exact, as v_fract), kL gets 1 - iZ and kU = ceil iZ
+        const float rz = __builtin_amdgcn_fractf(fz);
+        const float iXX = __builtin_amdgcn_fractf(fx) + 0.5f;
+        const float iYY = __builtin_amdgcn_fractf(fy) + 0.5f;
+        const float iZ = rz + 0.5f;
+        const float iZZ = 1 - iZ;
+        const float wxy = w0 * iXX * iYY;
+        float lo = wxy * iZZ;
+        float hi = wxy * iZ;
+        if (rz == 0.f) {  // kU == kL: both corners are the same voxel
+          lo = lo + hi;
+          hi = 0.f;
+        }
+        const uint32_t idx =
+            (uint32_t)mad_i24(cvt_flr(fx), BY * BZH, mad_i24(cvt_flr(fy), BZH, cvt_flr(fz))) - base;
+        const bool same = idx == pend;
+        if (in && !same && pend != ~0u) lds_add<DET>(lds, pend, wlo, whi, scale);
+        wlo = in ? (same ? wlo + lo : lo) : wlo;
+        whi = in ? (same ? whi + hi : hi) : whi;
+        pend = in ? idx : pend;
+        cur.x = cur.x + inc.x;
+        cur.y = cur.y + inc.y;
+        cur.z = cur.z + inc.z;
+      }
+      if (pend != ~0u) lds_add<DET>(lds, pend, wlo, whi, scale);
+    }
+  }
+  __syncthreads();
+  // the brick's voxels, x fastest across the threads
+  const size_t MM = (size_t)a.M * a.M;
+  for (uint32_t e = threadIdx.x; e < (uint32_t)(BX * BY * BZ); e += 256u) {
+    const int li = (int)(e % BX), lj = (int)((e / BX) % BY), lk = (int)(e / (BX * BY));
+    const int i = (int)bxi * BX + li, j = (int)byi * BY + lj, k = (int)bzi * BZ + lk;
+    if (i >= a.M || j >= a.M || k >= a.N) continue;
+    const Acc v = lds[(li * BY + lj) * BZH + lk];
+    const size_t o = (size_t)i + (size_t)a.M * j + MM * k;
+    if constexpr (DET) {
+      a.out64[o] = a.accumulate ? a.out64[o] + (int64_t)v : (int64_t)v;
+    } else {
+      a.out[o] = a.accumulate ? a.out[o] + v : v;
+    }
+  }
+  // the halo plane: voxel plane k = (bzi + 1) * BZ, owned by the brick above
+  if ((int)(bzi + 1) * BZ < a.N) {
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(BX * BY); e += 256u) {
+      const int li = (int)(e % BX), lj = (int)(e / BX);
+      const int i = (int)bxi * BX + li, j = (int)byi * BY + lj;
+      if (i >= a.M || j >= a.M) continue;
+      const Acc v = lds[(li * BY + lj) * BZH + BZ];
+      const size_t o = (size_t)bzi * MM + (size_t)j * a.M + i;
+      if constexpr (DET) {
+        a.halo64[o] = a.accumulate ? a.halo64[o] + (int64_t)v : (int64_t)v;
+      } else {
+        a.halo[o] = a.accumulate ? a.halo[o] + v : v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ halo
+// Adds halo plane bz to voxel plane (bz + 1) * BZ; in the deterministic mode
+// then turns the fixed-point sums into floats.
+template <bool DET>
+__global__ __launch_bounds__(256) void g3d_halo_kernel(G3DBrickArgs a) {
+  const size_t MM = (size_t)a.M * a.M;
+  const size_t nh = (size_t)(a.nbz - 1) * MM;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < nh; e += (size_t)gridDim.x * 256) {
+    const size_t bz = e / MM, ij = e - bz * MM;
+    const size_t o = ij + MM * (size_t)((bz + 1) * BZ);
+    if constexpr (DET)
+      a.out64[o] += a.halo64[e];
+    else
+      a.out[o] = a.out[o] + a.halo[e];
+  }
+}
+
+__global__ __launch_bounds__(256) void g3d_fixed_to_float_kernel(G3DBrickArgs a, size_t n) {
+  const double inv = 1.0 / (double)g3d_fixed_scale(a.pmax);
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
+    a.out[e] = (float)((double)a.out64[e] * inv);
+}
+
+// ------------------------------------------------------------------ host
+static unsigned grid_of(size_t n) {
+  return (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+}
+
+int g3d_brick_n_cap(int M, int N) {
+  // the longest chord of the [M,M,N] box, in 0.5 steps, plus rounding
+  const double diag = sqrt(2.0 * (double)M * M + (double)N * N);
+  return (int)ceil(diag / 0.5) + 4;
+}
+
+G3DBrickPlan g3d_brick_plan(int W, int H, int M, int N, int nviews) {
+  G3DBrickPlan p;
+  p.tiles_x = (uint32_t)((W + kTileSide - 1) / kTileSide);
+  p.tiles_per_view = p.tiles_x * (uint32_t)((H + kTileSide - 1) / kTileSide);
+  p.nbx = (uint32_t)((M + BX - 1) / BX);
+  p.nby = p.nbx;
+  p.nbz = (uint32_t)((N + BZ - 1) / BZ);
+  p.nbricks = p.nbx * p.nby * p.nbz;
+  p.n_cap = g3d_brick_n_cap(M, N);
+  p.S = (p.n_cap + kCkSamples - 1) / kCkSamples;
+  // views per chunk: checkpoints and ray set-ups within 4 GiB and the tile
+  // bits within 256 MiB (fewer chunks mean fewer brick passes, each with
+  // its own load tail and output read-modify-write)
+  const size_t ck_per_view = (size_t)(3 * p.S + 4) * 4 * 64u * p.tiles_per_view;
+  size_t vc = std::max<size_t>(1, ((size_t)4 << 30) / ck_per_view);
+  const size_t mask_per_view_bits = (size_t)p.nbricks * p.tiles_per_view;
+  vc = std::min(vc, std::max<size_t>(1, ((size_t)256 << 20) * 8 / std::max<size_t>(mask_per_view_bits, 1)));
+  // MEXCT_G3D_CHUNK_VIEWS caps the views per chunk (tests run several
+  // chunks on small workloads with it)
+  if (const char* cap = getenv("MEXCT_G3D_CHUNK_VIEWS")) {
+    const long v = strtol(cap, nullptr, 10);
+    if (v > 0) vc = std::min(vc, (size_t)v);
+  }
+  vc = std::min(vc, (size_t)std::max(nviews, 1));
+  p.views_per_chunk = (int)vc;
+  p.tw = (uint32_t)((vc * p.tiles_per_view + 31) / 32);
+  p.ck_plane = vc * p.tiles_per_view * (size_t)p.S * 64u;
+  p.ray_plane = vc * p.tiles_per_view * 64u;
+  p.ck_bytes = (3 * p.ck_plane + 4 * p.ray_plane) * sizeof(float);
+  p.mask_bytes = (size_t)p.nbricks * p.tw * sizeof(uint32_t);
+  p.halo_elems = (size_t)p.nbz * M * M;
+  // the bin kernel's per-wave brick bits fit the LDS
+  p.ok = (size_t)4 * ((p.nbricks + 31) / 32) * 4 <= (size_t)64 << 10;
+  return p;
+}
+
+hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
+  const G3DBrickPlan& p = L.plan;
+  G3DBrickArgs a{};
+  a.sino = L.sino;
+  a.pts = L.pts;
+  a.pts_view_stride = L.pts_view_stride;
+  a.sc = L.sc;
+  a.W = L.W;
+  a.H = L.H;
+  a.M = L.M;
+  a.N = L.N;
+  a.bx = (float)(-(double)(float)L.M / 2.0);
+  a.by = a.bx;
+  a.bz = (float)(-(double)(float)L.N / 2.0);
+  a.n_cap = p.n_cap;
+  a.S = p.S;
+  a.tiles_x = p.tiles_x;
+  a.tiles_per_view = p.tiles_per_view;
+  a.nbx = p.nbx;
+  a.nby = p.nby;
+  a.nbz = p.nbz;
+  a.nbricks = p.nbricks;
+  a.tw = p.tw;
+  a.mask = L.mask;
+  a.ck = L.ck;
+  a.ck_plane = p.ck_plane;
+  a.rays = L.ck + 3 * p.ck_plane;
+  a.ray_plane = p.ray_plane;
+  a.out = L.out;
+  a.halo = L.halo;
+  a.out64 = L.out64;
+  a.halo64 = L.halo64;
+  a.pmax = L.pmax;
+  const size_t MMN = (size_t)L.M * L.M * L.N;
+  const size_t lds_bin = (size_t)4 * ((p.nbricks + 31) / 32) * sizeof(uint32_t);
+  for (int v0 = 0; v0 < L.nviews; v0 += p.views_per_chunk) {
+    const int nv = std::min(p.views_per_chunk, L.nviews - v0);
+    a.v0 = v0;
+    a.ntiles = (uint32_t)nv * p.tiles_per_view;
+    a.accumulate = v0 > 0;
+    hipError_t e = launch_fill((float*)L.mask, 0.f, p.mask_bytes / 4, st);
+    if (e != hipSuccess) return e;
+    g3d_bin_kernel<<<(a.ntiles + 3) / 4, 256, lds_bin, st>>>(a);
+    const unsigned gb = (p.nbricks + 7u) & ~7u;
+    if (L.deterministic)
+      g3d_brick_kernel<true><<<gb, 256, 0, st>>>(a);
+    else
+      g3d_brick_kernel<false><<<gb, 256, 0, st>>>(a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (L.nviews == 0) {  // no chunk ran: the volume is zero
+    const hipError_t e = L.deterministic ? launch_fill((float*)L.out64, 0.f, 2 * MMN, st)
+                                         : launch_fill(L.out, 0.f, MMN, st);
+    if (e != hipSuccess) return e;
+  }
+  if (p.nbz > 1 && L.nviews > 0) {
+    const unsigned g = grid_of((size_t)(p.nbz - 1) * L.M * L.M);
+    if (L.deterministic)
+      g3d_halo_kernel<true><<<g, 256, 0, st>>>(a);
+    else
+      g3d_halo_kernel<false><<<g, 256, 0, st>>>(a);
+  }
+  if (L.nviews > 0) {  // rays too long to cross the volume, checked sample by sample
+    G3DBPLaunch f;
+    f.sino = L.sino;
+    f.pts = L.pts;
+    f.pts_view_stride = L.pts_view_stride;
+    f.sc = L.sc;
+    f.out = L.out;
+    f.W = L.W;
+    f.H = L.H;
+    f.nviews = L.nviews;
+    f.M = L.M;
+    f.N = L.N;
+    f.n_min = p.n_cap;
+    f.xfast = true;
+    f.out64 = L.deterministic ? L.out64 : nullptr;
+    f.pmax = L.pmax;
+    const hipError_t e = launch_g3d_bp(f, st);
+    if (e != hipSuccess) return e;
+  }
+  if (L.deterministic) g3d_fixed_to_float_kernel<<<grid_of(MMN), 256, 0, st>>>(a, MMN);
+  return hipGetLastError();
+}
+
+}  // namespace mexct

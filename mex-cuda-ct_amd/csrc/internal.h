@@ -62,7 +62,13 @@ hipError_t launch_fp(const FPLaunch& a, hipStream_t st);
 
 // standard-geometry FP on a z-fastest haloed volume (launch_pad3z layout)
 // z-fastest volume layouts of the standard-geometry FP (see fpz.hip)
-enum { LAYOUT_PLAIN = 0, LAYOUT_QUAD = 1, LAYOUT_DUAL = 2 };
+// DUAL64 / QUAD64: the DUAL / QUAD copies past 4 GiB, read with 64-bit
+// addresses instead of 32-bit buffer offsets (every index clamped)
+enum { LAYOUT_PLAIN = 0, LAYOUT_QUAD = 1, LAYOUT_DUAL = 2, LAYOUT_DUAL64 = 3, LAYOUT_QUAD64 = 4 };
+constexpr int base_layout(int l) {
+  return l == LAYOUT_DUAL64 ? LAYOUT_DUAL : l == LAYOUT_QUAD64 ? LAYOUT_QUAD : l;
+}
+constexpr bool wide64(int l) { return l == LAYOUT_DUAL64 || l == LAYOUT_QUAD64; }
 
 struct FPZLaunch {
   const void* vol;  // launch_pad3z (PLAIN), launch_pad3d (DUAL) or launch_pad3q (QUAD) output
@@ -82,6 +88,9 @@ struct FPZLaunch {
   int items = 1;
   uint64_t vol_stride = 0, out_stride = 0;
   int vchunk_log2 = -1;  // block order: views per chunk 2^vchunk_log2 (-1: default 16)
+  // the loads-only form: the same blocks, addresses and loads, the taps
+  // XOR-folded instead of interpolated (par/fan/cone, DUAL f32 or QUAD 2-byte)
+  bool loads_only = false;
 };
 hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st);
 
@@ -120,6 +129,9 @@ struct BPLaunch {
   int items = 1;
   uint64_t sino_stride = 0, out_stride = 0;
   int super_ = 0;    // block order: super-tile side in 8x8 tiles (0: default 4)
+  // the loads-only form: the same blocks and loads, each load XOR-folded
+  // instead of interpolated (cone only)
+  bool loads_only = false;
 };
 // floats of BP chain scratch per view (cone f32, z-range zr, super-tile side)
 size_t bp_chain_floats_per_view(int W, int H, int zr, int super_);
@@ -147,8 +159,44 @@ struct G3DBPLaunch {
   const float2* sc;
   float* out;         // z-fastest accumulator acc[(i*M + j)*N + k], zeroed by the caller
   int W, H, nviews, M, N;
+  // only rays of more than n_min samples (the bricks take the others)
+  int n_min = 0;
+  // the [M,M,N] x-fastest volume instead of the z-fastest accumulator;
+  // with out64 set, 64-bit fixed-point sums there (scale from pmax) instead
+  bool xfast = false;
+  int64_t* out64 = nullptr;
+  const uint32_t* pmax = nullptr;
 };
 hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st);
+
+// general-3D backprojection by voxel bricks (g3dbp.hip)
+constexpr int kCkSamples = 64;  // samples between a ray's checkpoints
+struct G3DBrickPlan {
+  uint32_t tiles_x, tiles_per_view, nbx, nby, nbz, nbricks, tw;
+  int n_cap, S, views_per_chunk;
+  size_t ck_plane, ray_plane, ck_bytes, mask_bytes, halo_elems;
+  bool ok;  // the bricks' bits fit the binning kernel's LDS
+};
+G3DBrickPlan g3d_brick_plan(int W, int H, int M, int N, int nviews);
+struct G3DBrickLaunch {
+  G3DBrickPlan plan;
+  const float* sino;
+  const float* pts;
+  int64_t pts_view_stride;
+  const float2* sc;
+  int W, H, nviews, M, N;
+  uint32_t* mask;   // plan.mask_bytes
+  float* ck;        // plan.ck_bytes
+  float* out;       // [M,M,N]
+  float* halo;      // plan.halo_elems floats (float mode)
+  bool deterministic;
+  int64_t* out64;   // [M,M,N] and plan.halo_elems fixed-point sums (deterministic)
+  int64_t* halo64;
+  const uint32_t* pmax;  // bits of max |sino| (deterministic)
+};
+// Every chunk of views, the halo planes, the rays left to the atomic kernel
+// and, when deterministic, the conversion to float: out is complete after.
+hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& a, hipStream_t st);
 // z-fastest accumulator -> [M,M,N] x-fastest volume
 hipError_t launch_g3d_unz(const float* acc, float* out, int M, int N, hipStream_t st);
 
@@ -214,5 +262,12 @@ hipError_t launch_gather_probe(const void* buf, int st_dt, int pattern, int W, i
 hipStream_t ctx_stream(const mexct_ctx* c);
 // sets the calling thread's mexct_last_error() text and returns code
 int set_error(int code, const char* msg);
+// Upper bound of the device bytes a context's workspace holds after one
+// call of operator op (MEXCT_OP_*: FP, BP, FDK = pre-filter + BP, OSC-TV)
+// over `items` batched operands, each workspace slot rounded up as the
+// context allocates it.  min_subsets: the fewest subsets of any OSC-TV
+// iteration (sets the largest per-subset ratio-term copies).
+size_t op_workspace_bytes(int op, int geom, int st, int64_t W, int64_t H, int64_t nProj, float sad,
+                          int items, int min_subsets);
 
 }  // namespace mexct

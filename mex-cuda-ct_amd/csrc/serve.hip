@@ -13,7 +13,10 @@
 // holds.  FP, BP and FDK groups run as one batch: every kernel of the
 // projector or backprojector is one launch for the whole group
 // (mexct_fp_batch_dev / mexct_bp_batch_dev), each item computed exactly as
-// alone.  OSC-TV groups run back to back with the tables resident.  The
+// alone; a batch of host-array requests runs in two halves whose copies
+// overlap the other half's kernels.  OSC-TV groups run back to back with
+// the tables resident.  Requests marked resident bring device arrays and
+// skip the host copies.  The
 // workers' contexts autotune the block order of a problem shape at its
 // second call.
 //
@@ -50,7 +53,7 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr int kMaxOvertake = 8;
-constexpr size_t kSlack = (size_t)24 << 20;  // tables, filter taps, 2 MiB rounding
+constexpr size_t kSlack = (size_t)4 << 20;  // host-table caches and allocator rounding
 
 double now_s() {
   return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
@@ -69,29 +72,42 @@ struct Req {
 };
 
 // Upper estimate of the device bytes a request needs on a worker: the
-// worker's staging arrays plus the C ABI's workspace for that operator
-// (haloed copies at 4 bytes per element, the widest storage).
+// worker's staging arrays (run_one) plus the workspace the C ABI takes for
+// that operator, from the ABI's own layouts (mexct::op_workspace_bytes).
+// Per request; a batched group of G is reserved as G times this, which
+// bounds the batch's staging and its workspace (only the haloed copies
+// scale with G).
 int64_t footprint_of(const mexct_request& r) {
-  // (per request; a batched group of G needs G times this)
   const size_t W = (size_t)r.W, H = (size_t)r.H, P = (size_t)r.nProj;
   const size_t v = W * W * H * 4, s = W * H * P * 4;
-  const size_t vh = (W + 4) * (W + 4) * (H + 4) * 4;
-  const size_t sh = (W + 4) * (H + 3) * P * 8;  // row-paired haloed sinogram
-  size_t b = 0;
+  size_t staging = 0;
+  int min_subsets = 1;
   switch (r.op) {
-    case MEXCT_OP_FP: b = v + s + vh; break;
-    case MEXCT_OP_BP: b = s + v + sh; break;
-    case MEXCT_OP_FDK: b = 2 * s + v + sh; break;
-    default: b = 8 * s + 2 * sh + 3 * vh + 3 * v; break;  // OSC-TV state
+    case MEXCT_OP_FP: staging = v + s; break;
+    case MEXCT_OP_BP: staging = s + v; break;
+    case MEXCT_OP_FDK: staging = 2 * s + v; break;
+    default:
+      staging = 2 * s + v;
+      if (r.osc_iters >= 2) {
+        min_subsets = INT32_MAX;
+        for (int k = 1; k <= r.osc_iters; ++k)
+          min_subsets = std::min(min_subsets, mexct_calc_num_subsets(r.osc_s_init, r.osc_s_final,
+                                                                     r.osc_p, r.osc_iters, k));
+        min_subsets = std::max(min_subsets, 1);
+      }
+      break;
   }
-  return (int64_t)(b + kSlack);
+  const size_t ws = mexct::op_workspace_bytes(r.op, r.geom, r.storage, r.W, r.H, r.nProj, r.sad, 1,
+                                              min_subsets);
+  return (int64_t)(staging + ws + kSlack);
 }
 
 bool same_key(const Req& a, const Req& b) {
   const mexct_request &x = a.r, &y = b.r;
   return x.op == y.op && x.geom == y.geom && x.storage == y.storage && x.W == y.W &&
          x.H == y.H && x.nProj == y.nProj && x.sad == y.sad && x.window == y.window &&
-         x.cutoff == y.cutoff && x.tv_steps == y.tv_steps && a.angles == b.angles;
+         x.cutoff == y.cutoff && x.tv_steps == y.tv_steps && x.resident == y.resident &&
+         a.angles == b.angles;
 }
 
 struct Buf {
@@ -114,6 +130,10 @@ struct mexct_server {
     // flight (allocated at first use, freed when the server is destroyed)
     void* pin[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // a batch of host requests runs in two halves: the copies of one half
+    // go on this stream while the other half's kernels run on the context's
+    hipStream_t cs = nullptr;
+    hipEvent_t ev_in = nullptr, ev_k[2] = {nullptr, nullptr};
   };
   int64_t n_batched = 0, n_batches = 0;  // requests run in multi-item launches, such groups
   int device = 0;
@@ -196,6 +216,30 @@ bool pinned_ready(Worker& w) {
   return true;
 }
 
+// The copy stream (default priority) and the events of pipelined batches.
+int worker_streams(int device, Worker& w) {
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&w.cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev_in, hipEventDisableTiming);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&w.ev_k[k], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return mexct::set_error(MEXCT_EHIP, hipGetErrorString(e));
+  }
+  return MEXCT_OK;
+}
+
+void free_streams(Worker& w) {
+  if (w.cs) (void)hipStreamSynchronize(w.cs);
+  for (hipEvent_t* e : {&w.ev_in, &w.ev_k[0], &w.ev_k[1]}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  if (w.cs) (void)hipStreamDestroy(w.cs);
+  w.cs = nullptr;
+}
+
 void free_pinned(Worker& w) {
   for (int k = 0; k < 2; ++k) {
     if (w.pin[k]) (void)hipHostFree(w.pin[k]);
@@ -205,16 +249,15 @@ void free_pinned(Worker& w) {
   }
 }
 
-// Device -> host copies of n (dst, src, bytes) segments on the worker's
-// stream, in order after the work already on it.  Returns when every
+// Device -> host copies of n (dst, src, bytes) segments on stream st, in
+// order after the work already on it.  Returns when every
 // output is in host memory.
 struct Seg {
   void* dst;
   const void* src;
   size_t bytes;
 };
-int copy_out_segs(Worker& w, const Seg* seg, int n) {
-  hipStream_t st = mexct::ctx_stream(w.ctx);
+int copy_out_segs(Worker& w, hipStream_t st, const Seg* seg, int n) {
   hipError_t e = hipSuccess;
   size_t total = 0;
   for (int i = 0; i < n; ++i) total += seg[i].bytes;
@@ -253,7 +296,7 @@ int copy_out_segs(Worker& w, const Seg* seg, int n) {
 
 int copy_out(Worker& w, void* dst, const void* src, size_t bytes) {
   const Seg s{dst, src, bytes};
-  return copy_out_segs(w, &s, 1);
+  return copy_out_segs(w, mexct::ctx_stream(w.ctx), &s, 1);
 }
 
 #define TRY(x)          \
@@ -262,11 +305,78 @@ int copy_out(Worker& w, void* dst, const void* src, size_t bytes) {
     if (r_) return r_;  \
   } while (0)
 
+#define HTRY(x)                                                                     \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return mexct::set_error(MEXCT_EHIP, hipGetErrorString(e_)); \
+  } while (0)
+
+// A resident request's array: device memory of the server's device holding
+// at least `bytes` from p on.
+bool on_device(const void* p, size_t bytes, int device) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (at.type != hipMemoryTypeDevice || at.device != device) return false;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return (const char*)p + bytes <= (const char*)base + size;
+}
+
+// Input and output bytes of a request (input2 has the input's size).
+void io_bytes(const mexct_request& r, size_t* in, size_t* out) {
+  const size_t vb = (size_t)r.W * r.W * r.H * 4, sb = (size_t)r.W * r.H * r.nProj * 4;
+  *in = r.op == MEXCT_OP_FP ? vb : sb;
+  *out = r.op == MEXCT_OP_FP ? sb : vb;
+}
+
+// A resident request: inputs and output in device memory, no staging.
+int run_one_resident(Worker& w, Req& q) {
+  const mexct_request& r = q.r;
+  const mexct_opts o{r.storage, r.tv_steps, 0, 0, MEXCT_G3D_BRICKS};
+  const size_t sb = (size_t)r.W * r.H * r.nProj * 4;
+  const float* ang = q.angles.data();
+  switch (r.op) {
+    case MEXCT_OP_FP:
+      TRY(mexct_fp_dev(w.ctx, r.input, MEXCT_F32, r.W, r.H, r.sad, ang, (int32_t)r.nProj, r.geom,
+                       r.output, &o));
+      break;
+    case MEXCT_OP_BP:
+    case MEXCT_OP_FDK: {
+      const void* bp_in = r.input;
+      if (r.op == MEXCT_OP_FDK) {
+        TRY(grow(w.x, sb));
+        TRY(mexct_prefilter_dev(w.ctx, r.input, r.W, r.H, r.nProj, r.geom, r.sad, ang, r.window,
+                                r.cutoff, MEXCT_NO_PARKER, (float*)w.x.p));
+        bp_in = w.x.p;
+      }
+      TRY(mexct_bp_dev(w.ctx, bp_in, MEXCT_F32, r.W, r.H, r.nProj, r.sad, ang, r.geom, r.output,
+                       &o));
+      break;
+    }
+    case MEXCT_OP_OSCTV:
+      TRY(mexct_osc_tv_recon_dev(w.ctx, r.input, r.input2, r.W, r.H, r.nProj, ang, r.sad, r.geom,
+                                 r.osc_iters, r.osc_tv, r.osc_s_init, r.osc_s_final, r.osc_p,
+                                 q.picks.data(), r.output, &o));
+      break;
+    default: return mexct::set_error(MEXCT_EINVAL, "server: unknown operator");
+  }
+  HTRY(hipStreamSynchronize(mexct::ctx_stream(w.ctx)));
+  return MEXCT_OK;
+}
+
 // Runs one request on a worker: host -> device, operator(s) on the worker's
 // stream, device -> host.  Returns when the output is in host memory.
 int run_one(Worker& w, Req& q) {
   const mexct_request& r = q.r;
-  const mexct_opts o{r.storage, r.tv_steps, 0, 0};
+  if (r.resident) return run_one_resident(w, q);
+  const mexct_opts o{r.storage, r.tv_steps, 0, 0, MEXCT_G3D_BRICKS};
   const size_t vb = (size_t)r.W * r.W * r.H * 4, sb = (size_t)r.W * r.H * r.nProj * 4;
   const float* ang = q.angles.data();
   switch (r.op) {
@@ -312,37 +422,95 @@ bool batchable(const mexct_request& r) {
   return r.op == MEXCT_OP_FP || r.op == MEXCT_OP_BP || r.op == MEXCT_OP_FDK;
 }
 
-// Runs a same-key group of FP/BP/FDK requests as one batch: inputs staged one
-// after another, one batched projector or backprojector, outputs copied out.
-int run_batch(Worker& w, const std::vector<Req*>& g) {
+// The batched operator over items [first, first + count) of a group whose
+// inputs sit one after another in w.a; outputs go to w.o (FDK filters into
+// w.x first).  Enqueued on the context's stream.
+int launch_items(Worker& w, const mexct_request& r, const float* ang, int first, int count) {
+  const mexct_opts o{r.storage, r.tv_steps, 0, 0, MEXCT_G3D_BRICKS};
+  size_t in_b, out_b;
+  io_bytes(r, &in_b, &out_b);
+  const char* in = (const char*)w.a.p + first * in_b;
+  float* out = (float*)((char*)w.o.p + first * out_b);
+  if (r.op == MEXCT_OP_FP)
+    return mexct_fp_batch_dev(w.ctx, in, MEXCT_F32, count, r.W, r.H, r.sad, ang, (int32_t)r.nProj,
+                              r.geom, out, &o);
+  const void* bp_in = in;
+  if (r.op == MEXCT_OP_FDK) {
+    char* x = (char*)w.x.p + first * in_b;
+    for (int i = 0; i < count; ++i)
+      TRY(mexct_prefilter_dev(w.ctx, (const float*)(in + i * in_b), r.W, r.H, r.nProj, r.geom,
+                              r.sad, ang, r.window, r.cutoff, MEXCT_NO_PARKER,
+                              (float*)(x + i * in_b)));
+    bp_in = x;
+  }
+  return mexct_bp_batch_dev(w.ctx, bp_in, MEXCT_F32, count, r.W, r.H, r.nProj, r.sad, ang, r.geom,
+                            out, &o);
+}
+
+// Outputs of items [first, first + count) to the host, on the copy stream
+// once the kernels recorded by ev have run.
+int copy_out_items(Worker& w, const std::vector<Req*>& g, int first, int count, hipEvent_t ev) {
+  size_t in_b, out_b;
+  io_bytes(g[0]->r, &in_b, &out_b);
+  HTRY(hipStreamWaitEvent(w.cs, ev, 0));
+  std::vector<Seg> segs((size_t)count);
+  for (int i = 0; i < count; ++i)
+    segs[i] = Seg{g[first + i]->r.output, (char*)w.o.p + (first + i) * out_b, out_b};
+  return copy_out_segs(w, w.cs, segs.data(), count);
+}
+
+// Runs a same-key group of FP/BP/FDK requests as one batch, each item
+// computed exactly as alone.
+//  - resident requests: inputs gathered into one array by device copies, one
+//    batched launch, outputs scattered back by device copies;
+//  - host requests: in two halves, so that copies and kernels overlap:
+//    inputs of half 0; inputs of half 1 while half 0 computes; outputs of
+//    half 0 while half 1 computes; outputs of half 1.
+int run_batch_impl(Worker& w, const std::vector<Req*>& g) {
   const mexct_request& r = g[0]->r;
-  const mexct_opts o{r.storage, r.tv_steps, 0, 0};
   const int n = (int)g.size();
-  const size_t vb = (size_t)r.W * r.W * r.H * 4, sb = (size_t)r.W * r.H * r.nProj * 4;
+  size_t in_b, out_b;
+  io_bytes(r, &in_b, &out_b);
   const float* ang = g[0]->angles.data();
-  const size_t in_b = r.op == MEXCT_OP_FP ? vb : sb, out_b = r.op == MEXCT_OP_FP ? sb : vb;
+  hipStream_t ks = mexct::ctx_stream(w.ctx);
   TRY(grow(w.a, in_b * n));
   TRY(grow(w.o, out_b * n));
-  for (int i = 0; i < n; ++i) TRY(copy_in(w, (char*)w.a.p + i * in_b, g[i]->r.input, in_b));
-  if (r.op == MEXCT_OP_FP) {
-    TRY(mexct_fp_batch_dev(w.ctx, w.a.p, MEXCT_F32, n, r.W, r.H, r.sad, ang, (int32_t)r.nProj,
-                           r.geom, (float*)w.o.p, &o));
-  } else {
-    const void* bp_in = w.a.p;
-    if (r.op == MEXCT_OP_FDK) {
-      TRY(grow(w.x, sb * n));
-      for (int i = 0; i < n; ++i)
-        TRY(mexct_prefilter_dev(w.ctx, (const float*)((const char*)w.a.p + i * sb), r.W, r.H,
-                                r.nProj, r.geom, r.sad, ang, r.window, r.cutoff, MEXCT_NO_PARKER,
-                                (float*)((char*)w.x.p + i * sb)));
-      bp_in = w.x.p;
-    }
-    TRY(mexct_bp_batch_dev(w.ctx, bp_in, MEXCT_F32, n, r.W, r.H, r.nProj, r.sad, ang, r.geom,
-                           (float*)w.o.p, &o));
+  if (r.op == MEXCT_OP_FDK) TRY(grow(w.x, in_b * n));
+  if (r.resident) {
+    for (int i = 0; i < n; ++i)
+      HTRY(hipMemcpyAsync((char*)w.a.p + i * in_b, g[i]->r.input, in_b, hipMemcpyDeviceToDevice, ks));
+    TRY(launch_items(w, r, ang, 0, n));
+    for (int i = 0; i < n; ++i)
+      HTRY(hipMemcpyAsync(g[i]->r.output, (char*)w.o.p + i * out_b, out_b,
+                          hipMemcpyDeviceToDevice, ks));
+    HTRY(hipStreamSynchronize(ks));
+    return MEXCT_OK;
   }
-  std::vector<Seg> segs((size_t)n);
-  for (int i = 0; i < n; ++i) segs[i] = Seg{g[i]->r.output, (char*)w.o.p + i * out_b, out_b};
-  return copy_out_segs(w, segs.data(), n);
+  const int h = (n + 1) / 2;
+  const int first[2] = {0, h}, count[2] = {h, n - h};
+  for (int p = 0; p < 2; ++p) {
+    if (count[p] > 0) {
+      for (int i = first[p]; i < first[p] + count[p]; ++i)
+        HTRY(hipMemcpyAsync((char*)w.a.p + i * in_b, g[i]->r.input, in_b, hipMemcpyHostToDevice,
+                            w.cs));
+      HTRY(hipEventRecord(w.ev_in, w.cs));
+      HTRY(hipStreamWaitEvent(ks, w.ev_in, 0));
+      TRY(launch_items(w, r, ang, first[p], count[p]));
+      HTRY(hipEventRecord(w.ev_k[p], ks));
+    }
+    if (p == 1) TRY(copy_out_items(w, g, first[0], count[0], w.ev_k[0]));
+  }
+  if (count[1] > 0) TRY(copy_out_items(w, g, first[1], count[1], w.ev_k[1]));
+  return MEXCT_OK;
+}
+
+int run_batch(Worker& w, const std::vector<Req*>& g) {
+  const int rc = run_batch_impl(w, g);
+  if (rc) {  // nothing of the batch may still run when its requests are retried alone
+    (void)hipStreamSynchronize(w.cs);
+    (void)hipStreamSynchronize(mexct::ctx_stream(w.ctx));
+  }
+  return rc;
 }
 
 enum class Pick { kWait, kRun, kRelease };
@@ -489,16 +657,25 @@ int mexct_server_create(mexct_server** out, int device, int64_t hbm_budget, int3
   sv->device = device;
   sv->budget = hbm_budget;
   sv->max_group = max_group > 0 ? max_group : 1;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
   for (int i = 0; i < slots; ++i) {
     std::unique_ptr<Worker> w(new Worker());
-    const int rc = new_ctx(sv.get(), *w);
+    int rc = new_ctx(sv.get(), *w);
+    if (!rc) rc = worker_streams(sv->device, *w);
     if (rc) {
+      free_streams(*w);
       if (w->ctx) mexct_ctx_destroy(w->ctx);
-      for (auto& p : sv->workers) mexct_ctx_destroy(p->ctx);
+      for (auto& p : sv->workers) {
+        free_streams(*p);
+        mexct_ctx_destroy(p->ctx);
+      }
+      (void)hipSetDevice(prev);
       return rc;
     }
     sv->workers.push_back(std::move(w));
   }
+  (void)hipSetDevice(prev);
   for (auto& w : sv->workers) w->th = std::thread(worker_loop, sv.get(), w.get());
   *out = sv.release();
   return MEXCT_OK;
@@ -513,6 +690,15 @@ int mexct_server_submit(mexct_server* sv, const mexct_request* r, uint64_t* tick
       !r->output ||
       (r->op == MEXCT_OP_OSCTV && (!r->input2 || !r->first_picks || r->osc_iters < 1)))
     return mexct::set_error(MEXCT_EINVAL, "server: bad request dims or null array");
+  if (r->resident) {
+    size_t in_b, out_b;
+    io_bytes(*r, &in_b, &out_b);
+    if (!on_device(r->input, in_b, sv->device) || !on_device(r->output, out_b, sv->device) ||
+        (r->op == MEXCT_OP_OSCTV && !on_device(r->input2, in_b, sv->device)))
+      return mexct::set_error(MEXCT_EINVAL,
+                              "server: a resident request's arrays must be device memory of the "
+                              "server's device, large enough for the request");
+  }
   std::unique_ptr<Req> q(new Req());
   q->r = *r;
   q->angles.assign(r->angles, r->angles + r->nProj);
@@ -529,6 +715,13 @@ int mexct_server_submit(mexct_server* sv, const mexct_request* r, uint64_t* tick
   sv->reqs[q->id] = std::move(q);
   sv->cv_work.notify_all();
   return MEXCT_OK;
+}
+
+int64_t mexct_server_footprint(const mexct_request* r) {
+  if (!r || r->op < MEXCT_OP_FP || r->op > MEXCT_OP_OSCTV || r->W < 1 || r->H < 1 || r->nProj < 1 ||
+      r->storage < MEXCT_F32 || r->storage > MEXCT_BF16)
+    return mexct::set_error(MEXCT_EINVAL, "server: bad request for a footprint");
+  return footprint_of(*r);
 }
 
 int mexct_server_wait(mexct_server* sv, uint64_t ticket, int32_t* status, double* latency_s,
@@ -574,6 +767,7 @@ int mexct_server_destroy(mexct_server* sv) {
     for (Buf* b : {&w->a, &w->b, &w->o, &w->x})
       if (b->p) (void)hipFree(b->p);
     free_pinned(*w);
+    free_streams(*w);
     mexct_ctx_destroy(w->ctx);
   }
   delete sv;

@@ -16,6 +16,8 @@ extern "C" void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   if (mexgw::dim(prhs[2], 0) != W || mexgw::dim(prhs[2], 1) != H || mexgw::dim(prhs[2], 2) != 6)
     mexErrMsgIdAndTxt(id, "pts must be MxNx6 with the projections' M and N");
   const int ppv = mxGetNumberOfDimensions(prhs[2]) >= 4 ? 1 : 0;
+  if (ppv && mexgw::dim(prhs[2], 3) < n)
+    mexErrMsgIdAndTxt(id, "per-view pts has fewer views than the projections");
   const mwSize dims[] = {M, M, N};
   plhs[0] = mxCreateNumericArray(3, dims, mxSINGLE_CLASS, mxREAL);
   mexgw::report(mexct_g3d_bp(P, (int64_t)W, (int64_t)H, (int64_t)n, angles,

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libmexct.so")
+# MEXCT_LIB names another build of the library (the sanitizer build, for one)
+LIB_PATH = os.environ.get("MEXCT_LIB") or os.path.join(_PKG, "lib", "libmexct.so")
 
 OK, EINVAL, EGEOM, EHIP, ENOMEM, ENOTSUP = 0, -1, -2, -3, -4, -5
 F32, F16, BF16 = 0, 1, 2
@@ -26,7 +27,10 @@ class MexctError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("storage", C.c_int32), ("tv_steps", C.c_int32), ("zrange", C.c_int32),
-                ("long_ray", C.c_int32)]
+                ("long_ray", C.c_int32), ("g3d_bp", C.c_int32)]
+
+
+G3D_BRICKS, G3D_DETERMINISTIC, G3D_ATOMICS = 0, 1, 2
 
 
 class Request(C.Structure):
@@ -37,7 +41,7 @@ class Request(C.Structure):
                 ("osc_p", C.c_double), ("osc_iters", C.c_int32), ("osc_s_init", C.c_int32),
                 ("osc_s_final", C.c_int32), ("tv_steps", C.c_int32), ("angles", C.c_void_p),
                 ("input", C.c_void_p), ("input2", C.c_void_p), ("output", C.c_void_p),
-                ("first_picks", C.c_void_p)]
+                ("first_picks", C.c_void_p), ("resident", C.c_int32)]
 
 
 OP_FP, OP_BP, OP_FDK, OP_OSCTV = 0, 1, 2, 3
@@ -57,6 +61,8 @@ SIGNATURES = {
     "mexct_fp_dev": [_vp, _vp, _i32, _i64, _i64, _f, _vp, _i32, _i32, _vp, _op],
     "mexct_bp": [_vp, _i64, _i64, _i64, _f, _vp, _i64, _i32, _vp, _op],
     "mexct_bp_dev": [_vp, _vp, _i32, _i64, _i64, _i64, _f, _vp, _i32, _vp, _op],
+    "mexct_fp_loads_dev": [_vp, _vp, _i32, _i64, _i64, _f, _vp, _i32, _i32, _vp, _op],
+    "mexct_bp_loads_dev": [_vp, _vp, _i32, _i64, _i64, _i64, _f, _vp, _i32, _vp, _op],
     "mexct_fp_batch_dev": [_vp, _vp, _i32, _i32, _i64, _i64, _f, _vp, _i32, _i32, _vp, _op],
     "mexct_bp_batch_dev": [_vp, _vp, _i32, _i32, _i64, _i64, _i64, _f, _vp, _i32, _vp, _op],
     "mexct_set_autotune": [_i32],
@@ -81,9 +87,14 @@ SIGNATURES = {
                            _d, _vp, _vp, _op],
     "mexct_osc_tv_recon_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _f, _i32, _i32, _f, _i32,
                                _i32, _d, _vp, _vp, _op],
+    "mexct_osc_tv_iterations": [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _f, _i32, _i32, _f, _i32,
+                                _i32, _d, _vp, _i32, _i32, _vp, _op],
+    "mexct_osc_tv_iterations_dev": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _f, _i32, _i32, _f,
+                                    _i32, _i32, _d, _vp, _i32, _i32, _vp, _op],
     "mexct_probe_gather_dev": [_vp, _i32, _i64, _i64, _i32, _i32, C.POINTER(_i64)],
     "mexct_server_create": [C.POINTER(_vp), C.c_int, _i64, _i32, _i32],
     "mexct_server_submit": [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)],
+    "mexct_server_footprint": [C.POINTER(Request)],
     "mexct_server_wait": [_vp, C.c_uint64, C.POINTER(_i32), C.POINTER(_d), C.POINTER(_d)],
     "mexct_server_stats": [_vp, C.POINTER(_i64), _i32],
     "mexct_server_destroy": [_vp],
@@ -103,11 +114,16 @@ def lib():
                                       "(there is no CPU fallback)")
         L = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
-            fn = getattr(L, name)
+            # a symbol the built library lacks stays unset (its callers raise
+            # AttributeError); the export test checks that none is missing
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
             fn.argtypes = args
             fn.restype = C.c_int
         L.mexct_last_error.restype = C.c_char_p
         L.mexct_ctx_workspace_bytes.restype = C.c_int64
+        L.mexct_server_footprint.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -117,5 +133,6 @@ def check(rc: int) -> None:
         raise MexctError(rc, lib().mexct_last_error().decode(errors="replace"))
 
 
-def opts(storage: int = F32, tv_steps: int = 0, zrange: int = 0, long_ray: int = 0):
-    return C.byref(Opts(storage, tv_steps, zrange, long_ray))
+def opts(storage: int = F32, tv_steps: int = 0, zrange: int = 0, long_ray: int = 0,
+         g3d_bp: int = G3D_BRICKS):
+    return C.byref(Opts(storage, tv_steps, zrange, long_ray, g3d_bp))

@@ -100,19 +100,24 @@ def CUDAmex_general3D_FP(vol, angles, pts, nProjections, storage=F32, long_ray=0
     return out
 
 
-def CUDAmex_general3D_BP(P, angles, pts, M, N):
-    """bp = CUDAmex_general3D_BP(P, angles, pts, M, N) (CUDAmex_general3D_BP.cu:194-282)."""
+def CUDAmex_general3D_BP(P, angles, pts, M, N, mode=0):
+    """bp = CUDAmex_general3D_BP(P, angles, pts, M, N) (CUDAmex_general3D_BP.cu:194-282).
+
+    mode: 0 voxel bricks (default), 1 the same with deterministic fixed-point
+    sums (the same bits on every run), 2 the ray-driven float atomics."""
     P = _single(P, "projections")
     pts = _single(pts, "pts")
     W, H, n = _dims3(P)
-    if pts.shape[0] != W or pts.shape[1] != H or pts.shape[2] != 6:
+    if pts.ndim not in (3, 4) or pts.shape[0] != W or pts.shape[1] != H or pts.shape[2] != 6:
         raise MexctError(EINVAL, f"pts must be [W,H,6(,nProj)] matching P, got {pts.shape}")
     ppv = 1 if pts.ndim == 4 else 0
+    if ppv and pts.shape[3] < n:
+        raise MexctError(EINVAL, "per-view pts has fewer views than the projections")
     ang = _vec(angles, np.float32, "angles")
     M, N = int(M), int(N)
     out = np.zeros((M, M, N), dtype=np.float32, order="F")
     check(lib().mexct_g3d_bp(_ptr(P), W, H, n, _ptr(ang), ang.size, _ptr(pts), ppv, M, N,
-                             _ptr(out), None))
+                             _ptr(out), opts(g3d_bp=int(mode))))
     return out
 
 
@@ -205,9 +210,14 @@ def first_picks(seed, s_init, s_final, p, n_iter):
 
 
 def OSC_TV_recon(pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit, numSubsFinal,
-                 pVal, tv_steps=20, seed=0, picks=None, storage=F32):
+                 pVal, tv_steps=20, seed=0, picks=None, storage=F32, iterations=None, mu=None):
     """recon = OSC_TV_recon(pre, post, angles, geom, nIter, TV, sInit, sFinal, p)
-    (OSC_TV_recon.m:1-102) with all iterations on the device."""
+    (OSC_TV_recon.m:1-102) with all iterations on the device.
+
+    iterations=(k_begin, k_end) runs only those iterations (1-based,
+    inclusive) of the numIter-iteration schedule, from mu = the volume after
+    iteration k_begin - 1 (not needed when k_begin is 1): a checkpointed run
+    resumed this way gives the bits of the uninterrupted one."""
     pre = _single(pre, "pre")
     post = _single(post, "post")
     W, H, n = _dims3(pre)
@@ -216,6 +226,20 @@ def OSC_TV_recon(pre, post, scanAngles, geom, numIter, TV_constant, numSubsInit,
     if picks is None:
         picks = first_picks(seed, numSubsInit, numSubsFinal, pVal, int(numIter))
     picks = _vec(picks, np.int32, "picks")
+    if picks.size < int(numIter):
+        raise MexctError(EINVAL, "picks needs numIter entries")
+    if iterations is not None:
+        k0, k1 = (int(k) for k in iterations)
+        out = np.zeros((W, W, H), dtype=np.float32, order="F")
+        if k0 > 1:
+            if mu is None:
+                raise MexctError(EINVAL, "resuming after iteration 1 needs the volume mu")
+            out[...] = _single(mu, "mu")
+        check(lib().mexct_osc_tv_iterations(
+            _ptr(pre), _ptr(post), W, H, n, _ptr(ang), ang.size, float(np.float32(geom["SAD"])),
+            g, int(numIter), float(np.float32(TV_constant)), int(numSubsInit), int(numSubsFinal),
+            float(pVal), _ptr(picks), k0, k1, _ptr(out), opts(int(storage), int(tv_steps))))
+        return out
     mu = np.zeros((W, W, H), dtype=np.float32, order="F")
     check(lib().mexct_osc_tv_recon(_ptr(pre), _ptr(post), W, H, n, _ptr(ang), ang.size,
                                    float(np.float32(geom["SAD"])), g, int(numIter),

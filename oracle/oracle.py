@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# MEXCT_ORACLE_LIB: another build of the oracle (the sanitizer build)
+_LIB_PATH = os.environ.get("MEXCT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 CLAMP, BORDER = 0, 1

@@ -7,7 +7,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIBDIR = os.path.join(ROOT, "mex-cuda-ct_amd", "lib")
+# MEXCT_LIB_DIR: another build of the shim and gateways (the sanitizer build)
+LIBDIR = os.environ.get("MEXCT_LIB_DIR") or os.path.join(ROOT, "mex-cuda-ct_amd", "lib")
 
 CLASS = {np.dtype(np.float64): 6, np.dtype(np.float32): 7, np.dtype(np.int32): 12,
          np.dtype(np.uint8): 9}

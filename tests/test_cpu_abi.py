@@ -153,3 +153,79 @@ def test_server_create_rejects_bad_arguments():
     assert L.lib().mexct_server_create(None, 0, 1 << 30, 2, 1) == L.EINVAL
     assert L.lib().mexct_server_submit(None, None, None) == L.EINVAL
     assert L.lib().mexct_server_destroy(None) == L.OK
+
+
+def test_device_wrappers_check_sizes_before_the_library():
+    # The _dev entry points take device pointers without lengths, so the
+    # Context methods check every size first; these raise before any
+    # library call (the context below has no handle and no GPU behind it).
+    torch = pytest.importorskip("torch")
+    from mexct import device
+
+    ctx = object.__new__(device.Context)
+    ctx._h, ctx.order_with_torch, ctx.stream = None, False, None
+    z = lambda *s: torch.zeros(s)  # noqa: E731
+    ang4 = np.zeros(4, np.float32)
+    bad = [
+        lambda: ctx.g3d_fp(z(8, 8, 8), ang4, z(3, 6, 5, 7), 4, z(4, 5, 7)),   # 3 views of pts
+        lambda: ctx.g3d_fp(z(8, 8, 8), np.zeros(2), z(6, 5, 7), 4, z(4, 5, 7)),  # 2 angles
+        lambda: ctx.g3d_fp(z(8, 8, 8), ang4, z(6, 5, 7), 4, z(4, 5, 6)),      # out too small
+        lambda: ctx.g3d_bp(z(4, 5, 7), ang4, z(2, 6, 5, 7), 8, 8, z(8, 8, 8)),
+        lambda: ctx.g3d_bp(z(4, 5, 7), ang4, z(6, 5, 7), 8, 8, z(8, 8, 7)),
+        lambda: ctx.osc_iter(z(4, 8, 8), z(6, 4, 8), z(6, 4, 8), 3, [0, 1], np.zeros(6), 100, 2),
+        lambda: ctx.osc_iter(z(4, 8, 8), z(6, 4, 8), z(6, 4, 8), 2, [0, 1], np.zeros(5), 100, 2),
+        lambda: ctx.osc_iter(z(4, 8, 8), z(6, 4, 8), z(5, 4, 8), 2, [0, 1], np.zeros(6), 100, 2),
+        lambda: ctx.prefilter(z(6, 4, 8), 2, 100, np.zeros(5), 3, 1.0, z(6, 4, 8)),
+        lambda: ctx.prefilter(z(6, 4, 8), 2, 100, np.zeros(6), 3, 1.0, z(6, 4, 7)),
+        lambda: ctx.parker_weight(z(6, 4, 8), np.zeros(6), 2, 100, 1.0, z(5, 4, 8)),
+        lambda: ctx.fbp_recon(z(6, 4, 8), z(6, 4, 8), np.zeros(6), 2, 100, 3, 1.0, z(4, 8, 7)),
+        lambda: ctx.osc_tv_recon(z(6, 4, 8), z(6, 4, 8), np.zeros(6), 100, 2, 5, 0.1, 4, 2, 1.0,
+                                 [1, 1, 1], z(4, 8, 8)),
+        lambda: ctx.osc_tv_recon(z(6, 4, 8), z(6, 4, 8), np.zeros(6), 100, 2, 3, 0.1, 4, 2, 1.0,
+                                 [1, 1, 1], z(4, 8, 9)),
+        lambda: ctx.tvmin(z(8, 8), 0.1),
+    ]
+    for i, f in enumerate(bad):
+        with pytest.raises(mexct.MexctError) as e:
+            f()
+        assert e.value.code == L.EINVAL, i
+        assert "device tensor" not in e.value.msg, (i, e.value.msg)  # a size check, not _dptr
+
+
+def test_server_footprint_counts_the_abi_workspace():
+    # The reservation of a request covers what its entry point allocates:
+    # the FP's x-pair volume copy (about 2x the haloed volume) and the cone
+    # f32 BP's row-start chain (up to 1 GiB), computed from the ABI's own
+    # layouts.  No GPU is involved.
+    from mexct import serve
+
+    W = H = 256
+    n = 360
+    v, s = W * W * H * 4, W * H * n * 4
+    dual = (W + 3) * (W + 4) * (H + 4) * 2 * 4
+    fp = serve.request_footprint(L.OP_FP, W, H, n, geom=2)
+    assert fp >= v + s + dual
+    bp_cone = serve.request_footprint(L.OP_BP, W, H, n, geom=2)
+    bp_par = serve.request_footprint(L.OP_BP, W, H, n, geom=0)
+    assert bp_cone - bp_par > 500 << 20  # the chain
+    assert serve.request_footprint(L.OP_FDK, W, H, n, geom=2) > bp_cone
+    # OSC-TV: the fewer subsets, the larger the per-subset ratio copies
+    a = serve.request_footprint(L.OP_OSCTV, W, H, n, osc_iters=5, s_init=10, s_final=10)
+    b = serve.request_footprint(L.OP_OSCTV, W, H, n, osc_iters=5, s_init=10, s_final=1)
+    assert b > a > 6 * s
+    with pytest.raises(mexct.MexctError):
+        serve.request_footprint(9, W, H, n)
+
+
+def test_osc_tv_iteration_range_checked_before_the_device():
+    f = np.zeros(4, np.float32)
+    p = np.ones(5, np.int32)
+    for k0, k1 in ((0, 2), (3, 2), (2, 6)):
+        rc = L.lib().mexct_osc_tv_iterations(f.ctypes.data, f.ctypes.data, 2, 1, 2, f.ctypes.data,
+                                             4, 100.0, 2, 5, 0.1, 2, 2, 1.0, p.ctypes.data, k0, k1,
+                                             f.ctypes.data, None)
+        assert rc == L.EINVAL and "iterations" in L.lib().mexct_last_error().decode()
+    with pytest.raises(mexct.MexctError):
+        pre = np.ones((8, 4, 4), np.float32, order="F")
+        mexct.OSC_TV_recon(pre, pre, np.zeros(4, np.float32), {"type": "cone3d", "SAD": 100.0},
+                           5, 0.1, 2, 2, 1.0, picks=np.ones(5, np.int32), iterations=(2, 3))

@@ -142,3 +142,35 @@ def test_autotune_mode2_tunes_recurring_shapes(torch_cuda):
     torch.cuda.synchronize()
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
     ctx.close()
+
+
+def test_tuned_shape_keeps_a_fixed_zrange(torch_cuda):
+    # 2-byte storage steps the rows in texel space from the z-range start,
+    # so its bits depend on the z-range: a shape tuned by a caller that
+    # fixed zrange = 64 must not hand 64 to a later caller that asked for
+    # the default, and a later zrange = 64 call keeps 64
+    torch = torch_cuda
+    W, H, n, sad = 84, 100, 20, 350.0  # a shape no other test uses
+    ang = _angles(n)
+    sino = torch.from_numpy(_rng(97).uniform(-1, 1, (n, H, W)).astype(np.float32)).cuda()
+    plain = device.Context(0)
+    plain.set_autotune(0)
+    want = {}
+    for zr in (0, 64):
+        want[zr] = torch.zeros((H, W, W), dtype=torch.float32, device="cuda")
+        plain.bp(sino, sad, ang, 2, want[zr], storage=mexct.F16, zrange=zr)
+    torch.cuda.synchronize()
+    assert not torch.equal(want[0], want[64])  # the range moves the bits
+    tuned = device.Context(0)
+    tuned.set_autotune(1)
+    got = torch.zeros_like(want[0])
+    tuned.bp(sino, sad, ang, 2, got, storage=mexct.F16, zrange=64)  # tunes the shape
+    torch.cuda.synchronize()
+    assert mexct.tuned_config(1, 2, mexct.F16, W, H, n)[0] == 1
+    assert torch.equal(got, want[64])
+    for zr in (0, 64):
+        plain.bp(sino, sad, ang, 2, got, storage=mexct.F16, zrange=zr)
+        torch.cuda.synchronize()
+        assert torch.equal(got, want[zr]), f"zrange {zr} after tuning"
+    plain.close()
+    tuned.close()

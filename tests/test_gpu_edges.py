@@ -95,3 +95,47 @@ def test_gather_probe_unroll_variants(gpu):
         ctx.probe_gather(mexct.F32, 40, 36, 6, 0x40)
     ctx.sync()
     ctx.close()
+
+
+@pytest.mark.parametrize("storage", [mexct.F32, mexct.F16, mexct.BF16])
+def test_loads_only_forms(gpu, storage):
+    # The projectors' loads-only forms (their gather ceilings) run where the
+    # z-fastest FP layouts and the cone BP apply, fold the same taps on
+    # every call, reject the other cases, and leave the real operators'
+    # results as they were.
+    import torch
+
+    from mexct import _lib as L
+    from mexct import device
+
+    W, H, n, sad = 48, 40, 12, 300.0
+    ang = (np.arange(n) * 2 * np.pi / n).astype(np.float32)
+    vol_np = _vol(W, H, 71)
+    vol = torch.from_numpy(vol_np.T.copy()).cuda()
+    ctx = device.Context(0)
+    try:
+        a = torch.empty((n, H, W), dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for geom in (0, 1, 2):
+            ctx.fp_loads(vol, sad, ang, n, geom, a, storage=storage)
+            ctx.fp_loads(vol, sad, ang, n, geom, b, storage=storage)
+            torch.cuda.synchronize()
+            assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+        sino = torch.from_numpy(_rng(72).uniform(0, 1, (n, H, W)).astype(np.float32)).cuda()
+        va = torch.empty((H, W, W), dtype=torch.float32, device="cuda")
+        vb = torch.empty_like(va)
+        ctx.bp_loads(sino, sad, ang, 2, va, storage=storage)
+        ctx.bp_loads(sino, sad, ang, 2, vb, storage=storage)
+        torch.cuda.synchronize()
+        assert torch.equal(va.view(torch.int32), vb.view(torch.int32))
+        with pytest.raises(mexct.MexctError) as e:
+            ctx.bp_loads(sino, sad, ang, 0, va, storage=storage)
+        assert e.value.code == L.ENOTSUP
+        # the real operators afterwards: bit-identical to the oracle (f32)
+        if storage == mexct.F32:
+            ctx.fp(vol, sad, ang, n, 2, a)
+            torch.cuda.synchronize()
+            got = a.cpu().numpy().transpose(2, 1, 0)
+            assert np.array_equal(got, O.fp(vol_np, sad, ang, n, 2))
+    finally:
+        ctx.close()

@@ -136,6 +136,45 @@ def test_g3d_bp(gpu):
     _check(got, ref, "g3D BP", rel_max=1e-5, rel_l2=1e-6)
 
 
+def _g3d_bp_case():
+    # per-view random rays through and around a 37 x 37 x 45 volume (no
+    # multiple of the 16 x 16 x 32 bricks): many cross it, some miss it, and
+    # one per view ends 400 voxels short of it, so that its walk back to the
+    # box face is far longer than the volume's diagonal (the atomic kernel
+    # takes such rays)
+    W, H, n, M, N = 20, 12, 5, 37, 45
+    sino = np.asfortranarray(_rng(61).uniform(size=(W, H, n)).astype(np.float32))
+    pts = np.stack([_random_pts(W, H, 62 + k, spread=35.0) for k in range(n)], axis=3)
+    for k in range(n):
+        pts[0, 0, :, k] = [-1000.0, 0.5 + k, 0.25, -420.0, 0.5 + k, 0.25]
+    return sino, (np.arange(n) * 0.7).astype(np.float32), np.asfortranarray(pts), M, N
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_g3d_bp_modes(gpu, mode):
+    sino, ang, pts, M, N = _g3d_bp_case()
+    got = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=mode)
+    ref = O.g3d_bp(sino, ang, pts, M, N)
+    # float sums in another order than the oracle's canonical one (modes 0
+    # and 2), or exact fixed-point sums rounded once (mode 1)
+    _check(got, ref, f"g3D BP mode {mode}", rel_max=1e-5, rel_l2=1e-6)
+
+
+def test_g3d_bp_deterministic_bits_any_chunking(gpu, monkeypatch):
+    # fixed-point sums are exact, so the bits depend neither on the run nor
+    # on how the views are cut into chunks; the float bricks split in chunks
+    # stay within the tolerance
+    sino, ang, pts, M, N = _g3d_bp_case()
+    a = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=1)
+    b = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=1)
+    assert np.array_equal(a, b)
+    monkeypatch.setenv("MEXCT_G3D_CHUNK_VIEWS", "2")
+    c = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=1)
+    assert np.array_equal(a, c)
+    f = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=0)
+    _check(f, O.g3d_bp(sino, ang, pts, M, N), "g3D BP bricks, 2-view chunks", 1e-5, 1e-6)
+
+
 def _osc_inputs(W, H, n, seed):
     r = _rng(seed)
     mu = np.asfortranarray(r.uniform(0.0, 0.03, size=(W, W, H)).astype(np.float32))
@@ -337,6 +376,27 @@ def test_osc_tv_recon(gpu):
                              tv_steps=10, picks=picks)
     ref = O.osc_tv_recon(pre, post, ang, 300.0, 2, 3, 0.1, 4, 2, 0.5, 10, picks)
     _check(got, ref, "OSC-TV", 1e-4, 1e-5)
+
+
+@pytest.mark.parametrize("storage", [mexct.F32, mexct.F16])
+def test_osc_tv_resume_equals_uninterrupted(gpu, storage):
+    # checkpointing after iterations 1 and 3 of a 5-iteration schedule that
+    # changes its subset count every iteration (6 down to 2) and resuming
+    # from (mu, k) gives the bits of the uninterrupted run
+    W, H, n = 32, 16, 24
+    from mexct import phantoms
+    vol = phantoms.osc_phantom(W, seed=3)[:, :, :H].copy(order="F")
+    ang = _angles(n)
+    pre, post = phantoms.poisson_transmission(O.fp(vol, 300.0, ang, n, 2), 1e5, seed=33)
+    picks = mexct.first_picks(9, 6, 2, 1.0, 5)
+    args = (pre, post, ang, {"type": "cone3d", "SAD": 300.0}, 5, 0.1, 6, 2, 1.0)
+    kw = dict(tv_steps=4, picks=picks, storage=storage)
+    whole = mexct.OSC_TV_recon(*args, **kw)
+    mu = mexct.OSC_TV_recon(*args, iterations=(1, 1), **kw)
+    mu = mexct.OSC_TV_recon(*args, iterations=(2, 3), mu=mu, **kw)
+    mu = mexct.OSC_TV_recon(*args, iterations=(4, 5), mu=mu, **kw)
+    assert np.array_equal(mu, whole)
+    assert not np.array_equal(mexct.OSC_TV_recon(*args, iterations=(1, 3), **kw), whole)
 
 
 # relative RMSE bound vs the FP64 oracle: FP16 storage (11-bit significand)

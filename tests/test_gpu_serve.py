@@ -86,6 +86,43 @@ def test_alone_equals_in_mix(gpu):
     assert st["peak_reserved_bytes"] <= 8 << 30
 
 
+@pytest.mark.parametrize("kind", ["fp", "bp", "fdk", "osc"])
+def test_footprint_bounds_what_a_worker_holds(gpu, kind):
+    # cone f32 at 256^3 x 360: the x-pair FP volume (~140 MB over the plain
+    # haloed copy's ~70 MB) and the BP row-start chain (~0.6 GB) are far
+    # above the fixed slack, so a footprint that left them out would fall
+    # below what the worker holds after the run
+    from mexct import _lib as L
+    from mexct.serve import request_footprint
+
+    W = H = 256 if kind != "osc" else 128
+    n, sad = (360, 1000.0) if kind != "osc" else (90, 400.0)
+    ang = _angles(n)
+    print(f"footprint case {kind}: W={W} H={H} nProj={n}", flush=True)
+    with Server(hbm_budget=1 << 40, slots=1) as s:
+        if kind == "fp":
+            t = s.submit_fp(_vol(W, H, 61), sad, ang, n, 2)
+            fp = request_footprint(L.OP_FP, W, H, n, 2, sad)
+        elif kind == "bp":
+            t = s.submit_bp(_sino(W, H, n, 62), sad, ang, 2)
+            fp = request_footprint(L.OP_BP, W, H, n, 2, sad)
+        elif kind == "fdk":
+            t = s.submit_fdk(_sino(W, H, n, 63), {"type": "cone3d", "SAD": sad}, ang)
+            fp = request_footprint(L.OP_FDK, W, H, n, 2, sad)
+        else:
+            pre = np.asfortranarray(np.full((W, H, n), 1e4, np.float32))
+            post = np.asfortranarray(pre * 0.6)
+            t = s.submit_osc_tv(pre, post, ang, {"type": "cone3d", "SAD": sad}, 3, 0.1, 6, 2, 1.0,
+                                tv_steps=2, seed=5)
+            fp = request_footprint(L.OP_OSCTV, W, H, n, 2, sad, osc_iters=3, s_init=6, s_final=2)
+        s.result(t)
+        held = s.stats()["reserved_bytes"]  # an idle worker reserves what it holds
+    print(f"  held {held / 2**20:.1f} MiB, footprint {fp / 2**20:.1f} MiB", flush=True)
+    # an upper bound that is not wildly above: the BP counts the largest
+    # chain of any tuning candidate, about 1.2x what the default order holds
+    assert held <= fp <= 1.5 * held + (32 << 20)
+
+
 def test_server_fp_matches_oracle(gpu):
     W, H, n = 36, 30, 12
     vol, ang = _vol(W, H, 41), _angles(n)
@@ -170,3 +207,61 @@ def test_submit_rejects_over_budget(gpu):
         with pytest.raises(mexct.MexctError) as e:
             s.submit_bp(_sino(64, 64, 64, 1), 300.0, _angles(64), 2)
         assert e.value.code == -4
+
+
+def _dev(a):
+    """A MATLAB-layout array as a device tensor in the device layout:
+    [W, W, H] -> (H, W, W), [W, H, n] -> (n, H, W)."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+
+
+def _resident_args(kind, a):
+    if kind in ("fp", "bp", "fdk"):
+        return (_dev(a[0]),) + tuple(a[1:])
+    return (_dev(a[0]), _dev(a[1])) + tuple(a[2:])
+
+
+def test_resident_requests_equal_host_requests(gpu):
+    cases = _mix()
+    alone = [fn() for _, _, fn in cases]
+    with Server(hbm_budget=8 << 30, slots=3, max_group=3) as s:
+        tickets = [(_submit(s, kind, _resident_args(kind, a)), i)
+                   for i, (kind, a, _) in enumerate(cases)]
+        for t, i in tickets:
+            out = s.result(t)
+            assert out.is_cuda
+            got = out.cpu().numpy().T  # back to the MATLAB layout
+            assert np.array_equal(got, alone[i]), f"{cases[i][0]} case {i} differs when resident"
+
+
+def test_resident_same_shape_requests_run_batched(gpu):
+    W, H, n, sad = 40, 20, 16, 200.0
+    ang = _angles(n)
+    g = {"type": "cone3d", "SAD": sad}
+    sinos = [_sino(W, H, n, 700 + k) for k in range(6)]
+    alone = [mexct.CUDAmex_BP(mexct.preFilterSinogram(P, g, ang, "hann", 0.9), sad, ang, 2)
+             for P in sinos]
+    with Server(hbm_budget=1 << 30, slots=1, max_group=3) as s:
+        ts = [s.submit_fdk(_dev(P), g, ang, "hann", 0.9) for P in sinos]
+        outs = [s.result(t).cpu().numpy().T for t in ts]
+        st = s.stats()
+    for k in range(6):
+        assert np.array_equal(outs[k], alone[k]), f"fdk {k}"
+    assert st["batches"] >= 1 and st["batched"] >= 3
+
+
+def test_resident_request_rejects_host_arrays(gpu):
+    import ctypes as C
+
+    from mexct._lib import EINVAL, OP_FP, Request, lib
+    W, H, n = 16, 8, 4
+    ang = _angles(n)
+    vol = _vol(W, H, 1)
+    host_out = np.zeros((W, H, n), np.float32, order="F")
+    with Server(hbm_budget=1 << 30, slots=1) as s:
+        r = Request()
+        r.op, r.geom, r.W, r.H, r.nProj, r.sad = OP_FP, 2, W, H, n, 300.0
+        r.angles, r.input, r.output, r.resident = ang.ctypes.data, vol.ctypes.data, host_out.ctypes.data, 1
+        t = C.c_uint64()
+        assert lib().mexct_server_submit(s._h, C.byref(r), C.byref(t)) == EINVAL

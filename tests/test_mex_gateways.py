@@ -45,6 +45,21 @@ def test_osc_picks_must_be_int32():
              nlhs=0)
 
 
+def test_g3d_bp_per_view_pts_must_cover_every_view():
+    # [W,H,6,k] pts with k < nProj would be read past its end
+    W, H, n = 4, 3, 5
+    P = f32(np.zeros((W, H, n)))
+    pts = f32(np.zeros((W, H, 6, n - 1)))
+    with pytest.raises(MexCallError) as e:
+        call("CUDAmex_general3D_BP", P, f32(np.zeros(n)), pts, 4.0, 4.0)
+    assert "fewer views" in e.value.msg
+    with pytest.raises(mexct.MexctError) as e2:
+        mexct.CUDAmex_general3D_BP(P, f32(np.zeros(n)), pts, 4, 4)
+    assert e2.value.code == mexct._lib.EINVAL
+    with pytest.raises(MexCallError):
+        call("CUDAmex_general3D_FP", f32(np.zeros((4, 4, 4))), f32(np.zeros(n)), pts, float(n))
+
+
 @pytest.mark.gpu
 def test_gateways_match_host_entry_points(gpu):
     r = np.random.Generator(np.random.PCG64(3))
