@@ -212,6 +212,23 @@ int mexct_prefilter_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H,
                         int32_t geom, double sad, const float* angles, int32_t window,
                         double cutoff, double parker_q, float* out);
 
+/* ------------------------------------------------------------------ allocation-free pre-filter
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mexct_prefilter_dev with the same arguments, kernels
+ * and results, but its tables (filter taps, cosine weights, angle steps,
+ * Parker window) live in a fixed 16 MiB device array that every GPU holds
+ * from library load to process end, not in the context's workspace.  It
+ * calls no hipMalloc, hipFree or hipHostMalloc.  Tables needing more than
+ * 16 MiB (about 8*(2W-1) + 4*W*H (fan/cone) + 4*nProj + 4*W*nProj (Parker)
+ * bytes) are rejected with MEXCT_EINVAL.  Every call on a GPU shares that
+ * GPU's array and it keeps the last call's tables, so calls that overlap on
+ * one GPU race: serialise them.  A call whose tables differ from the last
+ * call's copies them in and synchronises the context's stream. */
+int mexct_prefilter_noalloc_dev(mexct_ctx* ctx, const float* sino, int64_t W, int64_t H,
+                                int64_t nProj, int32_t geom, double sad, const float* angles,
+                                int32_t window, double cutoff, double parker_q, float* out);
+
 /* ------------------------------------------------------------------ Parker weights
  * weightedSino = parker_weight(sino, angles, geom, q)
  * replaces fbp_reconstruction/parker_weight.m:1-79 (Wesarg et al. 2002;

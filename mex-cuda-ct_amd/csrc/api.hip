@@ -113,6 +113,60 @@ int store_bytes(int dt) { return dt == MEXCT_F32 ? 4 : 2; }
 
 }  // namespace
 
+// Fixed device scratch of mexct_prefilter_noalloc_dev: the pre-filter's
+// tables (filter taps, cosine weights, angle steps, Parker window) of the
+// last call on each GPU.  Every GPU the library runs on holds its own 16 MiB
+// copy from module load to process end.  In host code `scratch` only names
+// the symbol; kernels and copies get its device address from
+// scratch_address().
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// What `scratch` holds on one device: the tables of the call described by
+// the key, when valid.
+struct ScratchState {
+  unsigned char* ptr = nullptr;  // device address of `scratch` on this device
+  bool valid = false;
+  int64_t W = -1, H = -1, nProj = -1;
+  int geom = -1, window = -1;
+  double sad = 0.0, cutoff = 0.0, q = 0.0;
+  std::vector<float> angles;
+};
+std::mutex g_scratch_mu;  // guards g_scratch (host side only)
+ScratchState g_scratch[kMaxDevices];
+
+// Looks up the device address of `scratch` once per device, with that
+// device current.  The caller holds g_scratch_mu.
+int scratch_address_locked(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(MEXCT_EINVAL, "device %d beyond the %d the scratch table covers", device, kMaxDevices);
+  ScratchState& s = g_scratch[device];
+  if (!s.ptr) {
+    DevGuard g(device);
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(MEXCT_EHIP, "hipGetSymbolAddress(scratch) on device %d failed (%s)", device,
+                  hipGetErrorString(e));
+    }
+    s.ptr = (unsigned char*)p;
+  }
+  *out = s.ptr;
+  return MEXCT_OK;
+}
+
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return scratch_address_locked(device, out);
+}
+
+}  // namespace
+
 struct mexct_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -227,6 +281,8 @@ int ctx_init(mexct_ctx* c, int device, void* stream) {
     return fail(MEXCT_EINVAL, "device %d out of range (%d devices)", device, ndev);
   c->device = device;
   DevGuard g(device);
+  unsigned char* sp;  // set-up of the fixed scratch, once per device
+  RC(scratch_address(device, &sp));
   if (stream) {
     c->stream = (hipStream_t)stream;
     c->own_stream = false;
@@ -1507,6 +1563,103 @@ int mexct_prefilter(const float* sino, int64_t W, int64_t H, int64_t nProj, int3
                    n_angles, geom, sad, angles, window, cutoff, parker_q, (float*)dout));
   HIPCHK(hipMemcpyAsync(out, dout, sb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return MEXCT_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free pre-filter
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte offsets of the pre-filter tables inside `scratch`.
+struct ScratchLayout {
+  size_t taps, wgt, dA, pk, win, total;
+};
+ScratchLayout scratch_layout(int64_t W, int64_t H, int64_t nProj, int geom, bool parker) {
+  ScratchLayout l;
+  size_t off = 0;
+  l.taps = off;
+  off += align256(sizeof(double) * (size_t)(2 * W - 1));
+  l.wgt = off;
+  if (geom != GEOM_PAR) off += align256(sizeof(float) * (size_t)W * H);
+  l.dA = off;
+  off += align256(sizeof(float) * (size_t)nProj);
+  l.pk = off;
+  l.win = off;
+  if (parker) {
+    l.win = off + parker_col_bytes(W) + parker_beta_bytes(nProj);
+    off = l.win + align256(sizeof(float) * (size_t)W * nProj);
+  }
+  l.total = off;
+  return l;
+}
+}  // namespace
+
+int mexct_prefilter_noalloc_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H,
+                                int64_t nProj, int32_t geom, double sad, const float* angles,
+                                int32_t window, double cutoff, double parker_q, float* out) {
+  RC(check_prefilter_args(geom, W, H, nProj, window));
+  const bool pk = parker_on(parker_q);
+  const ScratchLayout l = scratch_layout(W, H, nProj, geom, pk);
+  if (l.total > kScratchBytes)
+    return fail(MEXCT_EINVAL,
+                "the filter tables need %zu bytes; the allocation-free pre-filter has %zu "
+                "(W=%lld H=%lld nProj=%lld%s)",
+                l.total, kScratchBytes, (long long)W, (long long)H, (long long)nProj,
+                pk ? ", Parker window" : "");
+  if (!c) return fail(MEXCT_EINVAL, "ctx is null");
+  if (!sino || !out || !angles) return fail(MEXCT_EINVAL, "null array argument");
+  DevGuard g(c->device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  unsigned char* base;
+  RC(scratch_address_locked(c->device, &base));
+  ScratchState& s = g_scratch[c->device];
+  const bool same = s.valid && s.W == W && s.H == H && s.nProj == nProj && s.geom == geom &&
+                    s.window == window && s.sad == sad && s.cutoff == cutoff &&
+                    (pk ? s.q == parker_q : !parker_on(s.q)) && s.angles.size() == (size_t)nProj &&
+                    memcmp(s.angles.data(), angles, (size_t)nProj * sizeof(float)) == 0;
+  if (!same) {
+    // The tables of mexct_prefilter_dev, copied from pageable host memory;
+    // the stream is synchronised before the host copies go out of scope.
+    s.valid = false;
+    std::vector<double> full, taps;
+    design_filter(window, (int)W, cutoff, full);
+    conv_taps(full, (int)W, taps);
+    HIPCHK(hipMemcpyAsync(base + l.taps, taps.data(), taps.size() * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+    std::vector<float> w, dA;
+    if (geom != GEOM_PAR) {
+      cosine_weights(W, H, geom, sad, w);
+      HIPCHK(hipMemcpyAsync(base + l.wgt, w.data(), w.size() * sizeof(float),
+                            hipMemcpyHostToDevice, c->stream));
+    }
+    angle_steps(nProj, angles, dA);
+    HIPCHK(hipMemcpyAsync(base + l.dA, dA.data(), dA.size() * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+    std::vector<char> h;
+    if (pk) {
+      ParkerConsts k;
+      parker_tables(W, nProj, nProj, geom, sad, angles, parker_q, h, k);
+      HIPCHK(hipMemcpyAsync(base + l.pk, h.data(), h.size(), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(launch_parker((const float4*)(base + l.pk),
+                           (const float*)(base + l.pk + parker_col_bytes(W)), k, (int)W,
+                           (int)nProj, (float*)(base + l.win), c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    s.W = W;
+    s.H = H;
+    s.nProj = nProj;
+    s.geom = geom;
+    s.window = window;
+    s.sad = sad;
+    s.cutoff = cutoff;
+    s.q = pk ? parker_q : MEXCT_NO_PARKER;
+    s.angles.assign(angles, angles + nProj);
+    s.valid = true;
+  }
+  FilterIn in{sino, nullptr, geom != GEOM_PAR ? (const float*)(base + l.wgt) : nullptr,
+              pk ? (const float*)(base + l.win) : nullptr};
+  HIPCHK(launch_prefilter(in, out, (const double*)(base + l.taps), (const float*)(base + l.dA),
+                          (int)W, (int)H, (int)nProj, c->stream));
   return MEXCT_OK;
 }
 

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mexct_tvmin_dev": [_vp, _vp, _i64, _i64, _i64, _f, _op],
     "mexct_prefilter": [_vp, _i64, _i64, _i64, _i32, _d, _vp, _i64, _i32, _d, _d, _vp],
     "mexct_prefilter_dev": [_vp, _vp, _i64, _i64, _i64, _i32, _d, _vp, _i32, _d, _d, _vp],
+    "mexct_prefilter_noalloc_dev": [_vp, _vp, _i64, _i64, _i64, _i32, _d, _vp, _i32, _d, _d, _vp],
     "mexct_parker_weight": [_vp, _i64, _i64, _i64, _vp, _i64, _i32, _d, _d, _vp],
     "mexct_parker_weight_dev": [_vp, _vp, _i64, _i64, _i64, _vp, _i32, _d, _d, _vp],
     "mexct_fbp_recon": [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i32, _d, _i32, _d, _d, _vp, _op],

@@ -155,6 +155,18 @@ def test_server_create_rejects_bad_arguments():
     assert L.lib().mexct_server_destroy(None) == L.OK
 
 
+def test_prefilter_noalloc_rejects_tables_beyond_scratch_before_the_device():
+    # the size check runs before the context is looked at, so it needs no GPU
+    rc = L.lib().mexct_prefilter_noalloc_dev(None, None, 2048, 2048, 10, 2, 1000.0, None, 3, 1.0,
+                                             -1.0, None)
+    assert rc == L.EINVAL
+    msg = L.lib().mexct_last_error().decode()
+    assert "allocation-free" in msg and str(16 << 20) in msg
+    rc = L.lib().mexct_prefilter_noalloc_dev(None, None, 64, 64, 10, 2, 1000.0, None, 3, 1.0,
+                                             -1.0, None)
+    assert rc == L.EINVAL and "ctx is null" in L.lib().mexct_last_error().decode()
+
+
 def test_device_wrappers_check_sizes_before_the_library():
     # The _dev entry points take device pointers without lengths, so the
     # Context methods check every size first; these raise before any
