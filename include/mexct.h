@@ -150,14 +150,17 @@ int mexct_g3d_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t
  * replaces projection_kernels/CUDAmex_general3D_BP.cu:194-282 (kernel :61-188).
  * Every 0.5-voxel sample of every ray adds the reference's two weighted
  * corners to [M,M,N].  opts->g3d_bp selects how the sums are formed:
- *   MEXCT_G3D_BRICKS: per 16x16x32 voxel brick in LDS, written once (float
- *     sums; their order is not fixed, as the reference's atomicAdd order);
- *   MEXCT_G3D_DETERMINISTIC: the same with 64-bit fixed-point sums, exact
- *     in any order, so every run gives the same bits (up to 2^23 samples per
- *     voxel; the fixed-point step is max|P| * 2^-39 or finer);
- *   MEXCT_G3D_ATOMICS: the ray-driven scatter with float atomics.
- * Volumes whose bricks exceed the binning pass's LDS (about 800^3) take
- * the atomics (MEXCT_ENOTSUP for the deterministic mode). */
+ *   MEXCT_G3D_BRICKS: per 16x16x12 voxel brick in LDS, written once, with
+ *     64-bit fixed-point sums that are exact in any order, so every run
+ *     gives the same bits (the fixed-point step is max|P| * 2^-27 or finer,
+ *     per merged run of at most 4 samples in one cell; up to 2^32 samples
+ *     per voxel);
+ *   MEXCT_G3D_DETERMINISTIC: the same, and MEXCT_ENOTSUP where the bricks
+ *     cannot run instead of the atomics;
+ *   MEXCT_G3D_ATOMICS: the ray-driven scatter with float atomics (the
+ *     reference's; its sum order is not fixed).
+ * Volumes whose bricks exceed the binning pass's LDS (about 730^3) take
+ * the atomics in MEXCT_G3D_BRICKS mode. */
 #define MEXCT_G3D_BRICKS 0
 #define MEXCT_G3D_DETERMINISTIC 1
 #define MEXCT_G3D_ATOMICS 2

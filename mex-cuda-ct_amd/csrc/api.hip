@@ -92,8 +92,9 @@ enum Slot {
   WS_FILTERED, // FBP: the filtered sinogram handed to the backprojector
   WS_G3D_CK,   // general-3D BP bricks: ray checkpoints
   WS_G3D_MASK, // (brick, tile) bits
-  WS_G3D_HALO, // halo planes (float, or 64-bit fixed point)
-  WS_G3D_ACC,  // deterministic mode: 64-bit fixed-point volume
+  WS_G3D_HALO, // halo planes (64-bit fixed point)
+  WS_G3D_ACC,  // 64-bit fixed-point volume
+  WS_OUTPTRS,  // batches with an output pointer per item
   WS_COUNT
 };
 
@@ -380,13 +381,26 @@ hipError_t build_fp_volume(int layout, const void* in, int in_dt, int64_t sy, in
 constexpr int kMaxItems = 256;  // batch items per launch
 constexpr int kSlotVol = 64, kSlotSino = 64 + kMaxItems;
 int max_bits(mexct_ctx* c, const float* x, size_t n, int k, uint32_t** out, int items = 1,
-             size_t stride = 0) {
+             size_t stride = 0, const void* const* ptrs = nullptr) {
   void* m;
   RC(c->get(WS_MAXBITS, 4 * (64 + 2 * kMaxItems), &m));
   uint32_t* p = (uint32_t*)m + k;
   HIPCHK(launch_fill((float*)p, 0.f, (size_t)items, c->stream));
-  for (int g = 0; g < items; ++g) HIPCHK(launch_absmax(x + g * stride, n, p + g, c->stream));
+  for (int g = 0; g < items; ++g)
+    HIPCHK(launch_absmax(ptrs ? (const float*)ptrs[g] : x + g * stride, n, p + g, c->stream));
   *out = p;
+  return MEXCT_OK;
+}
+
+// A batch's per-item output pointers (host array of device pointers) as a
+// device array in the workspace; null when the outputs are one array.
+int upload_ptrs(mexct_ctx* c, float* const* ptrs, int items, float* const** dev) {
+  *dev = nullptr;
+  if (!ptrs) return MEXCT_OK;
+  void* d;
+  RC(c->get(WS_OUTPTRS, sizeof(float*) * (size_t)kMaxItems, &d));
+  HIPCHK(hipMemcpyAsync(d, ptrs, sizeof(float*) * (size_t)items, hipMemcpyHostToDevice, c->stream));
+  *dev = (float* const*)d;
   return MEXCT_OK;
 }
 
@@ -665,14 +679,19 @@ int64_t mexct_ctx_workspace_bytes(const mexct_ctx* ctx) {
 namespace {
 // FP of `items` volumes of one geometry stored one after another (vol_dtype
 // elements), outputs one after another: one launch for all of them.
+// in_ptrs / out_ptrs (host arrays of `items` device pointers, or null):
+// item i's volume and output, in place of vol and out stored one after
+// another.
 int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t W, int64_t H,
            float sad, const float* angles, int32_t nProj, int32_t geom, float* out,
-           const mexct_opts* opts, bool loads_only = false) {
+           const mexct_opts* opts, bool loads_only = false, const void* const* in_ptrs = nullptr,
+           float* const* out_ptrs = nullptr) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims W=%lld H=%lld nProj=%d", (long long)W, (long long)H, nProj);
   if (items < 1 || items > kMaxItems) return fail(MEXCT_EINVAL, "batch of %d volumes (1..%d)", items, kMaxItems);
-  if (nProj > 0 && (!vol || !angles || !out)) return fail(MEXCT_EINVAL, "null array argument");
+  if (nProj > 0 && ((!vol && !in_ptrs) || !angles || (!out && !out_ptrs)))
+    return fail(MEXCT_EINVAL, "null array argument");
   RC(check_dtype(vol_dtype, "vol"));
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
@@ -692,10 +711,11 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
     RC(c->get(WS_VOL, lbytes * items, &pad));
     uint32_t* vmax = nullptr;
     if (st == MEXCT_F16 && vol_dtype == MEXCT_F32)
-      RC(max_bits(c, (const float*)vol, vel, kSlotVol, &vmax, items, vel));
+      RC(max_bits(c, (const float*)vol, vel, kSlotVol, &vmax, items, vel, in_ptrs));
     const size_t in_es = (size_t)store_bytes(vol_dtype);
     for (int i = 0; i < items; ++i)
-      HIPCHK(build_fp_volume(layout, (const char*)vol + i * vel * in_es, vol_dtype, W, W * W,
+      HIPCHK(build_fp_volume(layout, in_ptrs ? in_ptrs[i] : (const char*)vol + i * vel * in_es,
+                             vol_dtype, W, W * W,
                              (char*)pad + i * lbytes, st, W, H, ADDR_CLAMP, c->stream,
                              vmax ? vmax + i : nullptr));
     RC(c->sincos_table(angles, nProj, &sc));
@@ -715,6 +735,7 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
     z.items = items;
     z.vol_stride = lbytes;
     z.out_stride = (uint64_t)W * H * nProj;
+    RC(upload_ptrs(c, out_ptrs, items, &z.outs));
     z.loads_only = loads_only;
     const TuneKey key{MEXCT_OP_FP, geom, st, W, H, nProj};
     TuneVal tv;
@@ -740,12 +761,13 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
   if (items != 1) return fail(MEXCT_ENOTSUP, "volume too large to batch");
   if (loads_only) return fail(MEXCT_ENOTSUP, "no loads-only form for the x-fastest projector");
   RC(c->get(WS_VOL, pbytes, &pad));
-  HIPCHK(launch_pad3(vol, vol_dtype, pad, st, (int)W, (int)W, (int)H, ADDR_CLAMP, c->stream));
+  HIPCHK(launch_pad3(in_ptrs ? in_ptrs[0] : vol, vol_dtype, pad, st, (int)W, (int)W, (int)H,
+                     ADDR_CLAMP, c->stream));
   RC(c->sincos_table(angles, nProj, &sc));
   FPLaunch a;
   a.vol = pad;
   a.st_dt = st;
-  a.out = out;
+  a.out = out_ptrs ? out_ptrs[0] : out;
   a.sc = sc;
   a.geom = geom;
   a.mode = ADDR_CLAMP;
@@ -808,14 +830,17 @@ namespace {
 // BP of `items` sinograms of one geometry stored one after another
 // (sino_dtype elements), volumes one after another: one launch per chunk of
 // views for all of them.
+// in_ptrs / out_ptrs as for fp_run: item i's sinogram and volume.
 int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_t W, int64_t H,
            int64_t nProj, float sad, const float* angles, int32_t geom, float* out,
-           const mexct_opts* opts, bool loads_only = false) {
+           const mexct_opts* opts, bool loads_only = false, const void* const* in_ptrs = nullptr,
+           float* const* out_ptrs = nullptr) {
   if (!c) return fail(MEXCT_EINVAL, "ctx is null");
   RC(check_geom(geom));
   if (W < 1 || H < 1 || nProj < 0) return fail(MEXCT_EINVAL, "bad dims");
   if (items < 1 || items > kMaxItems) return fail(MEXCT_EINVAL, "batch of %d sinograms (1..%d)", items, kMaxItems);
-  if (!out || (nProj > 0 && (!sino || !angles))) return fail(MEXCT_EINVAL, "null array argument");
+  if ((!out && !out_ptrs) || (nProj > 0 && ((!sino && !in_ptrs) || !angles)))
+    return fail(MEXCT_EINVAL, "null array argument");
   RC(check_dtype(sino_dtype, "sino"));
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
@@ -839,11 +864,12 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   uint32_t* smax = nullptr;
   const size_t sel = (size_t)W * H * nProj;  // elements of one input sinogram
   if (st == MEXCT_F16 && sino_dtype == MEXCT_F32 && nProj > 0)
-    RC(max_bits(c, (const float*)sino, sel, kSlotSino, &smax, items, sel));
+    RC(max_bits(c, (const float*)sino, sel, kSlotSino, &smax, items, sel, in_ptrs));
   const size_t in_es = (size_t)store_bytes(sino_dtype);
   if (nProj > 0)
     for (int i = 0; i < items; ++i)
-      HIPCHK(launch_pad2((const char*)sino + i * sel * in_es, sino_dtype, (char*)pad + i * sino_stride,
+      HIPCHK(launch_pad2(in_ptrs ? in_ptrs[i] : (const char*)sino + i * sel * in_es, sino_dtype,
+                         (char*)pad + i * sino_stride,
                          st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
                          smax ? smax + i : nullptr, rows));
   const float2* sc = nullptr;
@@ -866,6 +892,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   a.items = items;
   a.sino_stride = sino_stride;
   a.out_stride = (uint64_t)W * W * H;
+  RC(upload_ptrs(c, out_ptrs, items, &a.outs));
   a.loads_only = loads_only;
   const TuneKey key{MEXCT_OP_BP, geom, st, W, H, nProj};
   TuneVal tv;
@@ -946,6 +973,23 @@ int mexct_bp_batch_dev(mexct_ctx* c, const void* sinos, int32_t sino_dtype, int3
                        int32_t geom, float* outs, const mexct_opts* opts) {
   return bp_run(c, sinos, sino_dtype, count, W, H, nProj, sad, angles, geom, outs, opts);
 }
+
+extern "C++" {
+namespace mexct {
+int fp_batch_ptrs(mexct_ctx* c, const void* const* vols, int32_t vol_dtype, int32_t count,
+                  int64_t W, int64_t H, float sad, const float* angles, int32_t nProj,
+                  int32_t geom, float* const* outs, const mexct_opts* opts) {
+  return fp_run(c, nullptr, vol_dtype, count, W, H, sad, angles, nProj, geom, nullptr, opts, false,
+                vols, outs);
+}
+int bp_batch_ptrs(mexct_ctx* c, const void* const* sinos, int32_t sino_dtype, int32_t count,
+                  int64_t W, int64_t H, int64_t nProj, float sad, const float* angles,
+                  int32_t geom, float* const* outs, const mexct_opts* opts) {
+  return bp_run(c, nullptr, sino_dtype, count, W, H, nProj, sad, angles, geom, nullptr, opts, false,
+                sinos, outs);
+}
+}  // namespace mexct
+}  // extern "C++"
 
 // ------------------------------------------------------------------ autotuner
 int mexct_set_autotune(int32_t mode) {
@@ -1114,7 +1158,6 @@ int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int6
     return fail(MEXCT_ENOTSUP, "deterministic general-3D BP: %u bricks exceed the binning LDS",
                 plan.nbricks);
   if (mode != MEXCT_G3D_ATOMICS && plan.ok) {
-    const bool det = mode == MEXCT_G3D_DETERMINISTIC;
     G3DBrickLaunch L;
     L.plan = plan;
     L.sino = sino;
@@ -1126,24 +1169,19 @@ int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int6
     L.nviews = (int)nProj;
     L.M = (int)M;
     L.N = (int)N;
-    void *ck, *mk, *hl, *acc = nullptr;
+    void *ck, *mk, *hl, *acc;
     RC(c->get(WS_G3D_CK, plan.ck_bytes, &ck));
     RC(c->get(WS_G3D_MASK, plan.mask_bytes, &mk));
-    RC(c->get(WS_G3D_HALO, plan.halo_elems * (det ? 8 : 4), &hl));
-    if (det) RC(c->get(WS_G3D_ACC, (size_t)M * M * N * 8, &acc));
+    RC(c->get(WS_G3D_HALO, plan.halo_elems * 8, &hl));
+    RC(c->get(WS_G3D_ACC, (size_t)M * M * N * 8, &acc));
     L.mask = (uint32_t*)mk;
     L.ck = (float*)ck;
     L.out = out;
-    L.deterministic = det;
-    L.halo = det ? nullptr : (float*)hl;
-    L.halo64 = det ? (int64_t*)hl : nullptr;
+    L.halo64 = (int64_t*)hl;
     L.out64 = (int64_t*)acc;
-    L.pmax = nullptr;
-    if (det) {
-      uint32_t* pm;
-      RC(max_bits(c, sino, (size_t)W * H * nProj, 2, &pm));
-      L.pmax = pm;
-    }
+    uint32_t* pm;
+    RC(max_bits(c, sino, (size_t)W * H * nProj, 2, &pm));
+    L.pmax = pm;
     HIPCHK(launch_g3d_bp_bricks(L, c->stream));
     return MEXCT_OK;
   }

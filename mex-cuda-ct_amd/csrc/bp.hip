@@ -76,6 +76,7 @@ struct BPKArgs {
   uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
   uint64_t sino_stride;  // bytes between the items' haloed sinograms
   uint64_t out_stride;   // floats between the items' volumes
+  float* const* outs;    // non-null: item i's volume at outs[i]
   int loads_only;        // launch the loads-only form
 };
 
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   const int z0 = (int)zr * ZR;
   const int zc = min(ZR, a.H - z0);
   const size_t WW = (size_t)a.W * a.W;
-  float* o = a.out + item * a.out_stride + (size_t)ix + (size_t)a.W * iy + WW * z0;
+  float* const ob = a.outs ? a.outs[item] : a.out + item * a.out_stride;
+  float* o = ob + (size_t)ix + (size_t)a.W * iy + WW * z0;
   const uint32_t hw = (uint32_t)a.W / 2u - 1u;
   const uint32_t dx = (uint32_t)ix - hw, dy = (uint32_t)iy - hw;
   if (!(dx * dx + dy * dy < hw * hw)) {  // outside the circle of reconstruction
@@ -461,6 +463,7 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.ncols = tpad * tpad * 64u;
   k.sino_stride = a.sino_stride;
   k.out_stride = a.out_stride;
+  k.outs = a.outs;
   k.loads_only = a.loads_only ? 1 : 0;
   if (a.loads_only && a.geom != GEOM_CONE) return hipErrorInvalidValue;
   if (k.n_logical == 0) return hipSuccess;

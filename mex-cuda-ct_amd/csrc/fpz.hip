@@ -63,6 +63,7 @@ struct FPZArgs {
   uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
   uint64_t vol_stride;   // bytes between the items' volumes
   uint64_t out_stride;   // floats between the items' outputs
+  float* const* outs;    // non-null: item i's output at outs[i]
 };
 
 template <typename S>
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   if (ix >= a.W || iy >= a.H) return;
   const int view = a.first + (int)jv * a.stride;
   const size_t p = (size_t)view * a.W * a.H + (size_t)iy * a.W + ix;
-  float* const out = a.out + item * a.out_stride;
+  float* const out = a.outs ? a.outs[item] : a.out + item * a.out_stride;
   float preval = 0.f, postval = 0.f;
   size_t q = 0;
   typedef typename OscPad<S>::T P;
@@ -687,6 +688,7 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   set_view_chunk(k, a.vchunk_log2 >= 0 ? (uint32_t)a.vchunk_log2 : kViewChunkLog2, (uint32_t)items);
   k.vol_stride = a.vol_stride;
   k.out_stride = a.out_stride;
+  k.outs = a.outs;
   if ((uint64_t)k.item_blocks * (uint64_t)items >= (1ull << 31)) return hipErrorInvalidValue;
   if (k.n_logical == 0) return hipSuccess;
   if (a.loads_only) {

@@ -7,14 +7,14 @@
 // the rays of one view barely overlap, so staging one view's rays in LDS
 // merges nothing, and one float atomic per sample corner costs ~90% of the
 // ray-driven kernel's time.  Here the volume is cut into bricks of
-// 16 x 16 x 32 voxels; a brick is one workgroup that adds, in LDS, every
+// 16 x 16 x 12 voxels; a brick is one workgroup that adds, in LDS, every
 // sample whose cell (iL, jL, kL) lies in it, from every ray of every view,
 // and writes each of its voxels once with a plain store.  Per chunk of views:
 //
 //   bin    one thread per ray marches the reference's running sum once,
-//          keeps the position every kCk samples (checkpoints) and marks, for
-//          its 8x8 tile of detector pixels, each brick that owns one of its
-//          samples (one bit per (brick, tile));
+//          keeps the position every kCk samples (checkpoints) and its set-up,
+//          and marks, for its 8x8 tile of detector pixels, each brick that
+//          owns one of its samples (one bit per (brick, tile));
 //   brick  for each tile marked for it, each ray of the tile bounds the
 //          samples the brick can own from its exact line (widened by the
 //          running sum's drift), resumes the running sum at the checkpoint
@@ -23,13 +23,16 @@
 //          belongs to the brick above and goes to a halo plane;
 //   halo   after the last chunk each halo plane is added to its plane.
 //
-// A brick's LDS sums are float atomics (their order is not fixed, as the
-// reference's atomicAdd order is not) or, in the deterministic mode,
-// 64-bit fixed-point integers whose sum is exact in any order, so the
-// result has the same bits on every run.  Rays of more samples than the
-// volume's diagonal admits cannot cross it; they go through the ray-driven
-// atomic kernel (launch_g3d_bp), which checks every sample as the reference
-// does.
+// The LDS sums are 64-bit fixed-point integers (ds_add_u64): on gfx950 an
+// LDS float atomic add costs about 15 times a 64-bit integer one (measured:
+// 2.0e11 against 3.0e12 lane additions per second over the chip), and the
+// integer sums are exact in any order, so every run gives the same bits.
+// The brick's z extent of 12 keeps a workgroup's LDS at 26 KB, six per CU:
+// the loop is latency-bound, and 16 x 16 x 12 ran the helical workload in
+// 63 ms against 71 ms for 16 x 16 x 16 and 93 ms for 16 x 16 x 32.  Rays of
+// more samples than the volume's diagonal admits cannot cross it; they go
+// through the ray-driven atomic kernel (launch_g3d_bp, with the same fixed
+// point), which checks every sample as the reference does.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -41,7 +44,7 @@
 
 namespace mexct {
 
-constexpr int BX = 16, BY = 16, BZ = 32;  // brick (voxels)
+constexpr int BX = 16, BY = 16, BZ = 12;  // brick (voxels)
 constexpr int BZH = BZ + 1;               // LDS z extent: the brick plus its halo plane
 constexpr int kTileSide = 8;              // tiles of 8 x 8 detector pixels: one wave
 
@@ -62,11 +65,10 @@ struct G3DBrickArgs {
   size_t ck_plane;
   float* rays;       // per ray of the chunk: inc x, y, z and the sample count (bits)
   size_t ray_plane;  // planes of ray_plane entries, one per (tile, lane)
-  float* out;        // [M,M,N] (float mode)
-  float* halo;       // [nbz][M][M]: corners kL + 1 of each brick's top plane
-  int64_t* out64;    // deterministic mode: fixed-point accumulators
-  int64_t* halo64;
-  const uint32_t* pmax;  // deterministic mode: bits of max |P|
+  float* out;        // [M,M,N] floats, written at the end
+  int64_t* out64;    // [M,M,N] fixed-point sums
+  int64_t* halo64;   // [nbz][M][M]: corners kL + 1 of each brick's top plane
+  const uint32_t* pmax;  // bits of max |P| (the fixed point's scale)
   int accumulate;        // a later chunk: add to out/halo
 };
 
@@ -188,39 +190,23 @@ __device__ __forceinline__ float axis_hi(int lo, int B, int n) {
   return lo + B <= n - 1 ? (float)(lo + B) : __uint_as_float(__float_as_uint((float)(n - 1)) + 1u);
 }
 
-template <bool DET>
-struct BrickAcc {
-  typedef float T;
-};
-template <>
-struct BrickAcc<true> {
-  typedef unsigned long long T;
-};
-
 // Adds one (register-merged) cell's two corners to the brick's LDS sums
 // (hi is 0 when both corners are one voxel: adding it changes nothing).
-template <bool DET>
-__device__ __forceinline__ void lds_add(typename BrickAcc<DET>::T* lds, uint32_t idx, float lo,
-                                        float hi, float scale) {
-  if constexpr (DET) {
-    atomicAdd(&lds[idx], g3d_fixed(lo * scale));
-    atomicAdd(&lds[idx + 1], g3d_fixed(hi * scale));
-  } else {
-    atomicAdd(&lds[idx], lo);
-    atomicAdd(&lds[idx + 1], hi);
-  }
+__device__ __forceinline__ void lds_add(unsigned long long* lds, uint32_t idx, float lo, float hi,
+                                        float scale) {
+  atomicAdd(&lds[idx], g3d_fixed(lo * scale));
+  atomicAdd(&lds[idx + 1], g3d_fixed(hi * scale));
 }
 
-template <bool DET>
 __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
-  typedef typename BrickAcc<DET>::T Acc;
+  typedef unsigned long long Acc;
   __shared__ Acc lds[BX * BY * BZH];
   const uint32_t brick = xcd_remap(blockIdx.x, gridDim.x);
   if (brick >= a.nbricks) return;
   const uint32_t bxi = brick % a.nbx, byi = (brick / a.nbx) % a.nby, bzi = brick / (a.nbx * a.nby);
   for (uint32_t i = threadIdx.x; i < (uint32_t)(BX * BY * BZH); i += 256u) lds[i] = (Acc)0;
   __syncthreads();
-  const float scale = DET ? g3d_fixed_scale(a.pmax) : 1.f;
+  const float scale = g3d_fixed_scale(a.pmax);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const float lox = (float)(bxi * BX), loy = (float)(byi * BY), loz = (float)(bzi * BZ);
   // a sample is added here when its cell (iL, jL, kL) passes the reference's
@@ -304,7 +290,7 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
         const uint32_t idx =
             (uint32_t)mad_i24(cvt_flr(fx), BY * BZH, mad_i24(cvt_flr(fy), BZH, cvt_flr(fz))) - base;
         const bool same = idx == pend;
-        if (in && !same && pend != ~0u) lds_add<DET>(lds, pend, wlo, whi, scale);
+        if (in && !same && pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
         wlo = in ? (same ? wlo + lo : lo) : wlo;
         whi = in ? (same ? whi + hi : hi) : whi;
         pend = in ? idx : pend;
@@ -312,7 +298,7 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
         cur.y = cur.y + inc.y;
         cur.z = cur.z + inc.z;
       }
-      if (pend != ~0u) lds_add<DET>(lds, pend, wlo, whi, scale);
+      if (pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
     }
   }
   __syncthreads();
@@ -322,13 +308,9 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
     const int li = (int)(e % BX), lj = (int)((e / BX) % BY), lk = (int)(e / (BX * BY));
     const int i = (int)bxi * BX + li, j = (int)byi * BY + lj, k = (int)bzi * BZ + lk;
     if (i >= a.M || j >= a.M || k >= a.N) continue;
-    const Acc v = lds[(li * BY + lj) * BZH + lk];
+    const int64_t v = (int64_t)lds[(li * BY + lj) * BZH + lk];
     const size_t o = (size_t)i + (size_t)a.M * j + MM * k;
-    if constexpr (DET) {
-      a.out64[o] = a.accumulate ? a.out64[o] + (int64_t)v : (int64_t)v;
-    } else {
-      a.out[o] = a.accumulate ? a.out[o] + v : v;
-    }
+    a.out64[o] = a.accumulate ? a.out64[o] + v : v;
   }
   // the halo plane: voxel plane k = (bzi + 1) * BZ, owned by the brick above
   if ((int)(bzi + 1) * BZ < a.N) {
@@ -336,34 +318,25 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
       const int li = (int)(e % BX), lj = (int)(e / BX);
       const int i = (int)bxi * BX + li, j = (int)byi * BY + lj;
       if (i >= a.M || j >= a.M) continue;
-      const Acc v = lds[(li * BY + lj) * BZH + BZ];
+      const int64_t v = (int64_t)lds[(li * BY + lj) * BZH + BZ];
       const size_t o = (size_t)bzi * MM + (size_t)j * a.M + i;
-      if constexpr (DET) {
-        a.halo64[o] = a.accumulate ? a.halo64[o] + (int64_t)v : (int64_t)v;
-      } else {
-        a.halo[o] = a.accumulate ? a.halo[o] + v : v;
-      }
+      a.halo64[o] = a.accumulate ? a.halo64[o] + v : v;
     }
   }
 }
 
 // ------------------------------------------------------------------ halo
-// Adds halo plane bz to voxel plane (bz + 1) * BZ; in the deterministic mode
-// then turns the fixed-point sums into floats.
-template <bool DET>
+// Adds halo plane bz to voxel plane (bz + 1) * BZ.
 __global__ __launch_bounds__(256) void g3d_halo_kernel(G3DBrickArgs a) {
   const size_t MM = (size_t)a.M * a.M;
   const size_t nh = (size_t)(a.nbz - 1) * MM;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < nh; e += (size_t)gridDim.x * 256) {
     const size_t bz = e / MM, ij = e - bz * MM;
-    const size_t o = ij + MM * (size_t)((bz + 1) * BZ);
-    if constexpr (DET)
-      a.out64[o] += a.halo64[e];
-    else
-      a.out[o] = a.out[o] + a.halo[e];
+    a.out64[ij + MM * (size_t)((bz + 1) * BZ)] += a.halo64[e];
   }
 }
 
+// The fixed-point sums as floats.
 __global__ __launch_bounds__(256) void g3d_fixed_to_float_kernel(G3DBrickArgs a, size_t n) {
   const double inv = 1.0 / (double)g3d_fixed_scale(a.pmax);
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
@@ -386,7 +359,7 @@ G3DBrickPlan g3d_brick_plan(int W, int H, int M, int N, int nviews) {
   p.tiles_x = (uint32_t)((W + kTileSide - 1) / kTileSide);
   p.tiles_per_view = p.tiles_x * (uint32_t)((H + kTileSide - 1) / kTileSide);
   p.nbx = (uint32_t)((M + BX - 1) / BX);
-  p.nby = p.nbx;
+  p.nby = (uint32_t)((M + BY - 1) / BY);
   p.nbz = (uint32_t)((N + BZ - 1) / BZ);
   p.nbricks = p.nbx * p.nby * p.nbz;
   p.n_cap = g3d_brick_n_cap(M, N);
@@ -446,7 +419,6 @@ hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
   a.rays = L.ck + 3 * p.ck_plane;
   a.ray_plane = p.ray_plane;
   a.out = L.out;
-  a.halo = L.halo;
   a.out64 = L.out64;
   a.halo64 = L.halo64;
   a.pmax = L.pmax;
@@ -461,24 +433,17 @@ hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
     if (e != hipSuccess) return e;
     g3d_bin_kernel<<<(a.ntiles + 3) / 4, 256, lds_bin, st>>>(a);
     const unsigned gb = (p.nbricks + 7u) & ~7u;
-    if (L.deterministic)
-      g3d_brick_kernel<true><<<gb, 256, 0, st>>>(a);
-    else
-      g3d_brick_kernel<false><<<gb, 256, 0, st>>>(a);
+    g3d_brick_kernel<<<gb, 256, 0, st>>>(a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (L.nviews == 0) {  // no chunk ran: the volume is zero
-    const hipError_t e = L.deterministic ? launch_fill((float*)L.out64, 0.f, 2 * MMN, st)
-                                         : launch_fill(L.out, 0.f, MMN, st);
+    const hipError_t e = launch_fill((float*)L.out64, 0.f, 2 * MMN, st);
     if (e != hipSuccess) return e;
   }
   if (p.nbz > 1 && L.nviews > 0) {
     const unsigned g = grid_of((size_t)(p.nbz - 1) * L.M * L.M);
-    if (L.deterministic)
-      g3d_halo_kernel<true><<<g, 256, 0, st>>>(a);
-    else
-      g3d_halo_kernel<false><<<g, 256, 0, st>>>(a);
+    g3d_halo_kernel<<<g, 256, 0, st>>>(a);
   }
   if (L.nviews > 0) {  // rays too long to cross the volume, checked sample by sample
     G3DBPLaunch f;
@@ -494,12 +459,12 @@ hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
     f.N = L.N;
     f.n_min = p.n_cap;
     f.xfast = true;
-    f.out64 = L.deterministic ? L.out64 : nullptr;
+    f.out64 = L.out64;
     f.pmax = L.pmax;
     const hipError_t e = launch_g3d_bp(f, st);
     if (e != hipSuccess) return e;
   }
-  if (L.deterministic) g3d_fixed_to_float_kernel<<<grid_of(MMN), 256, 0, st>>>(a, MMN);
+  g3d_fixed_to_float_kernel<<<grid_of(MMN), 256, 0, st>>>(a, MMN);
   return hipGetLastError();
 }
 

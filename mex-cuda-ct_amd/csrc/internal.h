@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 struct mexct_ctx;
+struct mexct_opts;
 
 namespace mexct {
 
@@ -87,6 +88,7 @@ struct FPZLaunch {
   // floats apart, vol_max[item] when scaled; one launch for all of them
   int items = 1;
   uint64_t vol_stride = 0, out_stride = 0;
+  float* const* outs = nullptr;  // device array: item i's output (else out + i * out_stride)
   int vchunk_log2 = -1;  // block order: views per chunk 2^vchunk_log2 (-1: default 16)
   // the loads-only form: the same blocks, addresses and loads, the taps
   // XOR-folded instead of interpolated (par/fan/cone, DUAL f32 or QUAD 2-byte)
@@ -128,6 +130,7 @@ struct BPLaunch {
   // out_stride floats apart, sino_max[item] when scaled; one launch per chunk
   int items = 1;
   uint64_t sino_stride = 0, out_stride = 0;
+  float* const* outs = nullptr;  // device array: item i's volume (else out + i * out_stride)
   int super_ = 0;    // block order: super-tile side in 8x8 tiles (0: default 4)
   // the loads-only form: the same blocks and loads, each load XOR-folded
   // instead of interpolated (cone only)
@@ -188,14 +191,12 @@ struct G3DBrickLaunch {
   uint32_t* mask;   // plan.mask_bytes
   float* ck;        // plan.ck_bytes
   float* out;       // [M,M,N]
-  float* halo;      // plan.halo_elems floats (float mode)
-  bool deterministic;
-  int64_t* out64;   // [M,M,N] and plan.halo_elems fixed-point sums (deterministic)
-  int64_t* halo64;
-  const uint32_t* pmax;  // bits of max |sino| (deterministic)
+  int64_t* out64;   // [M,M,N] fixed-point sums
+  int64_t* halo64;  // plan.halo_elems fixed-point sums
+  const uint32_t* pmax;  // bits of max |sino|: the fixed point's scale
 };
 // Every chunk of views, the halo planes, the rays left to the atomic kernel
-// and, when deterministic, the conversion to float: out is complete after.
+// and the conversion to float: out is complete after.
 hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& a, hipStream_t st);
 // z-fastest accumulator -> [M,M,N] x-fastest volume
 hipError_t launch_g3d_unz(const float* acc, float* out, int M, int N, hipStream_t st);
@@ -267,6 +268,14 @@ int set_error(int code, const char* msg);
 // over `items` batched operands, each workspace slot rounded up as the
 // context allocates it.  min_subsets: the fewest subsets of any OSC-TV
 // iteration (sets the largest per-subset ratio-term copies).
+// mexct_fp_batch_dev / mexct_bp_batch_dev with each item's input and output
+// at its own device address (host arrays of `count` pointers).
+int fp_batch_ptrs(mexct_ctx* c, const void* const* vols, int32_t vol_dtype, int32_t count,
+                  int64_t W, int64_t H, float sad, const float* angles, int32_t nProj,
+                  int32_t geom, float* const* outs, const mexct_opts* opts);
+int bp_batch_ptrs(mexct_ctx* c, const void* const* sinos, int32_t sino_dtype, int32_t count,
+                  int64_t W, int64_t H, int64_t nProj, float sad, const float* angles,
+                  int32_t geom, float* const* outs, const mexct_opts* opts);
 size_t op_workspace_bytes(int op, int geom, int st, int64_t W, int64_t H, int64_t nProj, float sad,
                           int items, int min_subsets);
 

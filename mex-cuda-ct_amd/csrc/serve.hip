@@ -461,8 +461,8 @@ int copy_out_items(Worker& w, const std::vector<Req*>& g, int first, int count, 
 
 // Runs a same-key group of FP/BP/FDK requests as one batch, each item
 // computed exactly as alone.
-//  - resident requests: inputs gathered into one array by device copies, one
-//    batched launch, outputs scattered back by device copies;
+//  - resident requests: one batched launch that reads and writes every item
+//    at its own address;
 //  - host requests: in two halves, so that copies and kernels overlap:
 //    inputs of half 0; inputs of half 1 while half 0 computes; outputs of
 //    half 0 while half 1 computes; outputs of half 1.
@@ -473,19 +473,34 @@ int run_batch_impl(Worker& w, const std::vector<Req*>& g) {
   io_bytes(r, &in_b, &out_b);
   const float* ang = g[0]->angles.data();
   hipStream_t ks = mexct::ctx_stream(w.ctx);
-  TRY(grow(w.a, in_b * n));
-  TRY(grow(w.o, out_b * n));
   if (r.op == MEXCT_OP_FDK) TRY(grow(w.x, in_b * n));
-  if (r.resident) {
-    for (int i = 0; i < n; ++i)
-      HTRY(hipMemcpyAsync((char*)w.a.p + i * in_b, g[i]->r.input, in_b, hipMemcpyDeviceToDevice, ks));
-    TRY(launch_items(w, r, ang, 0, n));
-    for (int i = 0; i < n; ++i)
-      HTRY(hipMemcpyAsync(g[i]->r.output, (char*)w.o.p + i * out_b, out_b,
-                          hipMemcpyDeviceToDevice, ks));
+  if (r.resident) {  // every item read and written in place
+    const mexct_opts o{r.storage, r.tv_steps, 0, 0, MEXCT_G3D_BRICKS};
+    std::vector<const void*> ins((size_t)n);
+    std::vector<float*> outs((size_t)n);
+    for (int i = 0; i < n; ++i) {
+      ins[i] = g[i]->r.input;
+      outs[i] = g[i]->r.output;
+    }
+    if (r.op == MEXCT_OP_FP) {
+      TRY(mexct::fp_batch_ptrs(w.ctx, ins.data(), MEXCT_F32, n, r.W, r.H, r.sad, ang,
+                               (int32_t)r.nProj, r.geom, outs.data(), &o));
+    } else {
+      if (r.op == MEXCT_OP_FDK)
+        for (int i = 0; i < n; ++i) {
+          float* x = (float*)((char*)w.x.p + i * in_b);
+          TRY(mexct_prefilter_dev(w.ctx, (const float*)ins[i], r.W, r.H, r.nProj, r.geom, r.sad,
+                                  ang, r.window, r.cutoff, MEXCT_NO_PARKER, x));
+          ins[i] = x;
+        }
+      TRY(mexct::bp_batch_ptrs(w.ctx, ins.data(), MEXCT_F32, n, r.W, r.H, r.nProj, r.sad, ang,
+                               r.geom, outs.data(), &o));
+    }
     HTRY(hipStreamSynchronize(ks));
     return MEXCT_OK;
   }
+  TRY(grow(w.a, in_b * n));
+  TRY(grow(w.o, out_b * n));
   const int h = (n + 1) / 2;
   const int first[2] = {0, h}, count[2] = {h, n - h};
   for (int p = 0; p < 2; ++p) {

@@ -235,19 +235,29 @@ def test_resident_requests_equal_host_requests(gpu):
             assert np.array_equal(got, alone[i]), f"{cases[i][0]} case {i} differs when resident"
 
 
-def test_resident_same_shape_requests_run_batched(gpu):
+@pytest.mark.parametrize("kind", ["fp", "bp", "fdk"])
+def test_resident_same_shape_requests_run_batched(gpu, kind):
     W, H, n, sad = 40, 20, 16, 200.0
     ang = _angles(n)
     g = {"type": "cone3d", "SAD": sad}
-    sinos = [_sino(W, H, n, 700 + k) for k in range(6)]
-    alone = [mexct.CUDAmex_BP(mexct.preFilterSinogram(P, g, ang, "hann", 0.9), sad, ang, 2)
-             for P in sinos]
+    if kind == "fp":
+        ins = [_vol(W, H, 700 + k) for k in range(6)]
+        alone = [mexct.CUDAmex_FP(v, sad, ang, n, 2) for v in ins]
+    else:
+        ins = [_sino(W, H, n, 700 + k) for k in range(6)]
+        alone = [mexct.CUDAmex_BP(mexct.preFilterSinogram(P, g, ang, "hann", 0.9)
+                                  if kind == "fdk" else P, sad, ang, 2) for P in ins]
     with Server(hbm_budget=1 << 30, slots=1, max_group=3) as s:
-        ts = [s.submit_fdk(_dev(P), g, ang, "hann", 0.9) for P in sinos]
+        if kind == "fp":
+            ts = [s.submit_fp(_dev(v), sad, ang, n, 2) for v in ins]
+        elif kind == "bp":
+            ts = [s.submit_bp(_dev(P), sad, ang, 2) for P in ins]
+        else:
+            ts = [s.submit_fdk(_dev(P), g, ang, "hann", 0.9) for P in ins]
         outs = [s.result(t).cpu().numpy().T for t in ts]
         st = s.stats()
     for k in range(6):
-        assert np.array_equal(outs[k], alone[k]), f"fdk {k}"
+        assert np.array_equal(outs[k], alone[k]), f"{kind} {k}"
     assert st["batches"] >= 1 and st["batched"] >= 3
 
 
