@@ -198,9 +198,83 @@ __device__ __forceinline__ void lds_add(unsigned long long* lds, uint32_t idx, f
   atomicAdd(&lds[idx + 1], g3d_fixed(hi * scale));
 }
 
+constexpr uint32_t kQueue = 128;  // work items per wave (a power of two above 127)
+
+// Work items [head, head + cnt) of a wave's queue, one per lane: the samples
+// [k0, k1] of the item's ray, resumed from the checkpoint before k0, added to
+// the brick's LDS sums when their cell lies in the brick.
+__device__ __forceinline__ void brick_items(const G3DBrickArgs& a, unsigned long long* lds,
+                                            const uint2* queue, uint32_t head, uint32_t cnt,
+                                            uint32_t lane, float lox, float loy, float loz,
+                                            float hx, float hy, float hz, uint32_t base,
+                                            float scale) {
+  // the queue's entries were written by other lanes of this wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < cnt) {
+    const uint2 it = queue[(head + lane) & (kQueue - 1u)];
+    const uint32_t t = it.x >> 6, l = it.x & 63u;
+    const size_t ri = it.x;
+    const int k0 = (int)(it.y & 0xffffu), k1 = (int)(it.y >> 16);
+    R3 cur, inc;
+    inc.x = a.rays[ri];
+    inc.y = a.rays[a.ray_plane + ri];
+    inc.z = a.rays[2 * a.ray_plane + ri];
+    // resume the running sum at the checkpoint before k0
+    const int c = k0 / kCkSamples;
+    const size_t ci = ((size_t)t * a.S + (size_t)c) * 64u + l;
+    cur.x = a.ck[ci];
+    cur.y = a.ck[a.ck_plane + ci];
+    cur.z = a.ck[2 * a.ck_plane + ci];
+    for (int k = c * kCkSamples; k < k0; ++k) {
+      cur.x = cur.x + inc.x;
+      cur.y = cur.y + inc.y;
+      cur.z = cur.z + inc.z;
+    }
+    const TileRay r = tile_ray(a, t, l);
+    const float pval = a.sino[(size_t)r.view * a.W * a.H + (size_t)r.iy * a.W + r.ix];
+    const float w0 = pval * 0.5f;  // P * stepsize
+    // consecutive samples in one cell are summed in registers first
+    uint32_t pend = ~0u;
+    float wlo = 0.f, whi = 0.f;
+    for (int k = k0; k <= k1; ++k) {
+      const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
+      const bool in = fx >= lox && fx < hx && fy >= loy && fy < hy && fz >= loz && fz < hz;
+      // CUDAmex_general3D_BP.cu:158-170: f - floor(f) + 0.5 per axis (exact,
+      // as v_fract), kL gets 1 - iZ and kU = ceil iZ
+      const float rz = __builtin_amdgcn_fractf(fz);
+      const float iXX = __builtin_amdgcn_fractf(fx) + 0.5f;
+      const float iYY = __builtin_amdgcn_fractf(fy) + 0.5f;
+      const float iZ = rz + 0.5f;
+      const float iZZ = 1 - iZ;
+      const float wxy = w0 * iXX * iYY;
+      float lo = wxy * iZZ;
+      float hi = wxy * iZ;
+      if (rz == 0.f) {  // kU == kL: both corners are the same voxel
+        lo = lo + hi;
+        hi = 0.f;
+      }
+      const uint32_t idx =
+          (uint32_t)mad_i24(cvt_flr(fx), BY * BZH, mad_i24(cvt_flr(fy), BZH, cvt_flr(fz))) - base;
+      const bool same = idx == pend;
+      if (in && !same && pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
+      wlo = in ? (same ? wlo + lo : lo) : wlo;
+      whi = in ? (same ? whi + hi : hi) : whi;
+      pend = in ? idx : pend;
+      cur.x = cur.x + inc.x;
+      cur.y = cur.y + inc.y;
+      cur.z = cur.z + inc.z;
+    }
+    if (pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
   typedef unsigned long long Acc;
   __shared__ Acc lds[BX * BY * BZH];
+  __shared__ uint2 wq[4][kQueue];
   const uint32_t brick = xcd_remap(blockIdx.x, gridDim.x);
   if (brick >= a.nbricks) return;
   const uint32_t bxi = brick % a.nbx, byi = (brick / a.nbx) % a.nby, bzi = brick / (a.nbx * a.nby);
@@ -217,6 +291,11 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
   // LDS index of cell (i, j, k): (i - lox) * BY * BZH + (j - loy) * BZH + (k - loz)
   const uint32_t base = (bxi * BX * BY + byi * BY) * BZH + bzi * BZ;
   const uint32_t* row = a.mask + (size_t)brick * a.tw;
+  // The rays of a marked tile that cross the brick (about half of them) are
+  // queued, per wave, as (ray, k0, k1) work items in LDS, and the sample
+  // loop runs on full waves of 64 items: ray = tile * 64 + lane, k0 | k1 << 16.
+  uint2* queue = wq[wave];
+  uint32_t head = 0, tail = 0;  // wave-uniform; at most 127 items wait
   uint32_t ord = 0;  // marked tiles seen; wave w takes every 4th from w
   for (uint32_t q = 0; q < a.tw; ++q) {
     uint32_t w = row[q];
@@ -224,82 +303,60 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
       const uint32_t t = q * 32u + (uint32_t)__builtin_ctz(w);
       w &= w - 1u;
       if ((ord++ & 3u) != wave) continue;
+      bool active = false;
+      int k0 = 0, k1 = -1;
       const TileRay r = tile_ray(a, t, lane);
-      if (!r.ok) continue;
-      // the ray as the bin pass set it up: increments, sample count and
-      // first position (checkpoint 0)
       const size_t ri = (size_t)t * 64u + lane;
-      const int n = __float_as_int(a.rays[3 * a.ray_plane + ri]);
-      if (n <= 0 || n > a.n_cap) continue;
-      R3 cur, inc;
-      inc.x = a.rays[ri];
-      inc.y = a.rays[a.ray_plane + ri];
-      inc.z = a.rays[2 * a.ray_plane + ri];
-      const size_t c0 = (size_t)t * a.S * 64u + lane;
-      cur.x = a.ck[c0];
-      cur.y = a.ck[a.ck_plane + c0];
-      cur.z = a.ck[2 * a.ck_plane + c0];
-      // samples the brick can own: the exact line X0 + k s inside the brick
-      // on every axis, widened by the running sum's drift (at most a half
-      // ulp of the largest coordinate per step)
-      const float X0 = cur.x - a.bx, Y0 = cur.y - a.by, Z0 = cur.z - a.bz;
-      const float fn = (float)n;
-      const float big = fmaxf(fmaxf(fabsf(cur.x) + fabsf(inc.x) * fn, fabsf(cur.y) + fabsf(inc.y) * fn),
-                              fabsf(cur.z) + fabsf(inc.z) * fn) + 1.f;
-      const float d = fn * big * 1.2e-7f + 1e-3f;
-      float klo = 0.f, khi = fn - 1.f;
-      axis_range(X0, inc.x, __builtin_amdgcn_rcpf(inc.x), lox, lox + (float)BX, d, klo, khi);
-      axis_range(Y0, inc.y, __builtin_amdgcn_rcpf(inc.y), loy, loy + (float)BY, d, klo, khi);
-      axis_range(Z0, inc.z, __builtin_amdgcn_rcpf(inc.z), loz, loz + (float)BZ, d, klo, khi);
-      if (!(klo <= khi)) continue;
-      const int k0 = max(0, (int)floorf(klo) - 1);
-      const int k1 = min(n - 1, (int)ceilf(khi) + 1);
-      // resume the running sum at the checkpoint before k0
-      const int c = k0 / kCkSamples;
-      const size_t ci = ((size_t)t * a.S + (size_t)c) * 64u + lane;
-      cur.x = a.ck[ci];
-      cur.y = a.ck[a.ck_plane + ci];
-      cur.z = a.ck[2 * a.ck_plane + ci];
-      for (int k = c * kCkSamples; k < k0; ++k) {
-        cur.x = cur.x + inc.x;
-        cur.y = cur.y + inc.y;
-        cur.z = cur.z + inc.z;
-      }
-      const float pval = a.sino[(size_t)r.view * a.W * a.H + (size_t)r.iy * a.W + r.ix];
-      const float w0 = pval * 0.5f;  // P * stepsize
-      // consecutive samples in one cell are summed in registers first
-      uint32_t pend = ~0u;
-      float wlo = 0.f, whi = 0.f;
-      for (int k = k0; k <= k1; ++k) {
-        const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
-        const bool in = fx >= lox && fx < hx && fy >= loy && fy < hy && fz >= loz && fz < hz;
-        // CUDAmex_general3D_BP.cu:158-170: f - floor(f) + 0.5 per axis
-        // (exact, as v_fract), kL gets 1 - iZ and kU = ceil iZ
-        const float rz = __builtin_amdgcn_fractf(fz);
-        const float iXX = __builtin_amdgcn_fractf(fx) + 0.5f;
-        const float iYY = __builtin_amdgcn_fractf(fy) + 0.5f;
-        const float iZ = rz + 0.5f;
-        const float iZZ = 1 - iZ;
-        const float wxy = w0 * iXX * iYY;
-        float lo = wxy * iZZ;
-        float hi = wxy * iZ;
-        if (rz == 0.f) {  // kU == kL: both corners are the same voxel
-          lo = lo + hi;
-          hi = 0.f;
+      if (r.ok) {
+        // the ray as the bin pass set it up: increments, sample count and
+        // first position (checkpoint 0)
+        const int n = __float_as_int(a.rays[3 * a.ray_plane + ri]);
+        if (n > 0 && n <= a.n_cap) {
+          R3 cur, inc;
+          inc.x = a.rays[ri];
+          inc.y = a.rays[a.ray_plane + ri];
+          inc.z = a.rays[2 * a.ray_plane + ri];
+          const size_t c0 = (size_t)t * a.S * 64u + lane;
+          cur.x = a.ck[c0];
+          cur.y = a.ck[a.ck_plane + c0];
+          cur.z = a.ck[2 * a.ck_plane + c0];
+          // samples the brick can own: the exact line X0 + k s inside the
+          // brick on every axis, widened by the running sum's drift (at most
+          // a half ulp of the largest coordinate per step)
+          const float X0 = cur.x - a.bx, Y0 = cur.y - a.by, Z0 = cur.z - a.bz;
+          const float fn = (float)n;
+          const float big = fmaxf(fmaxf(fabsf(cur.x) + fabsf(inc.x) * fn,
+                                        fabsf(cur.y) + fabsf(inc.y) * fn),
+                                  fabsf(cur.z) + fabsf(inc.z) * fn) + 1.f;
+          const float d = fn * big * 1.2e-7f + 1e-3f;
+          float klo = 0.f, khi = fn - 1.f;
+          axis_range(X0, inc.x, __builtin_amdgcn_rcpf(inc.x), lox, lox + (float)BX, d, klo, khi);
+          axis_range(Y0, inc.y, __builtin_amdgcn_rcpf(inc.y), loy, loy + (float)BY, d, klo, khi);
+          axis_range(Z0, inc.z, __builtin_amdgcn_rcpf(inc.z), loz, loz + (float)BZ, d, klo, khi);
+          if (klo <= khi) {
+            active = true;
+            k0 = max(0, (int)floorf(klo) - 1);
+            k1 = min(n - 1, (int)ceilf(khi) + 1);
+          }
         }
-        const uint32_t idx =
-            (uint32_t)mad_i24(cvt_flr(fx), BY * BZH, mad_i24(cvt_flr(fy), BZH, cvt_flr(fz))) - base;
-        const bool same = idx == pend;
-        if (in && !same && pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
-        wlo = in ? (same ? wlo + lo : lo) : wlo;
-        whi = in ? (same ? whi + hi : hi) : whi;
-        pend = in ? idx : pend;
-        cur.x = cur.x + inc.x;
-        cur.y = cur.y + inc.y;
-        cur.z = cur.z + inc.z;
       }
-      if (pend != ~0u) lds_add(lds, pend, wlo, whi, scale);
+      const uint64_t m = __builtin_amdgcn_ballot_w64(active);
+      if (active) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        queue[(tail + rank) & (kQueue - 1u)] = make_uint2((uint32_t)ri, (uint32_t)k0 | ((uint32_t)k1 << 16));
+      }
+      tail += (uint32_t)__builtin_popcountll(m);
+      if (tail - head >= 64u) {
+        brick_items(a, lds, queue, head, 64u, lane, lox, loy, loz, hx, hy, hz, base, scale);
+        head += 64u;
+      }
     }
+  }
+  while (tail != head) {
+    const uint32_t cnt = min(64u, tail - head);
+    brick_items(a, lds, queue, head, cnt, lane, lox, loy, loz, hx, hy, hz, base, scale);
+    head += cnt;
   }
   __syncthreads();
   // the brick's voxels, x fastest across the threads
@@ -386,7 +443,8 @@ G3DBrickPlan g3d_brick_plan(int W, int H, int M, int N, int nviews) {
   p.mask_bytes = (size_t)p.nbricks * p.tw * sizeof(uint32_t);
   p.halo_elems = (size_t)p.nbz * M * M;
   // the bin kernel's per-wave brick bits fit the LDS
-  p.ok = (size_t)4 * ((p.nbricks + 31) / 32) * 4 <= (size_t)64 << 10;
+  p.ok = (size_t)4 * ((p.nbricks + 31) / 32) * 4 <= (size_t)64 << 10 &&
+         p.n_cap < 65536;  // the brick pass packs sample indices in 16 bits
   return p;
 }
 
