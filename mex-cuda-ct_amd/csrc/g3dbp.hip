@@ -15,11 +15,12 @@
 //          keeps the position every kCk samples (checkpoints) and its set-up,
 //          and marks, for its 8x8 tile of detector pixels, each brick that
 //          owns one of its samples (one bit per (brick, tile));
-//   brick  for each tile marked for it, each ray of the tile bounds the
-//          samples the brick can own from its exact line (widened by the
-//          running sum's drift), resumes the running sum at the checkpoint
-//          before them and adds the samples it owns, with the reference's
-//          two-corner weights.  The corner kL + 1 of the brick's top plane
+//   brick  for each tile marked for it (each of the four waves takes a
+//          quarter of the brick's row of tile bits), each ray of the tile
+//          bounds the samples the brick can own from its exact line (widened
+//          by the running sum's drift), resumes the running sum at the
+//          checkpoint before them and adds the samples it owns, with the
+//          reference's two-corner weights.  The corner kL + 1 of the brick's top plane
 //          belongs to the brick above and goes to a halo plane;
 //   halo   after the last chunk each halo plane is added to its plane.
 //
@@ -296,13 +297,21 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
   // loop runs on full waves of 64 items: ray = tile * 64 + lane, k0 | k1 << 16.
   uint2* queue = wq[wave];
   uint32_t head = 0, tail = 0;  // wave-uniform; at most 127 items wait
-  uint32_t ord = 0;  // marked tiles seen; wave w takes every 4th from w
-  for (uint32_t q = 0; q < a.tw; ++q) {
-    uint32_t w = row[q];
+  // wave w scans the w-th quarter of the brick's row of tile bits (a run of
+  // views), 64 words per vector load; the words with bits set are walked
+  // one by one
+  const uint32_t qw = (a.tw + 3u) / 4u, qb = wave * qw, qe = min(a.tw, qb + qw);
+  for (uint32_t q0 = qb; q0 < qe; q0 += 64u) {
+   const uint32_t mv = q0 + lane < qe ? row[q0 + lane] : 0u;
+   uint64_t nzw = __builtin_amdgcn_ballot_w64(mv != 0u);
+   while (nzw) {
+    const uint32_t l = (uint32_t)__builtin_ctzll(nzw);
+    nzw &= nzw - 1ull;
+    uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mv, (int)l);
+    const uint32_t q = q0 + l;
     while (w) {
       const uint32_t t = q * 32u + (uint32_t)__builtin_ctz(w);
       w &= w - 1u;
-      if ((ord++ & 3u) != wave) continue;
       bool active = false;
       int k0 = 0, k1 = -1;
       const TileRay r = tile_ray(a, t, lane);
@@ -352,6 +361,7 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
         head += 64u;
       }
     }
+   }
   }
   while (tail != head) {
     const uint32_t cnt = min(64u, tail - head);
