@@ -447,7 +447,9 @@ def same_geometry_burst(Server, phantoms, n=16, slots=2, max_group=8):
     against servers that run them one by one (max_group 1), with host arrays
     (copied in and out) and with device-resident arrays (torch tensors, no
     host copies).  Each server first runs a burst untimed, which sizes its
-    buffers and tunes the shapes; the second burst is timed."""
+    buffers and tunes the shapes; three more bursts are timed and the best
+    one is reported beside all three (host bursts move by +-30% from burst
+    to burst)."""
     import torch
 
     vol = phantoms.par_phantom_128(seed=1)
@@ -456,8 +458,8 @@ def same_geometry_burst(Server, phantoms, n=16, slots=2, max_group=8):
     dvol = torch.from_numpy(np.ascontiguousarray(vol.T)).cuda()
     dsino = torch.from_numpy(np.ascontiguousarray(sino.T)).cuda()
     out = {"workload": f"bursts of {n} requests, parallel 128^3, 180 views, f32, queued at once; "
-                       "the second burst on a warm server is timed; host arrays, or resident "
-                       "device arrays",
+                       "three bursts on a warm server are timed, the best reported; host "
+                       "arrays, or resident device arrays",
            "slots": slots}
     for kind in ("fp", "bp"):
         for resident in (False, True):
@@ -469,13 +471,17 @@ def same_geometry_burst(Server, phantoms, n=16, slots=2, max_group=8):
                     for t in [sub() for _ in range(n)]:
                         s.result(t)
                     b0 = s.stats()["batched"]
-                    t0 = time.perf_counter()
-                    for t in [sub() for _ in range(n)]:
-                        s.result(t)
-                    wall = time.perf_counter() - t0
+                    rates = []
+                    for _ in range(3):
+                        t0 = time.perf_counter()
+                        for t in [sub() for _ in range(n)]:
+                            s.result(t)
+                        rates.append(n / (time.perf_counter() - t0))
                     st = s.stats()
                 key = f"{kind}_{'resident_' if resident else ''}max_group_{mg}"
-                out[key] = {"throughput_req_s": round(n / wall, 1), "batched": st["batched"] - b0}
+                out[key] = {"throughput_req_s": round(max(rates), 1),
+                            "bursts_req_s": [round(r, 1) for r in rates],
+                            "batched": st["batched"] - b0}
     return out
 
 
