@@ -12,9 +12,10 @@
 // and writes each of its voxels once with a plain store.  Per chunk of views:
 //
 //   bin    one thread per ray marches the reference's running sum once,
-//          keeps the position every kCk samples (checkpoints) and its set-up,
-//          and marks, for its 8x8 tile of detector pixels, each brick that
-//          owns one of its samples (one bit per (brick, tile));
+//          keeps the position every kCkSamples samples (checkpoints) and its
+//          set-up, and marks, for its 8x8 tile of detector pixels, each brick
+//          its samples can reach (one bit per (brick, tile)), walking its
+//          exact line 16 samples at a time;
 //   brick  for each tile marked for it (each of the four waves takes a
 //          quarter of the brick's row of tile bits), each ray of the tile
 //          bounds the samples the brick can own from its exact line (widened
@@ -100,8 +101,21 @@ __device__ __forceinline__ int ray_setup(const G3DBrickArgs& a, const TileRay& r
   return clip_ray(st, en, a.bx, a.by, a.bz, 0.5f, cur, inc);
 }
 
+// Bound on how far the running sum cur + k inc (float additions, k < n)
+// strays from the exact line: at most half an ulp of the largest coordinate
+// per step (doubled here), plus room for the rounding of the coordinates'
+// offsets from the box corner and of the line's own evaluation.
+__device__ __forceinline__ float ray_drift(const R3& cur, const R3& inc, int n) {
+  const float fn = (float)n;
+  const float big = fmaxf(fmaxf(fabsf(cur.x) + fabsf(inc.x) * fn, fabsf(cur.y) + fabsf(inc.y) * fn),
+                          fabsf(cur.z) + fabsf(inc.z) * fn) + 1.f;
+  return fn * big * 1.2e-7f + 1e-3f;
+}
+
 // ------------------------------------------------------------------ bin
 // One wave per tile.  LDS: one bit per brick per wave.
+constexpr int kBinRun = 16;  // samples per box of the brick walk
+static_assert(kCkSamples % kBinRun == 0, "checkpoints fall on the first sample of a run");
 __global__ __launch_bounds__(256) void g3d_bin_kernel(G3DBrickArgs a) {
   extern __shared__ uint32_t smask[];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -125,29 +139,59 @@ __global__ __launch_bounds__(256) void g3d_bin_kernel(G3DBrickArgs a) {
       a.rays[3 * a.ray_plane + ri] = __int_as_float(n);
       if (n > 0 && n <= a.n_cap) {
         const size_t cbase = (size_t)t * a.S * 64u + lane;
-        uint32_t last = ~0u;
-        for (int k = 0; k < n; ++k) {
-          if ((k & (kCkSamples - 1)) == 0) {
-            const size_t ci = cbase + (size_t)(k / kCkSamples) * 64u;
+        // The bricks come from the exact line X0 + k s, a run of kBinRun
+        // samples at a time: every brick that the run's box (the line between
+        // the run's first and last sample, widened by the running sum's
+        // drift) covers within the volume.  The running sum's samples lie in
+        // that box, so every brick owning one of them is marked; a brick
+        // marked without one costs the brick pass a range test and nothing
+        // more.  The reference's index test (CUDAmex_general3D_BP.cu:148-156),
+        // floor(f) >= 0 and ceil(f) < size on each axis, is 0 <= f <= size - 1
+        // on the exact coordinates, so the box is clipped to that range.
+        const float X0 = cur.x - a.bx, Y0 = cur.y - a.by, Z0 = cur.z - a.bz;
+        const float d = ray_drift(cur, inc, n);
+        int px0 = -1, px1 = -1, py0 = -1, py1 = -1, pz0 = -1, pz1 = -1;  // the previous run's bricks
+        for (int k0 = 0; k0 < n; k0 += kBinRun) {
+          if ((k0 & (kCkSamples - 1)) == 0) {
+            const size_t ci = cbase + (size_t)(k0 / kCkSamples) * 64u;
             a.ck[ci] = cur.x;
             a.ck[a.ck_plane + ci] = cur.y;
             a.ck[2 * a.ck_plane + ci] = cur.z;
           }
-          // the reference's index test (CUDAmex_general3D_BP.cu:148-156),
-          // floor(f) >= 0 and ceil(f) < size on each axis, is
-          // 0 <= f <= size - 1 on the exact coordinates
-          const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
-          if (fx >= 0.f && fx <= fm && fy >= 0.f && fy <= fm && fz >= 0.f && fz <= fn1) {
-            const uint32_t b = (uint32_t)cvt_flr(fx) / BX +
-                               a.nbx * ((uint32_t)cvt_flr(fy) / BY + a.nby * ((uint32_t)cvt_flr(fz) / BZ));
-            if (b != last) {
-              atomicOr(&m[b >> 5], 1u << (b & 31u));
-              last = b;
+          const float ka = (float)k0, kb = (float)min(k0 + kBinRun - 1, n - 1);
+          const float xa = __builtin_fmaf(ka, inc.x, X0), xb = __builtin_fmaf(kb, inc.x, X0);
+          const float ya = __builtin_fmaf(ka, inc.y, Y0), yb = __builtin_fmaf(kb, inc.y, Y0);
+          const float za = __builtin_fmaf(ka, inc.z, Z0), zb = __builtin_fmaf(kb, inc.z, Z0);
+          const float lx = fminf(xa, xb) - d, hx = fmaxf(xa, xb) + d;
+          const float ly = fminf(ya, yb) - d, hy = fmaxf(ya, yb) + d;
+          const float lz = fminf(za, zb) - d, hz = fmaxf(za, zb) + d;
+          // written so that a NaN skips the run
+          if (hx >= 0.f && lx <= fm && hy >= 0.f && ly <= fm && hz >= 0.f && lz <= fn1) {
+            const int bx0 = cvt_flr(fmaxf(lx, 0.f)) / BX, bx1 = cvt_flr(fminf(hx, fm)) / BX;
+            const int by0 = cvt_flr(fmaxf(ly, 0.f)) / BY, by1 = cvt_flr(fminf(hy, fm)) / BY;
+            const int bz0 = cvt_flr(fmaxf(lz, 0.f)) / BZ, bz1 = cvt_flr(fminf(hz, fn1)) / BZ;
+            // consecutive runs mostly cover the same bricks
+            if (bx0 != px0 || bx1 != px1 || by0 != py0 || by1 != py1 || bz0 != pz0 || bz1 != pz1) {
+              px0 = bx0;
+              px1 = bx1;
+              py0 = by0;
+              py1 = by1;
+              pz0 = bz0;
+              pz1 = bz1;
+              for (int bz = bz0; bz <= bz1; ++bz)
+                for (int by = by0; by <= by1; ++by)
+                  for (int bx = bx0; bx <= bx1; ++bx) {
+                    const uint32_t b = (uint32_t)bx + a.nbx * ((uint32_t)by + a.nby * (uint32_t)bz);
+                    atomicOr(&m[b >> 5], 1u << (b & 31u));
+                  }
             }
           }
-          cur.x = cur.x + inc.x;
-          cur.y = cur.y + inc.y;
-          cur.z = cur.z + inc.z;
+#pragma unroll
+          for (int u = 0; u < kBinRun; ++u) {  // the running sum over the run
+            cur.x = cur.x + inc.x;
+            cur.y = cur.y + inc.y;
+            cur.z = cur.z + inc.z;
+          }
         }
       }
     }
@@ -259,6 +303,61 @@ __device__ __forceinline__ void brick_items(const G3DBrickArgs& a, unsigned long
   __builtin_amdgcn_wave_barrier();
 }
 
+// The marked tiles of one wave's run of a brick's tile-bit row [q0, qe), in
+// order: 64 words per vector load, the nonzero words and their bits walked
+// one by one.  All of it is wave-uniform but the lane's word mv.
+struct TileScan {
+  uint32_t q0, qe;  // q0: the word at lane 0 of the current load (starts 64 before)
+  uint32_t mv;      // this lane's word of the current load
+  uint32_t w, q;    // bits left of word q
+  uint64_t nzw;     // lanes whose word has bits left to walk
+  __device__ __forceinline__ bool next(const uint32_t* row, uint32_t lane, uint32_t& t) {
+    for (;;) {
+      if (w) {
+        t = q * 32u + (uint32_t)__builtin_ctz(w);
+        w &= w - 1u;
+        return true;
+      }
+      if (nzw) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(nzw);
+        nzw &= nzw - 1ull;
+        w = (uint32_t)__builtin_amdgcn_readlane((int)mv, (int)l);
+        q = q0 + l;
+        continue;
+      }
+      q0 += 64u;
+      if (q0 >= qe) return false;
+      mv = q0 + lane < qe ? row[q0 + lane] : 0u;
+      nzw = __builtin_amdgcn_ballot_w64(mv != 0u);
+    }
+  }
+};
+
+// A lane's ray of tile t as the bin pass set it up: increments, sample count
+// and first position (checkpoint 0).  Every entry exists for every (tile,
+// lane) of the chunk, so the loads need no guard; the values are used only
+// where ok and 0 < n <= n_cap, the rays whose set-up and checkpoints the bin
+// pass wrote.
+struct TileSetup {
+  R3 cur, inc;
+  int n;
+  bool ok;
+};
+__device__ __forceinline__ TileSetup tile_setup(const G3DBrickArgs& a, uint32_t t, uint32_t lane) {
+  TileSetup d;
+  d.ok = tile_ray(a, t, lane).ok;
+  const size_t ri = (size_t)t * 64u + lane;
+  d.n = __float_as_int(a.rays[3 * a.ray_plane + ri]);
+  d.inc.x = a.rays[ri];
+  d.inc.y = a.rays[a.ray_plane + ri];
+  d.inc.z = a.rays[2 * a.ray_plane + ri];
+  const size_t c0 = (size_t)t * a.S * 64u + lane;
+  d.cur.x = a.ck[c0];
+  d.cur.y = a.ck[a.ck_plane + c0];
+  d.cur.z = a.ck[2 * a.ck_plane + c0];
+  return d;
+}
+
 __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
   typedef unsigned long long Acc;
   __shared__ Acc lds[BX * BY * BZH];
@@ -286,69 +385,48 @@ __global__ __launch_bounds__(256) void g3d_brick_kernel(G3DBrickArgs a) {
   uint32_t head = 0, tail = 0;  // wave-uniform; at most 127 items wait
   // wave w scans the w-th quarter of the brick's row of tile bits (a run of
   // views), 64 words per vector load; the words with bits set are walked
-  // one by one
+  // one by one.  The set-up of the next marked tile is loaded while the
+  // current one is ranged and queued.
   const uint32_t qw = (a.tw + 3u) / 4u, qb = wave * qw, qe = min(a.tw, qb + qw);
-  for (uint32_t q0 = qb; q0 < qe; q0 += 64u) {
-   const uint32_t mv = q0 + lane < qe ? row[q0 + lane] : 0u;
-   uint64_t nzw = __builtin_amdgcn_ballot_w64(mv != 0u);
-   while (nzw) {
-    const uint32_t l = (uint32_t)__builtin_ctzll(nzw);
-    nzw &= nzw - 1ull;
-    uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)mv, (int)l);
-    const uint32_t q = q0 + l;
-    while (w) {
-      const uint32_t t = q * 32u + (uint32_t)__builtin_ctz(w);
-      w &= w - 1u;
-      bool active = false;
-      int k0 = 0, k1 = -1;
-      const TileRay r = tile_ray(a, t, lane);
-      const size_t ri = (size_t)t * 64u + lane;
-      if (r.ok) {
-        // the ray as the bin pass set it up: increments, sample count and
-        // first position (checkpoint 0)
-        const int n = __float_as_int(a.rays[3 * a.ray_plane + ri]);
-        if (n > 0 && n <= a.n_cap) {
-          R3 cur, inc;
-          inc.x = a.rays[ri];
-          inc.y = a.rays[a.ray_plane + ri];
-          inc.z = a.rays[2 * a.ray_plane + ri];
-          const size_t c0 = (size_t)t * a.S * 64u + lane;
-          cur.x = a.ck[c0];
-          cur.y = a.ck[a.ck_plane + c0];
-          cur.z = a.ck[2 * a.ck_plane + c0];
-          // samples the brick can own: the exact line X0 + k s inside the
-          // brick on every axis, widened by the running sum's drift (at most
-          // a half ulp of the largest coordinate per step)
-          const float X0 = cur.x - a.bx, Y0 = cur.y - a.by, Z0 = cur.z - a.bz;
-          const float fn = (float)n;
-          const float big = fmaxf(fmaxf(fabsf(cur.x) + fabsf(inc.x) * fn,
-                                        fabsf(cur.y) + fabsf(inc.y) * fn),
-                                  fabsf(cur.z) + fabsf(inc.z) * fn) + 1.f;
-          const float d = fn * big * 1.2e-7f + 1e-3f;
-          float klo = 0.f, khi = fn - 1.f;
-          axis_range(X0, inc.x, __builtin_amdgcn_rcpf(inc.x), lox, lox + (float)BX, d, klo, khi);
-          axis_range(Y0, inc.y, __builtin_amdgcn_rcpf(inc.y), loy, loy + (float)BY, d, klo, khi);
-          axis_range(Z0, inc.z, __builtin_amdgcn_rcpf(inc.z), loz, loz + (float)BZ, d, klo, khi);
-          if (klo <= khi) {
-            active = true;
-            k0 = max(0, (int)floorf(klo) - 1);
-            k1 = min(n - 1, (int)ceilf(khi) + 1);
-          }
-        }
-      }
-      const uint64_t m = __builtin_amdgcn_ballot_w64(active);
-      if (active) {
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        queue[(tail + rank) & (kQueue - 1u)] = make_uint2((uint32_t)ri, (uint32_t)k0 | ((uint32_t)k1 << 16));
-      }
-      tail += (uint32_t)__builtin_popcountll(m);
-      if (tail - head >= 64u) {
-        brick_items(a, lds, queue, head, 64u, lane, lox, loy, loz, hx, hy, hz, base, scale);
-        head += 64u;
+  TileScan sc{qb - 64u, qe, 0u, 0u, 0u, 0ull};
+  uint32_t t;
+  bool have = sc.next(row, lane, t);
+  TileSetup cs;
+  if (have) cs = tile_setup(a, t, lane);
+  while (have) {
+    const TileSetup d = cs;
+    const uint32_t tc = t;
+    have = sc.next(row, lane, t);
+    if (have) cs = tile_setup(a, t, lane);  // in flight while tile tc is ranged
+    bool active = false;
+    int k0 = 0, k1 = -1;
+    if (d.ok && d.n > 0 && d.n <= a.n_cap) {
+      // samples the brick can own: the exact line X0 + k s inside the brick
+      // on every axis, widened by the running sum's drift
+      const float X0 = d.cur.x - a.bx, Y0 = d.cur.y - a.by, Z0 = d.cur.z - a.bz;
+      const float dr = ray_drift(d.cur, d.inc, d.n);
+      float klo = 0.f, khi = (float)(d.n - 1);
+      axis_range(X0, d.inc.x, __builtin_amdgcn_rcpf(d.inc.x), lox, lox + (float)BX, dr, klo, khi);
+      axis_range(Y0, d.inc.y, __builtin_amdgcn_rcpf(d.inc.y), loy, loy + (float)BY, dr, klo, khi);
+      axis_range(Z0, d.inc.z, __builtin_amdgcn_rcpf(d.inc.z), loz, loz + (float)BZ, dr, klo, khi);
+      if (klo <= khi) {
+        active = true;
+        k0 = max(0, (int)floorf(klo) - 1);
+        k1 = min(d.n - 1, (int)ceilf(khi) + 1);
       }
     }
-   }
+    const uint64_t m = __builtin_amdgcn_ballot_w64(active);
+    if (active) {
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      queue[(tail + rank) & (kQueue - 1u)] =
+          make_uint2(tc * 64u + lane, (uint32_t)k0 | ((uint32_t)k1 << 16));
+    }
+    tail += (uint32_t)__builtin_popcountll(m);
+    if (tail - head >= 64u) {
+      brick_items(a, lds, queue, head, 64u, lane, lox, loy, loz, hx, hy, hz, base, scale);
+      head += 64u;
+    }
   }
   while (tail != head) {
     const uint32_t cnt = min(64u, tail - head);
