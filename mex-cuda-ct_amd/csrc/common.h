@@ -182,8 +182,12 @@ __device__ __forceinline__ float g3d_fixed_scale(const uint32_t* pmax) {
   (void)frexpf(__uint_as_float(*pmax), &e);
   return ldexpf(1.f, 27 - e);
 }
-__device__ __forceinline__ unsigned long long g3d_fixed(float v) {  // v: a scaled corner
-  return (unsigned long long)(long long)(int)rintf(v);
+// A scaled corner v to the nearest integer, ties upward (floor(v + 0.5), one
+// v_cvt_rpi_i32_f32), sign-extended to 64 bits.
+__device__ __forceinline__ unsigned long long g3d_fixed(float v) {
+  int i;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(v));
+  return (unsigned long long)(long long)i;
 }
 
 // float threshold equivalent to the reference's `fabsf(d) >= 1e-5`

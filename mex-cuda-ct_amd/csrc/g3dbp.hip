@@ -213,8 +213,9 @@ __global__ __launch_bounds__(256) void g3d_bin_kernel(G3DBrickArgs a) {
 
 // ------------------------------------------------------------------ brick
 // Range [klo, khi] of the samples k of the line X0 + k s (one axis) that
-// can fall in [lo, hi) once the running sum's drift d is allowed for.  rs is
-// 1/s to within an ulp; the caller widens the range by a sample on each side.
+// can fall in [lo, hi) once the running sum's drift d is allowed for (with
+// -d: the samples that fall in it for sure).  rs is 1/s to within an ulp;
+// the caller moves the range's ends by a sample to cover its rounding.
 __device__ __forceinline__ void axis_range(float X0, float s, float rs, float lo, float hi,
                                            float d, float& klo, float& khi) {
   if (s > 0.f) {
@@ -233,6 +234,58 @@ __device__ __forceinline__ void axis_range(float X0, float s, float rs, float lo
 // float just above n - 1 when the brick reaches past it.
 __device__ __forceinline__ float axis_hi(int lo, int B, int n) {
   return lo + B <= n - 1 ? (float)(lo + B) : __uint_as_float(__float_as_uint((float)(n - 1)) + 1u);
+}
+
+// min(v, 1) as one v_min_u32 (the compiler would make it a compare and a
+// select)
+__device__ __forceinline__ uint32_t min1_u32(uint32_t v) {
+  uint32_t r;
+  asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+// (v << 3) + c as one v_lshl_add_u32 (the compiler would subtract first and
+// shift after)
+__device__ __forceinline__ int lshl3_add(int v, int c) {
+  int r;
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r) : "v"(v), "v"(c));
+  return r;
+}
+
+// One sample of a work item: its two corners added to the brick's LDS sums
+// when its cell lies in the brick (CHECK: tested here; otherwise known), and
+// the running sum advanced.  Each sample goes to LDS as it is taken: integer
+// LDS adds are cheap, and summing the samples of one cell in registers first
+// cost more instructions than it saved (50.2 against 45.5 ms, helical 320^3).
+template <bool CHECK>
+__device__ __forceinline__ void brick_sample(const G3DBrickArgs& a, unsigned long long* lds, R3& cur,
+                                             const R3& inc, float w0, int base8, float lox, float loy,
+                                             float loz, float hx, float hy, float hz) {
+  const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
+  const bool in = !CHECK || (fx >= lox && fx < hx && fy >= loy && fy < hy && fz >= loz && fz < hz);
+  // CUDAmex_general3D_BP.cu:158-170: f - floor(f) + 0.5 per axis (exact, as
+  // v_fract); kL gets 1 - iZ, kU = ceil gets iZ (kL itself when fz is a
+  // whole number).  w0 carries the fixed point's scale.
+  const float rz = __builtin_amdgcn_fractf(fz);
+  const float iXX = __builtin_amdgcn_fractf(fx) + 0.5f;
+  const float iYY = __builtin_amdgcn_fractf(fy) + 0.5f;
+  const float iZ = rz + 0.5f;
+  const float iZZ = 1 - iZ;
+  const float wxy = w0 * iXX * iYY;
+  const float lo = wxy * iZZ;
+  const float hi = wxy * iZ;
+  // byte offset of cell (iL, jL, kL) in the brick's LDS, and 8 more for kU
+  // unless rz is zero (the bits of rz >= 0 are zero only then)
+  const uint32_t at = (uint32_t)mad_i24(cvt_flr(fx), BY * BZH * 8,
+                                        mad_i24(cvt_flr(fy), BZH * 8, lshl3_add(cvt_flr(fz), -base8)));
+  const uint32_t at_hi = at + (min1_u32(__float_as_uint(rz)) << 3);
+  if (in) {
+    atomicAdd((unsigned long long*)((char*)lds + at), g3d_fixed(lo));
+    atomicAdd((unsigned long long*)((char*)lds + at_hi), g3d_fixed(hi));
+  }
+  cur.x = cur.x + inc.x;
+  cur.y = cur.y + inc.y;
+  cur.z = cur.z + inc.z;
 }
 
 constexpr uint32_t kQueue = 128;  // work items per wave (a power of two above 127)
@@ -271,34 +324,36 @@ __device__ __forceinline__ void brick_items(const G3DBrickArgs& a, unsigned long
     }
     const TileRay r = tile_ray(a, t, l);
     const float pval = a.sino[(size_t)r.view * a.W * a.H + (size_t)r.iy * a.W + r.ix];
-    const float w0 = pval * 0.5f;  // P * stepsize
-    // each sample's two corners go to LDS as it is taken: integer LDS adds
-    // are cheap, and summing the samples of one cell in registers first cost
-    // more instructions than it saved (50.2 against 45.5 ms, helical 320^3)
-    for (int k = k0; k <= k1; ++k) {
-      const float fx = cur.x - a.bx, fy = cur.y - a.by, fz = cur.z - a.bz;
-      const bool in = fx >= lox && fx < hx && fy >= loy && fy < hy && fz >= loz && fz < hz;
-      // CUDAmex_general3D_BP.cu:158-170: f - floor(f) + 0.5 per axis (exact,
-      // as v_fract); kL gets 1 - iZ, kU = ceil gets iZ (kL itself when fz is
-      // a whole number)
-      const float rz = __builtin_amdgcn_fractf(fz);
-      const float iXX = __builtin_amdgcn_fractf(fx) + 0.5f;
-      const float iYY = __builtin_amdgcn_fractf(fy) + 0.5f;
-      const float iZ = rz + 0.5f;
-      const float iZZ = 1 - iZ;
-      const float wxy = w0 * iXX * iYY;
-      const float lo = wxy * iZZ;
-      const float hi = wxy * iZ;
-      const uint32_t idx =
-          (uint32_t)mad_i24(cvt_flr(fx), BY * BZH, mad_i24(cvt_flr(fy), BZH, cvt_flr(fz))) - base;
-      if (in) {
-        atomicAdd(&lds[idx], g3d_fixed(lo * scale));
-        atomicAdd(&lds[idx + (rz != 0.f ? 1u : 0u)], g3d_fixed(hi * scale));
+    // P * stepsize, times the fixed point's scale: both powers of two, so
+    // every weight below equals the reference's float weight times the scale
+    // exactly (for P above the float's subnormal range)
+    const float w0 = pval * 0.5f * scale;
+    const int base8 = (int)base * 8;
+    // The samples [m0, m1] lie inside the brick for sure: the line from the
+    // running sum at k0, X + (k - k0) s, stays at least the drift bound
+    // inside every face of the brick's index range there (one sample less on
+    // each side for the rounding of this bound).  They skip the inside test.
+    int m0 = k1 + 1, m1 = k1;
+    {
+      const float fl = (float)(k1 - k0);
+      const float dr = ray_drift(cur, inc, k1 - k0 + 1);
+      float ilo = 0.f, ihi = fl;
+      axis_range(cur.x - a.bx, inc.x, __builtin_amdgcn_rcpf(inc.x), lox, hx, -dr, ilo, ihi);
+      axis_range(cur.y - a.by, inc.y, __builtin_amdgcn_rcpf(inc.y), loy, hy, -dr, ilo, ihi);
+      axis_range(cur.z - a.bz, inc.z, __builtin_amdgcn_rcpf(inc.z), loz, hz, -dr, ilo, ihi);
+      if (ilo <= ihi) {
+        m0 = max(k0, k0 + (int)ceilf(ilo) + 1);
+        m1 = min(k1, k0 + (int)floorf(ihi) - 1);
+        if (m1 < m0) {
+          m0 = k1 + 1;
+          m1 = k1;
+        }
       }
-      cur.x = cur.x + inc.x;
-      cur.y = cur.y + inc.y;
-      cur.z = cur.z + inc.z;
     }
+    int k = k0;
+    for (; k < m0; ++k) brick_sample<true>(a, lds, cur, inc, w0, base8, lox, loy, loz, hx, hy, hz);
+    for (; k <= m1; ++k) brick_sample<false>(a, lds, cur, inc, w0, base8, lox, loy, loz, hx, hy, hz);
+    for (; k <= k1; ++k) brick_sample<true>(a, lds, cur, inc, w0, base8, lox, loy, loz, hx, hy, hz);
   }
   __builtin_amdgcn_wave_barrier();
 }
