@@ -74,6 +74,9 @@ struct BPKArgs {
   uint32_t ncols;  // chain columns per (view, range): padded tiles x 64
   uint32_t super_;       // super-tile side in tiles (block order)
   uint32_t item_blocks;  // blocks of one batch item; n_logical = item_blocks * items
+  // the z-ranges run one after another, each in a phase of phase_blocks grid
+  // blocks of which the first phase_logical work
+  uint32_t phase_blocks, phase_logical;
   uint64_t sino_stride;  // bytes between the items' haloed sinograms
   uint64_t out_stride;   // floats between the items' volumes
   float* const* outs;    // non-null: item i's volume at outs[i]
@@ -166,14 +169,19 @@ __device__ __forceinline__ uint32_t load_bits(const u4v& q) { return q.x ^ q.y ^
 // interpolated; every voxel of the range receives the fold.
 template <int GEOM, int ZR, typename S, bool SMALL, int RQ, bool LOADS = false>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
-  const uint32_t b0 = xcd_remap(blockIdx.x, gridDim.x);
-  if (b0 >= a.n_logical) return;
-  const uint32_t item = b0 / a.item_blocks;  // batch item
-  const uint32_t b = b0 - item * a.item_blocks;
-  const uint32_t zr = b % a.nz;
-  // tiles in super-tiles (row-major inside), so that the blocks an XCD runs
-  // together cover a compact region whose view footprints overlap
-  const uint32_t tile = b / a.nz;
+  // Block order: the z-ranges one after another over the whole chip (a
+  // phase each), and in a phase (batch item, tile), each XCD taking a
+  // contiguous range.  Tiles go in super-tiles (row-major inside), so that
+  // the blocks an XCD runs together cover a compact region whose view
+  // footprints overlap; the voxels of one z-range project to a band of
+  // detector rows whose row-paired copy stays in the memory-side cache
+  // across the phase.
+  const uint32_t zr = blockIdx.x / a.phase_blocks;
+  const uint32_t bl = xcd_remap(blockIdx.x - zr * a.phase_blocks, a.phase_blocks);
+  if (bl >= a.phase_logical) return;
+  const uint32_t tiles = a.item_blocks / a.nz;
+  const uint32_t item = bl / tiles;  // batch item
+  const uint32_t tile = bl - item * tiles;
   const uint32_t chain_col = tile * 64u + threadIdx.x;
   int ix, iy;
   tile_xy(tile, threadIdx.x, a.tiles_x, a.super_, ix, iy);
@@ -394,8 +402,11 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
 
 template <int ZR, typename S, bool SMALL, int RQ>
-static void bp_dispatch_g(const BPKArgs& k, int geom, hipStream_t st) {
-  const unsigned g = padded_grid(k.n_logical);
+static void bp_dispatch_g(const BPKArgs& k0, int geom, hipStream_t st) {
+  BPKArgs k = k0;  // z-ranges outermost (see bp_kernel)
+  k.phase_logical = k.item_blocks / k.nz * (k.n_logical / k.item_blocks);
+  k.phase_blocks = padded_grid(k.phase_logical);
+  const unsigned g = k.phase_blocks * k.nz;
   if (k.loads_only) {  // cone only (the host checks)
     bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ, true><<<g, 64, 0, st>>>(k);
     return;
