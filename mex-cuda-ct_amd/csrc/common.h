@@ -155,9 +155,10 @@ __device__ __forceinline__ float lerp_h(uint32_t A, uint32_t B, float t) {
 // f16 in both halves (pk_h2).  Two instructions where two lerp_h and a
 // float lerp take six, at a rounding of the same size as the f16 storage's
 // own (2^-11 relative); a hardware texture unit's 8-bit weights round the
-// lerp far more coarsely.  The f16 backprojector uses it (1.06x at
-// 384^3 x 512 cone); the forward projector keeps lerp_h, where it gained
-// nothing measurable.
+// lerp far more coarsely.  The f16 backprojector uses it for its column
+// lerps (1.06x at 384^3 x 512 cone), the f16 forward projector for the x
+// lerps of both y rows and then the z lerp of both (its y lerp stays in
+// float).
 constexpr bool kPackedF16Lerp = true;
 __device__ __forceinline__ uint32_t pk_h2(float t) {
   return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(t, t));

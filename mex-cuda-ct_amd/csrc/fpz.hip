@@ -34,7 +34,10 @@
 
 namespace mexct {
 
-constexpr uint32_t kViewChunkLog2 = 4;  // default: 16 views per chunk
+// default views per chunk of the block order: 8 for par, fan and cone (row
+// groups outermost; 16 views ran 1-2% slower at 384^3 x 512 cone f32), 16
+// for general-3D rays and the OSC subset projections
+constexpr uint32_t kViewChunkLog2 = 3, kViewChunkLog2G3D = 4, kViewChunkLog2Osc = 4;
 // default ray length (samples) above which 2-byte ray sums are compensated
 constexpr int kLongRay = 1024;
 
@@ -248,7 +251,10 @@ __device__ __forceinline__ Tap8<S> fetch_z(__amdgpu_buffer_rsrc_t r, const FPZAr
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
-template <typename S, int LAYOUT_>
+// ZPK: the f16 corner-quad volume's z lerp in packed f16 too (the plain
+// projection; the OSC projection keeps its z lerp in float, since its
+// multiplicative update compounds the added rounding)
+template <typename S, int LAYOUT_, bool ZPK = false>
 __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
   constexpr int LAYOUT = base_layout(LAYOUT_);
   float c00, c10, c01, c11;  // c_{y,z}: lerps along x
@@ -266,10 +272,14 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
     }
   } else if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD && kPackedF16Lerp) {
     // x lerps of both y rows at once in packed f16, then y and z in float
+    // x lerps of both y rows at z and at z+1 in packed f16, then (ZPK) the z
+    // lerp of both rows in packed f16 and the y lerp in float, or the y lerps
+    // and the z lerp in float
     const uint32_t ax2 = pk_h2(t.ax);
-    const float c0 = lerp_x(pk_lerp_h(t.w0, t.w1, ax2), t.ay);  // {c00, c10} at z
-    const float c1 = lerp_x(pk_lerp_h(t.w2, t.w3, ax2), t.ay);  // at z+1
-    return lerp(c0, c1, t.az);
+    const uint32_t h0 = pk_lerp_h(t.w0, t.w1, ax2);  // {c00, c10} at z
+    const uint32_t h1 = pk_lerp_h(t.w2, t.w3, ax2);  // at z+1
+    if constexpr (ZPK) return lerp_x(pk_lerp_h(h0, h1, pk_h2(t.az)), t.ay);
+    return lerp(lerp_x(h0, t.ay), lerp_x(h1, t.ay), t.az);
   } else if constexpr (__is_same(S, _Float16)) {
     if (LAYOUT == LAYOUT_DUAL) {
       c00 = lerp_x(t.w0, t.ax);
@@ -326,7 +336,7 @@ __device__ __forceinline__ bool in_halo(const R3& c, const R3& inc, int n, float
 // however many samples the ray has instead of growing with their count.
 // Without COMP the samples are added one by one in ray order, as the
 // reference does (march_loopd is that loop with more samples ahead).
-template <typename S, int STEP2, int LAYOUT, bool COMP, bool CLAMP>
+template <typename S, int STEP2, int LAYOUT, bool COMP, bool CLAMP, bool ZPK = false>
 __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                             const R3& inc, int n, float fnx, float fny, float fnz,
                                             int base, const char* vb) {
@@ -342,7 +352,7 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
     cur.z = cur.z + inc.z;
     const Tap8<S> t1 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
     __builtin_amdgcn_sched_barrier(0);  // t1's loads stay ahead of t0's interpolation
-    float x = interp_z<S, LAYOUT>(t0) * step;
+    float x = interp_z<S, LAYOUT, ZPK>(t0) * step;
     if constexpr (!COMP) dval = dval + x;
     cur.x = cur.x + inc.x;
     cur.y = cur.y + inc.y;
@@ -350,17 +360,17 @@ __device__ __forceinline__ float march_loop(__amdgpu_buffer_rsrc_t r, const FPZA
     t0 = fetch_z<S, LAYOUT, CLAMP>(r, a, cur, fnx, fny, fnz, base, vb);
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (COMP) {
-      x = x + interp_z<S, LAYOUT>(t1) * step;
+      x = x + interp_z<S, LAYOUT, ZPK>(t1) * step;
       const float y = x - comp;
       const float t = dval + y;
       comp = (t - dval) - y;
       dval = t;
     } else {
-      dval = dval + interp_z<S, LAYOUT>(t1) * step;
+      dval = dval + interp_z<S, LAYOUT, ZPK>(t1) * step;
     }
   }
   if (k < n) {
-    const float x = interp_z<S, LAYOUT>(t0) * step;
+    const float x = interp_z<S, LAYOUT, ZPK>(t0) * step;
     if constexpr (COMP) {
       const float y = x - comp;
       dval = dval + y;
@@ -391,7 +401,7 @@ __device__ __forceinline__ uint32_t tap_bits(const Tap8<S>& t) {
 // clamped indices) and are discarded.  LOADS: the loads-only form (the
 // projector's gather ceiling): every address and load of the real loop, the
 // taps XOR-folded instead of interpolated; returns the fold's bits.
-template <typename S, int STEP2, int LAYOUT, bool CLAMP, int D, bool LOADS = false>
+template <typename S, int STEP2, int LAYOUT, bool CLAMP, int D, bool LOADS = false, bool ZPK = false>
 __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZArgs& a, R3 cur,
                                              const R3& inc, int n, float fnx, float fny,
                                              float fnz, int base, const char* vb) {
@@ -419,7 +429,7 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
       if constexpr (LOADS)
         fold ^= tap_bits(t[u]);
       else
-        dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
+        dval = dval + interp_z<S, LAYOUT, ZPK>(t[u]) * step;
     }
   }
 #pragma unroll
@@ -428,7 +438,7 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
       if constexpr (LOADS)
         fold ^= tap_bits(t[u]);
       else
-        dval = dval + interp_z<S, LAYOUT>(t[u]) * step;
+        dval = dval + interp_z<S, LAYOUT, ZPK>(t[u]) * step;
     }
   return LOADS ? __uint_as_float(fold) : dval;
 }
@@ -465,11 +475,11 @@ __device__ __forceinline__ float march_z(__amdgpu_buffer_rsrc_t r, const FPZArgs
     const bool clamp = LAYOUT == LAYOUT_PLAIN || wide64(LAYOUT) ||
                        __builtin_amdgcn_ballot_w64(!in_halo(cur, inc, n, fnx, fny, fnz)) != 0;
     if (comp && !loads_only(EPI))
-      return clamp ? march_loop<S, STEP2, LAYOUT, true, true>(r, a, cur, inc, n, fnx, fny, fnz, base, vb)
-                   : march_loop<S, STEP2, LAYOUT, true, false>(r, a, cur, inc, n, fnx, fny, fnz, base, vb);
-    return clamp ? march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>(), loads_only(EPI)>(
+      return clamp ? march_loop<S, STEP2, LAYOUT, true, true, EPI == 0>(r, a, cur, inc, n, fnx, fny, fnz, base, vb)
+                   : march_loop<S, STEP2, LAYOUT, true, false, EPI == 0>(r, a, cur, inc, n, fnx, fny, fnz, base, vb);
+    return clamp ? march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>(), loads_only(EPI), EPI == 0>(
                        r, a, cur, inc, n, fnx, fny, fnz, base, vb)
-                 : march_loopd<S, STEP2, LAYOUT, false, fetch_ahead<S, EPI>(), loads_only(EPI)>(
+                 : march_loopd<S, STEP2, LAYOUT, false, fetch_ahead<S, EPI>(), loads_only(EPI), EPI == 0>(
                        r, a, cur, inc, n, fnx, fny, fnz, base, vb);
   }
   return march_loopd<S, STEP2, LAYOUT, true, fetch_ahead<S, EPI>(), loads_only(EPI)>(
@@ -510,8 +520,8 @@ __global__ __launch_bounds__(256) void fpz_kernel(FPZArgs a) {
   // views, whose rays lie within a few voxels of each other, so the volume
   // lines they gather are shared in that XCD's L2 (view-major order:
   // 47.8 ms -> 32.4 ms at 384^3 x 512 cone, f32).  The row group (64
-  // detector rows) is either innermost, or, for par, fan and cone
-  // (phase_blocks), outermost in dispatch order: the whole chip then
+  // detector rows) is either innermost, or, for the par, fan and cone
+  // projections (phase_blocks), outermost in dispatch order: the whole chip then
   // projects one row group, whose rays cross a slab of the volume about 64
   // voxels thick whose copy stays in the memory-side cache across views
   // (384^3 x 512 cone: f32 29.6 -> 26.0 ms, f16 18.9 -> 17.4 ms).
@@ -660,7 +670,11 @@ template <int EPI, typename S, int LAYOUT>
 static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
   FPZArgs k = k0;
   unsigned g = (k.n_logical + 7u) & ~7u;
-  if (geom != GEOM_G3D) {  // row groups outermost (see fpz_kernel)
+  // row groups outermost (see fpz_kernel), except for general-3D rays and
+  // the OSC subset projections: a few views, all of whose blocks are
+  // resident at once (at 256^3 x 60 views, 10 subsets, the OSC-TV run took
+  // 119 ms with the phases against 103 ms without)
+  if (geom != GEOM_G3D && EPI != 1) {
     k.phase_logical = k.item_blocks / k.n_vg * (k.n_logical / k.item_blocks);
     k.phase_blocks = (k.phase_logical + 7u) & ~7u;
     g = k.phase_blocks * k.n_vg;
@@ -710,7 +724,11 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
   if (a.long_ray != 0) k.long_ray = a.long_ray;
   if (a.geom == GEOM_G3D) k.rows = a.H;  // every detector row is a ray
   const int items = a.items > 0 ? a.items : 1;
-  set_view_chunk(k, a.vchunk_log2 >= 0 ? (uint32_t)a.vchunk_log2 : kViewChunkLog2, (uint32_t)items);
+  set_view_chunk(k,
+                 a.vchunk_log2 >= 0          ? (uint32_t)a.vchunk_log2
+                 : a.geom == GEOM_G3D ? kViewChunkLog2G3D
+                                      : kViewChunkLog2,
+                 (uint32_t)items);
   k.vol_stride = a.vol_stride;
   k.out_stride = a.out_stride;
   k.outs = a.outs;
@@ -733,6 +751,7 @@ hipError_t launch_fpz(const FPZLaunch& a, hipStream_t st) {
 
 hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st) {
   FPZArgs k = make_zargs(a.mu_z, a.st_dt, a.layout, a.sc, a.W, a.H, a.count, a.sad);
+  set_view_chunk(k, kViewChunkLog2Osc, 1);
   k.pre = a.pre;
   k.post = a.post;
   k.pre_pad = a.pre_pad;
