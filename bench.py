@@ -26,11 +26,12 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker (cpu_baseline, F
 
 HBM_PEAK_GBS = 8000.0
 VC_PEAK_GBS = 256 * 64 * 2.4  # vector L1 caches: 64 B/clock per CU, 256 CUs, 2.4 GHz
-# HBM-side bytes of one cone 384^3 x 512 f32 FP launch from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (copy-calibrated), valid for the kernel sources
-# whose hash is stamped beside it; a changed source reports null until the
-# counters are taken again.
-FP_TRAFFIC = {"bytes": 133.31e9, "sources_sha16": "a4de12f23fcccbef"}
+# Memory-side bytes of one cone 384^3 x 512 f32 FP launch from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (copy-calibrated): what the L2s fetched from
+# and wrote to beyond themselves (the memory-side cache or HBM).  Valid for
+# the kernel sources whose hash is stamped beside it; a changed source
+# reports null until the counters are taken again.
+FP_TRAFFIC = {"bytes": 57.33e9, "sources_sha16": "9ab67383965da922"}
 FP_KERNEL_SOURCES = ("mex-cuda-ct_amd/csrc/fpz.hip", "mex-cuda-ct_amd/csrc/ray.h",
                      "mex-cuda-ct_amd/csrc/common.h")
 
@@ -223,9 +224,11 @@ def main():
     # volume read once per view plus the projections written once.
     fp_bytes = n * (float(W) * W * H * 4) + n * float(W) * H * 4
     fp_gbs = fp_bytes / (fp_ms * 1e-3) / 1e9
-    # traffic: HBM-side bytes per FP launch from rocprofv3 --pmc FETCH_SIZE and
-    # WRITE_SIZE on this workload, scaled by what they read for a 1 GiB device
-    # copy (FETCH x2.0, WRITE x1.0); rerun the counters when the kernel changes
+    # traffic: memory-side bytes per FP launch from rocprofv3 --pmc FETCH_SIZE
+    # and WRITE_SIZE on this workload, scaled by what they read for a 1 GiB
+    # device copy (FETCH x2.0, WRITE x1.0); rerun the counters when the kernel
+    # changes.  It is below the algorithmic bytes: a row group's volume slab
+    # stays in the memory-side cache across the views.
     sha = kernel_sources_sha16()
     fresh = sha == FP_TRAFFIC["sources_sha16"] and (W, n) == (384, 512)
     roofline = {"bound": "hbm", "kernel": "fpz_kernel<cone,f32,x-pair volume>",
@@ -238,7 +241,10 @@ def main():
                                  f"kernel sources {sha} differ from the counter run's "
                                  f"{FP_TRAFFIC['sources_sha16']} (or another workload): "
                                  "no measured traffic"),
-                "algorithmic_bytes_per_launch": fp_bytes}
+                "algorithmic_bytes_per_launch": fp_bytes,
+                "bound_note": ("the kernel's own limit is the vector memory path (texture "
+                               "addresser busy ~98% of the launch, rocprofv3 TA_TA_BUSY); "
+                               "see gather_roofline.fp.frac_of_vector_cache_peak")}
 
     f16 = None
     if not args.no_f16:
