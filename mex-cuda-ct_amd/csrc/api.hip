@@ -1171,10 +1171,11 @@ int mexct_g3d_bp_dev(mexct_ctx* c, const float* sino, int64_t W, int64_t H, int6
     L.N = (int)N;
     void *ck, *mk, *hl, *acc;
     RC(c->get(WS_G3D_CK, plan.ck_bytes, &ck));
-    RC(c->get(WS_G3D_MASK, plan.mask_bytes, &mk));
+    RC(c->get(WS_G3D_MASK, plan.mask_bytes + 256, &mk));  // + the long-ray count
     RC(c->get(WS_G3D_HALO, plan.halo_elems * 8, &hl));
     RC(c->get(WS_G3D_ACC, (size_t)M * M * N * 8, &acc));
     L.mask = (uint32_t*)mk;
+    L.long_rays = (uint32_t*)((char*)mk + plan.mask_bytes);
     L.ck = (float*)ck;
     L.out = out;
     L.halo64 = (int64_t*)hl;

@@ -749,6 +749,7 @@ struct G3DBPKArgs {
   int xfast;       // out is the [M,M,N] x-fastest volume
   int64_t* out64;  // fixed-point sums (x-fastest) instead of out
   const uint32_t* pmax;
+  const uint32_t* gate;  // non-null and 0: nothing to do
 };
 
 __device__ __forceinline__ int floor_index(float v) {
@@ -765,6 +766,7 @@ __device__ __forceinline__ int ceil_index(float v) {
 // consecutive cache lines instead of one line per lane.  g3d_unz_kernel
 // turns the accumulator into the [M,M,N] x-fastest output.
 __global__ __launch_bounds__(256) void g3d_bp_kernel(G3DBPKArgs a) {
+  if (a.gate && *a.gate == 0u) return;  // no ray of more than n_min samples
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= a.n_logical) return;
   const uint32_t view = b / a.tiles_per_view;
@@ -910,6 +912,7 @@ hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st) {
   k.xfast = a.xfast ? 1 : 0;
   k.out64 = a.out64;
   k.pmax = a.pmax;
+  k.gate = a.gate;
   if (k.n_logical == 0) return hipSuccess;
   g3d_bp_kernel<<<padded_grid(k.n_logical), 256, 0, st>>>(k);
   return hipGetLastError();

@@ -72,6 +72,7 @@ struct G3DBrickArgs {
   int64_t* halo64;   // [nbz][M][M]: corners kL + 1 of each brick's top plane
   const uint32_t* pmax;  // bits of max |P| (the fixed point's scale)
   int accumulate;        // a later chunk: add to out/halo
+  uint32_t* long_rays;   // count of the rays of more than n_cap samples
 };
 
 // Ray of chunk tile t, lane `lane`: detector pixel and view, and whether it
@@ -137,6 +138,7 @@ __global__ __launch_bounds__(256) void g3d_bin_kernel(G3DBrickArgs a) {
       a.rays[a.ray_plane + ri] = inc.y;
       a.rays[2 * a.ray_plane + ri] = inc.z;
       a.rays[3 * a.ray_plane + ri] = __int_as_float(n);
+      if (n > a.n_cap) atomicAdd(a.long_rays, 1u);  // left to the atomic kernel
       if (n > 0 && n <= a.n_cap) {
         const size_t cbase = (size_t)t * a.S * 64u + lane;
         // The bricks come from the exact line X0 + k s, a run of kBinRun
@@ -619,7 +621,12 @@ hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
   a.out64 = L.out64;
   a.halo64 = L.halo64;
   a.pmax = L.pmax;
+  a.long_rays = L.long_rays;
   const size_t MMN = (size_t)L.M * L.M * L.N;
+  if (L.nviews > 0) {
+    const hipError_t e = launch_fill((float*)L.long_rays, 0.f, 1, st);
+    if (e != hipSuccess) return e;
+  }
   const size_t lds_bin = (size_t)4 * ((p.nbricks + 31) / 32) * sizeof(uint32_t);
   for (int v0 = 0; v0 < L.nviews; v0 += p.views_per_chunk) {
     const int nv = std::min(p.views_per_chunk, L.nviews - v0);
@@ -658,6 +665,7 @@ hipError_t launch_g3d_bp_bricks(const G3DBrickLaunch& L, hipStream_t st) {
     f.xfast = true;
     f.out64 = L.out64;
     f.pmax = L.pmax;
+    f.gate = L.long_rays;  // nothing to do unless the bin passes counted a long ray
     const hipError_t e = launch_g3d_bp(f, st);
     if (e != hipSuccess) return e;
   }

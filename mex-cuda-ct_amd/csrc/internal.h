@@ -169,6 +169,9 @@ struct G3DBPLaunch {
   bool xfast = false;
   int64_t* out64 = nullptr;
   const uint32_t* pmax = nullptr;
+  // non-null: the launch does nothing when *gate is 0 (the bricks' count of
+  // rays longer than n_min)
+  const uint32_t* gate = nullptr;
 };
 hipError_t launch_g3d_bp(const G3DBPLaunch& a, hipStream_t st);
 
@@ -194,6 +197,7 @@ struct G3DBrickLaunch {
   int64_t* out64;   // [M,M,N] fixed-point sums
   int64_t* halo64;  // plan.halo_elems fixed-point sums
   const uint32_t* pmax;  // bits of max |sino|: the fixed point's scale
+  uint32_t* long_rays;   // one word: the count of rays left to the atomic kernel
 };
 // Every chunk of views, the halo planes, the rays left to the atomic kernel
 // and the conversion to float: out is complete after.
