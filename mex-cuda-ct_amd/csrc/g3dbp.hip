@@ -29,12 +29,14 @@
 // LDS float atomic add costs about 15 times a 64-bit integer one (measured:
 // 2.0e11 against 3.0e12 lane additions per second over the chip), and the
 // integer sums are exact in any order, so every run gives the same bits.
-// The brick's z extent of 12 keeps a workgroup's LDS at 26 KB, six per CU:
-// the loop is latency-bound, and 16 x 16 x 12 ran the helical workload in
-// 63 ms against 71 ms for 16 x 16 x 16 and 93 ms for 16 x 16 x 32.  Rays of
-// more samples than the volume's diagonal admits cannot cross it; they go
+// The brick's z extent of 12 keeps a workgroup's LDS at 31 KB (26.6 KB of
+// sums, 4 KB of work queues), five workgroups per CU; when the brick loop
+// was still latency-bound, 16 x 16 x 12 ran the helical workload in 63 ms
+// against 71 ms for 16 x 16 x 16 and 93 ms for 16 x 16 x 32.  Rays of more
+// samples than the volume's diagonal admits cannot cross it; they go
 // through the ray-driven atomic kernel (launch_g3d_bp, with the same fixed
-// point), which checks every sample as the reference does.
+// point), which checks every sample as the reference does, and which exits
+// at once when the bin passes counted none.
 #include <stdlib.h>
 
 #include <algorithm>
