@@ -152,9 +152,10 @@ int mexct_g3d_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t
  * corners to [M,M,N].  opts->g3d_bp selects how the sums are formed:
  *   MEXCT_G3D_BRICKS: per 16x16x12 voxel brick in LDS, written once, with
  *     64-bit fixed-point sums that are exact in any order, so every run
- *     gives the same bits (the fixed-point step is max|P| * 2^-27 or finer,
- *     per merged run of at most 4 samples in one cell; up to 2^32 samples
- *     per voxel);
+ *     gives the same bits (each added corner is rounded to a step of
+ *     max|P| * 2^-27 or finer: one sample's corner in the bricks, a run of
+ *     at most 4 samples in one cell in the long-ray kernel; up to 2^32 of
+ *     them per voxel);
  *   MEXCT_G3D_DETERMINISTIC: the same, and MEXCT_ENOTSUP where the bricks
  *     cannot run instead of the atomics;
  *   MEXCT_G3D_ATOMICS: the ray-driven scatter with float atomics (the
