@@ -175,6 +175,58 @@ def test_g3d_bp_deterministic_bits_any_chunking(gpu, monkeypatch):
     _check(f, O.g3d_bp(sino, ang, pts, M, N), "g3D BP bricks, 2-view chunks", 1e-5, 1e-6)
 
 
+def _boundary_rays():
+    """Rays on and near the 16 x 16 x 12 brick faces of a 48 x 48 x 36
+    volume (world x, y = -8 and 8, z = -6 and 6 are brick faces; -24 / 24
+    and -18 / 18 the volume's): along the faces and their edges, through a
+    brick corner, with slopes so small that they cross a face only after
+    a hundred samples or more, starting or ending inside the volume, and
+    grazing the volume's own faces; each also reversed.  The brick pass
+    marks bricks from the exact line widened by the running sum's drift and
+    skips the cell test where the line is that far inside a brick: these
+    are the rays such bounds must still cover."""
+    e = 1e-4
+    rays = [
+        (-40, -8, -6, 40, -8, -6), (-40, -8 + e, -6, 40, -8 + e, -6), (-40, -8, 6 - e, 40, -8, 6 - e),
+        (8, -40, 0, 8, 40, 0), (8 - e, -40, 6, 8 - e, 40, 6), (-8, 8, -30, -8, 8, 30),
+        (-8 + e, 8 - e, -30, -8 + e, 8 - e, 30), (-40, -8.001, 0.5, 40, -7.999, 0.5),
+        (-40, 3.0, 5.999, 40, 3.0, 6.001), (-40, -7.9995, -6.0005, 40, -8.0005, -5.9995),
+        (-30, -30, -22.5, 30, 30, 22.5), (-30, 30, 22.5, 30, -30, -22.5), (0, 0, 0, 50, 3, 1),
+        (1, -2, 3, -60, 5, -2), (-40, -24, 0, 40, -24, 0), (-40, 0, 18, 40, 0, 18),
+        (-40, 23.9999, -17.9999, 40, 23.9999, -17.9999), (-24, -40, 5, -24, 40, 5),
+        (-40, 16, -6, 40, 16.0002, -6), (-40, -8, 12, 40, 8, 12.00001),
+    ]
+    rays += [(r[3], r[4], r[5], r[0], r[1], r[2]) for r in rays]
+    return np.array(rays, np.float64)
+
+
+def test_g3d_bp_brick_boundary_rays(gpu):
+    W, H, M, N = 8, 5, 48, 36  # 40 rays per view
+    rays = _boundary_rays()
+    rng = _rng(71)
+    n = 2
+    pts = np.zeros((W, H, 6, n), np.float32)
+    for k in range(n):
+        # both views hold every boundary ray, in another order, then random
+        # rays near the faces
+        order = rng.permutation(len(rays))
+        for q in range(W * H):
+            if q < len(rays):
+                r = rays[order[q]]
+            else:
+                a = rng.choice([-8.0, 8.0, -6.0, 6.0]) + rng.uniform(-1e-3, 1e-3, 6)
+                r = np.array([-40, a[0], a[1], 40, a[2], a[3]])
+            pts[q % W, q // W, :, k] = r
+    pts = np.asfortranarray(pts)
+    sino = np.asfortranarray(rng.uniform(0.5, 1.5, size=(W, H, n)).astype(np.float32))
+    ang = np.zeros(n, np.float32)  # no rotation: the rays lie exactly where written
+    ref = O.g3d_bp(sino, ang, pts, M, N)
+    assert np.count_nonzero(ref) > 0
+    for mode in (0, 2):
+        got = mexct.CUDAmex_general3D_BP(sino, ang, pts, M, N, mode=mode)
+        _check(got, ref, f"g3D BP boundary rays mode {mode}", 1e-5, 1e-6)
+
+
 def _osc_inputs(W, H, n, seed):
     r = _rng(seed)
     mu = np.asfortranarray(r.uniform(0.0, 0.03, size=(W, W, H)).astype(np.float32))
