@@ -121,6 +121,20 @@ __global__ __launch_bounds__(256) void pad3z_kernel(const void* __restrict__ in,
   }
 }
 
+// The two values of a pair element in one store: 8 bytes for f32 (the
+// element's address is a multiple of 8), 4 for 2-byte storage.
+template <typename S>
+__device__ __forceinline__ void store_pair(S* p, float a, float b) {
+  if constexpr (__is_same(S, float)) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    *(f2v*)p = f2v{a, b};
+  } else {
+    const uint32_t lo = __builtin_bit_cast(uint16_t, Store<S>::from(a));
+    const uint32_t hi = __builtin_bit_cast(uint16_t, Store<S>::from(b));
+    *(uint32_t*)p = lo | (hi << 16);
+  }
+}
+
 // x-pair copy (see launch_pad3d).  A block covers 32 z x 32 x of one y: the
 // (z, x..x+32) tile of input row y is staged in LDS (read along x), then
 // each lane writes one pair element (along z).
@@ -154,8 +168,7 @@ __global__ __launch_bounds__(256) void pad3d_kernel(const void* __restrict__ in,
     const int dx = dx0 + lx, dz = dz0 + tz;
     if (dx < DX && dz < DZ) {
       const size_t e = ((size_t)dx * DY + dy) * DZ + dz;
-      out[2 * e] = Store<S>::from(tile[tz][lx]);
-      out[2 * e + 1] = Store<S>::from(tile[tz][lx + 1]);
+      store_pair<S>(out + 2 * e, tile[tz][lx], tile[tz][lx + 1]);
     }
   }
 }
