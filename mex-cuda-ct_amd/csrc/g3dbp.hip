@@ -249,10 +249,10 @@ __device__ __forceinline__ uint32_t min1_u32(uint32_t v) {
 }
 
 // (v << 3) + c as one v_lshl_add_u32 (the compiler would subtract first and
-// shift after)
+// shift after); c is wave-uniform and stays in its SGPR
 __device__ __forceinline__ int lshl3_add(int v, int c) {
   int r;
-  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r) : "v"(v), "v"(c));
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r) : "v"(v), "s"(c));
   return r;
 }
 
