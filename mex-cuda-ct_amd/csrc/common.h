@@ -169,6 +169,27 @@ __device__ __forceinline__ uint32_t pk_lerp_h(uint32_t A, uint32_t B, uint32_t t
   asm("v_pk_fma_f16 %0, %1, %2, %3" : "=v"(c) : "v"(t2), "v"(d), "v"(A));
   return c;
 }
+// {a, b} rounded to f16 (toward zero) in one dword
+__device__ __forceinline__ uint32_t pk_h2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+// fma(t, D, A) on both f16 halves of D and A, t the low (HI = 0) or the high
+// (HI = 1) half of t2 in both
+template <int HI>
+__device__ __forceinline__ uint32_t pk_fma_b(uint32_t t2, uint32_t D, uint32_t A) {
+  uint32_t c;
+  if (HI)
+    asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,1,1]" : "=v"(c) : "v"(t2), "v"(D), "v"(A));
+  else
+    asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel_hi:[0,1,1]" : "=v"(c) : "v"(t2), "v"(D), "v"(A));
+  return c;
+}
+// H.lo + t * H.hi of the two f16 halves of H, in float (one rounding)
+__device__ __forceinline__ float fma_hl(float t, uint32_t H) {
+  float c;
+  asm("v_fma_mix_f32 %0, %1, %2, %2 op_sel:[0,1,0] op_sel_hi:[0,1,1]" : "=v"(c) : "v"(t), "v"(H));
+  return c;
+}
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 

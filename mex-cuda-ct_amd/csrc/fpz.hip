@@ -16,8 +16,12 @@
 //     (i+1), (i+1,j+1) strides in scalar offsets.
 //   DUAL (f32): element (x,y,z) = {V(x,y,z), V(x+1,y,z)}; the two elements
 //     at z, z+1 of rows j and j+1 are two 16-byte loads per sample.
-//   QUAD (f16, bf16): element (x,y,z) = {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
-//     V(x+1,y+1,z)}; one 16-byte load per sample.
+//   QUAD (f16, bf16): element (x,y,z) holds the four xy corners at z; one
+//     16-byte load per sample.  bf16: {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
+//     V(x+1,y+1,z)}.  f16 (difference quad): {V, Dy, Dx, Dxy} with
+//     Dy = V(x,y+1,z) - V, Dx = V(x+1,y,z) - V and Dxy the x difference of
+//     row y+1 minus Dx, each rounded once from float, so that an x lerp of
+//     both rows is one packed FMA and the y lerp one mixed FMA.
 // The load unit spends about the same time on a wave's load whether its
 // lanes fetch 4, 8 or 16 bytes when the data is cached, and L2 supplies
 // about 64 bytes per clock per CU, so the best layout issues the fewest
@@ -94,7 +98,7 @@ struct Tap8 {
 // widened by a shift or a mask.
 //   plain layout: w0..w3 = (z, z+1) pairs of (i,j), (i,j+1), (i+1,j), (i+1,j+1)
 //   quad layout:  w0 = {(i,j), (i,j+1)} and w1 = {(i+1,j), (i+1,j+1)} at z,
-//                 w2, w3 the same at z+1
+//                 w2, w3 the same at z+1 (f16: w0 = {V, Dy}, w1 = {Dx, Dxy})
 //   dual layout:  w0 = {(i,j), (i+1,j)} at z, w1 the same at z+1, w2, w3
 //                 the same for row j+1
 template <>
@@ -270,27 +274,27 @@ __device__ __forceinline__ float interp_z(const Tap8<S>& t) {
       c01 = lerp(bf_hi(t.w0), bf_hi(t.w2), t.ax);
       c11 = lerp(bf_hi(t.w1), bf_hi(t.w3), t.ax);
     }
-  } else if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD && kPackedF16Lerp) {
-    // x lerps of both y rows at once in packed f16, then y and z in float
-    // x lerps of both y rows at z and at z+1 in packed f16, then (ZPK) the z
-    // lerp of both rows in packed f16 and the y lerp in float, or the y lerps
-    // and the z lerp in float
-    const uint32_t ax2 = pk_h2(t.ax);
-    const uint32_t h0 = pk_lerp_h(t.w0, t.w1, ax2);  // {c00, c10} at z
-    const uint32_t h1 = pk_lerp_h(t.w2, t.w3, ax2);  // at z+1
-    if constexpr (ZPK) return lerp_x(pk_lerp_h(h0, h1, pk_h2(t.az)), t.ay);
-    return lerp(lerp_x(h0, t.ay), lerp_x(h1, t.ay), t.az);
+  } else if constexpr (__is_same(S, _Float16) && LAYOUT == LAYOUT_QUAD) {
+    // difference quad: w0 = {V(i,j), V(i,j+1) - V(i,j)}, w1 = {the x
+    // differences of both}, at z; w2, w3 at z+1.  One packed FMA per z gives
+    // {c_j, c_j+1 - c_j} (the x lerp of row j and the y difference); then
+    // (ZPK) the z lerp of both in packed f16 and the y lerp as one mixed FMA,
+    // or the y lerps and the z lerp in float.
+    const uint32_t axz = pk_h2(t.ax, t.az);
+    const uint32_t h0 = pk_fma_b<0>(axz, t.w1, t.w0);  // at z
+    const uint32_t h1 = pk_fma_b<0>(axz, t.w3, t.w2);  // at z+1
+    if constexpr (ZPK) {
+      uint32_t d;
+      asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1]" : "=v"(d) : "v"(h1), "v"(h0));
+      return fma_hl(t.ay, pk_fma_b<1>(axz, d, h0));
+    }
+    return lerp(fma_hl(t.ay, h0), fma_hl(t.ay, h1), t.az);
   } else if constexpr (__is_same(S, _Float16)) {
     if (LAYOUT == LAYOUT_DUAL) {
       c00 = lerp_x(t.w0, t.ax);
       c01 = lerp_x(t.w1, t.ax);
       c10 = lerp_x(t.w2, t.ax);
       c11 = lerp_x(t.w3, t.ax);
-    } else if (LAYOUT == LAYOUT_QUAD) {
-      c00 = lerp_h<0>(t.w0, t.w1, t.ax);
-      c10 = lerp_h<1>(t.w0, t.w1, t.ax);
-      c01 = lerp_h<0>(t.w2, t.w3, t.ax);
-      c11 = lerp_h<1>(t.w2, t.w3, t.ax);
     } else {
       c00 = lerp_h<0>(t.w0, t.w2, t.ax);
       c10 = lerp_h<0>(t.w1, t.w3, t.ax);

@@ -28,7 +28,8 @@ hipError_t launch_pad3d(const void* in, int in_dt, int64_t in_sy, int64_t in_sz,
                         const uint32_t* scale_max = nullptr);
 // z-fastest corner-quad copy: element (x,y,z), x,y in [-2,n], z in [-2,nz+1], at
 // ((x+2)*(ny+3) + y+2)*(nz+4) + z+2 holds {V(x,y,z), V(x,y+1,z), V(x+1,y,z),
-// V(x+1,y+1,z)} of the haloed volume V (same input addressing as launch_pad3z).
+// V(x+1,y+1,z)} of the haloed volume V (same input addressing as launch_pad3z);
+// f16 holds the differences {V, Dy, Dx, Dxy} of those four instead (pad.hip).
 hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
                         int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
                         const uint32_t* scale_max = nullptr);

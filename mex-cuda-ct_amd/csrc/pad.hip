@@ -186,11 +186,15 @@ struct Quad<float> {
     *(v4*)(out + 4 * e) = q;
   }
 };
+// f16: the difference quad {a, b - a, c - a, (d - b) - (c - a)}, each term
+// computed in float and rounded to f16 once (a, b, c, d = V(x,y), V(x,y+1),
+// V(x+1,y), V(x+1,y+1))
 template <>
 struct Quad<_Float16> {
   typedef _Float16 v4 __attribute__((ext_vector_type(4)));
   __device__ static void put(_Float16* out, size_t e, float a, float b, float c, float d) {
-    v4 q = {(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+    const float dx = c - a;
+    v4 q = {(_Float16)a, (_Float16)(b - a), (_Float16)dx, (_Float16)((d - b) - dx)};
     *(v4*)(out + 4 * e) = q;
   }
 };
