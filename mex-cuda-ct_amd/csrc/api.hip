@@ -321,10 +321,12 @@ int check_dtype(int dt, const char* what) {
   return MEXCT_OK;
 }
 
-// Padded operands are indexed with 32-bit element offsets.
-int check_padded(int64_t a, int64_t b, int64_t c, const char* what) {
+// Padded operands are indexed with 32-bit element offsets: signed (below
+// 2^31 haloed elements) by default, unsigned where `limit` says so.
+int check_padded(int64_t a, int64_t b, int64_t c, const char* what,
+                 double limit = 2147483648.0) {
   double n = (double)(a + 2 * HALO) * (double)(b + 2 * HALO) * (double)(c + 2 * HALO);
-  if (n >= 2147483648.0) return fail(MEXCT_EINVAL, "%s too large (%g padded elements)", what, n);
+  if (n >= limit) return fail(MEXCT_EINVAL, "%s too large (%g padded elements)", what, n);
   return MEXCT_OK;
 }
 
@@ -695,7 +697,11 @@ int fp_run(mexct_ctx* c, const void* vol, int32_t vol_dtype, int items, int64_t 
   RC(check_dtype(vol_dtype, "vol"));
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
-  RC(check_padded(W, W, H, "volume"));
+  // below 2^32 haloed elements, the reference's own unsigned 32-bit index
+  // range (CUDAmex_FP.cu:394): the z-fastest copies past 4 GiB read through
+  // 64-bit addresses from 32-bit element indices, and the x-fastest
+  // projector's element offsets are unsigned 32-bit
+  RC(check_padded(W, W, H, "volume", 4294967296.0));
   if (nProj == 0) return MEXCT_OK;
   DevGuard g(c->device);
   void* pad;
@@ -844,7 +850,10 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   RC(check_dtype(sino_dtype, "sino"));
   const int st = storage_of(opts);
   RC(check_dtype(st, "storage"));
-  RC(check_padded(W, H, nProj, "sinogram"));
+  // any number of views: the row-grouped copy is indexed per view and the
+  // kernel takes the views in chunks below 2 GiB (the reference walks them
+  // one at a time, CUDAmex_BP.cu:287-291)
+  if (nProj > INT32_MAX) return fail(MEXCT_EINVAL, "%lld views", (long long)nProj);
   if (loads_only && geom != GEOM_CONE) return fail(MEXCT_ENOTSUP, "the loads-only BP is cone only");
   // row-grouped haloed copy: (W+4) x (H+3) elements of two values per view,
   // four for 2-byte storage when the rows of consecutive z-voxels step by

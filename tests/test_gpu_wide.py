@@ -13,7 +13,11 @@ projector.  These tests reach every one of those paths:
   against the oracle bit for bit, and its f16 form against the f32 result;
 - a 64 x 64 x 123400 volume goes to the x-fastest projector, checked on
   sampled rays against the oracle bit for bit (the cone rays of the rows
-  the reference computes cross the volume near slice 30850).
+  the reference computes cross the volume near slice 30850);
+- past 2^31 haloed elements, up to the reference's unsigned 32-bit index
+  range: a 1296^3 FP (1300^3 haloed) on sampled rays, and a 512^2 x 8200
+  view BP (the reference walks any number of views) on sampled voxel
+  columns, each against the oracle bit for bit and in f16 against f32.
 """
 import sys
 
@@ -225,3 +229,87 @@ def test_fp_tall_volume_xfastest_sampled(torch_cuda, geom, sad):
     ref = O.fp_rays(vnp.T, sad, ang, geom, iu, iv, ik)
     assert np.array_equal(fp[ik, iv, iu], ref)
     assert fp[ik, iv, iu].max() > 10.0
+
+
+def test_fp_past_2_31_haloed_sampled(torch_cuda):
+    """1296^3: (W+4)^2 (H+4) = 1300^3 haloed elements, past 2^31.  The
+    reference indexes its volume in unsigned 32 bits (CUDAmex_FP.cu:394), so
+    it takes volumes up to 2^32 elements; the x-pair and corner-quad copies
+    read through 64-bit addresses from 32-bit element indices."""
+    torch = torch_cuda
+    W = H = 1296
+    n, geom, sad = 4, 2, 2600.0
+    assert (W + 4) ** 2 * (H + 4) >= 1 << 31
+    _say(f"fp {W}^3 f32 cone n={n}: past 2^31 haloed elements")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1296)
+    vol = torch.rand((H, W, W), generator=g, device="cuda", dtype=torch.float32)
+    ang = (np.arange(n) * (2 * np.pi / n) + 0.2).astype(np.float32)
+    ctx = device.Context(0)
+    out = torch.zeros((n, H, W), dtype=torch.float32, device="cuda")
+    ctx.fp(vol, sad, ang, n, geom, out)
+    torch.cuda.synchronize()
+    fp = out.cpu().numpy()
+    _say(f"fp {W}^3 f16 storage")
+    out16 = torch.zeros_like(out)
+    ctx.fp(vol, sad, ang, n, geom, out16, storage=mexct.F16)
+    torch.cuda.synchronize()
+    fp16 = out16.cpu().numpy()
+    ctx.close()
+    vnp = vol.cpu().numpy()
+    del vol, out, out16
+    torch.cuda.empty_cache()
+    rng = _rng(1300)
+    iu, iv, ik = rng.integers(0, W, 4096), rng.integers(0, H, 4096), rng.integers(0, n, 4096)
+    ref = O.fp_rays(vnp.T, sad, ang, geom, iu, iv, ik)
+    assert np.array_equal(fp[ik, iv, iu], ref)
+    assert fp[ik, iv, iu].max() > 100.0
+    e = rel_rmse(fp16, fp)
+    _say(f"f16 storage relRMSE vs f32: {e:.3e}")
+    assert e <= 1e-3
+
+
+def test_bp_past_2_31_haloed_sinogram_sampled(torch_cuda):
+    """512^2 detector x 8200 views: (W+4)(H+4)(n+4) past 2^31 haloed
+    elements.  The reference walks any number of views, one projection at a
+    time (CUDAmex_BP.cu:287-291); here the views run in launches below 2 GiB
+    of sinogram copy that continue each other's sums."""
+    torch = torch_cuda
+    W = H = 512
+    n, geom, sad = 8200, 2, 1500.0
+    assert (W + 4) * (H + 4) * (n + 4) >= 1 << 31
+    _say(f"bp {W}^2 x {n} views f32 cone: past 2^31 haloed sinogram elements")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8200)
+    sino = torch.rand((n, H, W), generator=g, device="cuda", dtype=torch.float32)
+    ang = (np.arange(n) * (2 * np.pi / n)).astype(np.float32)
+    ctx = device.Context(0)
+    out = torch.zeros((H, W, W), dtype=torch.float32, device="cuda")
+    ctx.bp(sino, sad, ang, geom, out)
+    torch.cuda.synchronize()
+    rec = out.cpu().numpy()  # [z, y, x]
+    _say("bp f16 storage")
+    out16 = torch.zeros_like(out)
+    ctx.bp(sino, sad, ang, geom, out16, storage=mexct.F16)
+    torch.cuda.synchronize()
+    rec16 = out16.cpu().numpy()
+    ctx.close()
+    P = sino.cpu().numpy()
+    del sino, out, out16
+    torch.cuda.empty_cache()
+    rng = _rng(516)
+    hw = W // 2 - 1
+    cols = []
+    while len(cols) < 16:
+        a, b = rng.integers(0, W, 2)
+        if (a - hw) ** 2 + (b - hw) ** 2 < hw * hw:
+            cols.append((a, b))
+    cx = np.array([c[0] for c in cols])
+    cy = np.array([c[1] for c in cols])
+    # the oracle takes the MATLAB layout sino[u, v, view]: the (view, v, u) array transposed
+    ref = O.bp_cols(P.T, sad, ang, geom, cx, cy)
+    assert np.array_equal(rec[:, cy, cx].T, ref)
+    assert ref.max() > 100.0
+    e = rel_rmse(rec16, rec)
+    _say(f"f16 storage relRMSE vs f32: {e:.3e}")
+    assert e <= 1e-3
