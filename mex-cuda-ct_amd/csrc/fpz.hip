@@ -40,8 +40,11 @@ namespace mexct {
 
 // default views per chunk of the block order: 8 for par, fan and cone (row
 // groups outermost; 16 views ran 1-2% slower at 384^3 x 512 cone f32), 16
-// for general-3D rays and the OSC subset projections
-constexpr uint32_t kViewChunkLog2 = 3, kViewChunkLog2G3D = 4, kViewChunkLog2Osc = 4;
+// for general-3D rays, 1 for the OSC subset projections (a subset's few
+// views padded to a chunk of 16 left most blocks empty; OSC-TV 256^3 x 60:
+// f32 104.6 -> 97.8 ms, f16 99.4 -> 90.8 ms with 1, 104.0 / 117.6 / 106.2 ms
+// f32 with 2 / 4 / 8)
+constexpr uint32_t kViewChunkLog2 = 3, kViewChunkLog2G3D = 4, kViewChunkLog2Osc = 0;
 // default ray length (samples) above which 2-byte ray sums are compensated
 constexpr int kLongRay = 1024;
 
