@@ -78,7 +78,9 @@ int mexct_version(void);
  * vol [W,W,H] -> out [W,H,nProj].  Cone marches with step 1.0, par/fan 0.5,
  * clamp addressing.  Detector rows iy >= 32*ceil(W/32) stay 0 as in the
  * reference's grid (CUDAmex_FP.cu:432-440).  n_angles >= nProj is required
- * (the reference reads past the end otherwise). */
+ * (the reference reads past the end otherwise).  Volumes up to
+ * (W+4)^2 (H+4) < 2^32 haloed elements (the reference's unsigned 32-bit
+ * index range, CUDAmex_FP.cu:394); larger ones are MEXCT_EINVAL. */
 int mexct_fp(const float* vol, int64_t W, int64_t H, float sad, const float* angles,
              int64_t n_angles, int32_t nProj, int32_t geom, float* out, const mexct_opts* opts);
 int mexct_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t W, int64_t H,
@@ -90,7 +92,9 @@ int mexct_fp_dev(mexct_ctx* ctx, const void* vol, int32_t vol_dtype, int64_t W, 
  * replaces projection_kernels/CUDAmex_BP.cu:223-324 (kernels :44-221).
  * sino [W,H,nProj] -> out [W,W,H]; voxels outside the circle
  * (ix-hw)^2+(iy-hw)^2 < hw^2, hw = W/2-1, are 0.  Views accumulate in
- * view order; no 1/nProj or pi factor (as the reference). */
+ * view order; no 1/nProj or pi factor (as the reference).  Volumes below
+ * 2^32 elements (CUDAmex_BP.cu:217) and any number of views (the reference
+ * walks them one at a time, :287-291). */
 int mexct_bp(const float* sino, int64_t W, int64_t H, int64_t nProj, float sad,
              const float* angles, int64_t n_angles, int32_t geom, float* out,
              const mexct_opts* opts);
