@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <climits>
 #include <map>
@@ -587,6 +588,43 @@ int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 
 // Mirrors the slots fp_run, bp_run, prefilter_run and mexct_osc_tv_recon_dev
 // take from the context (mexct_ctx::get rounds each up to 2 MiB).
+// Rows of halo above and below the BP's row-grouped sinogram copy, enough
+// that no in-circle voxel's detector row needs the kernel's clamp (the
+// halo's edge copies are the clamped taps): par and fan rows stay in
+// [0, H-1]; a cone voxel at z sits on row v0 + (z - v0) U, U = SAD / (SAD - s),
+// |s| < hw + 1 for the in-circle columns, so within v0 (Umax - 1) of
+// [0, H-1], plus the running sum's drift and a margin.  *zclamp: a source
+// within 5% of the circle, or a copy that would pass 2 GiB per view, keeps
+// the kernel's clamp and the 2-row halo; so does f32 storage, whose
+// backprojector ran no faster without the clamp (384^3 x 512 cone: 15.8
+// against 15.7 ms at SAD 1000, 16.9 against 16.3 at SAD 500, the taller copy
+// costing what the clamp saved), and so does a halo adding more than a
+// third of the copy's rows (f16 at SAD 1000, a quarter more rows: 9.5
+// against 9.8 ms; at SAD 500, 62% more: 10.2 against 9.9 ms).
+int bp_row_halo(int64_t W, int64_t H, int geom, float sad, int rows, int st, bool* zclamp) {
+  int zh = HALO;
+  *zclamp = st == MEXCT_F32;
+  if (*zclamp) return zh;
+  if (geom == GEOM_CONE) {
+    const double r = (double)(W / 2 - 1) + 1.0, v0 = (double)(H - 1) / 2.0;
+    const double ext = (double)sad > 1.05 * r ? v0 * ((double)sad / ((double)sad - r) - 1.0) + 4.0
+                                              : 1e30;
+    if (ext < (double)H + 8.0)
+      zh = std::max(HALO, (int)std::ceil(ext));
+    else
+      *zclamp = true;
+  }
+  if (2 * (zh - HALO) > (H + 3) / 3) {
+    zh = HALO;
+    *zclamp = true;
+  }
+  if ((double)(W + 4) * (double)(H + 2 * zh - 1) * rows * store_bytes(st) >= 2147483648.0) {
+    zh = HALO;
+    *zclamp = true;
+  }
+  return zh;
+}
+
 size_t op_workspace_bytes(int op, int geom, int st, int64_t W, int64_t H, int64_t nProj, float sad,
                           int items, int min_subsets) {
   auto slot = [](size_t b) {
@@ -607,7 +645,9 @@ size_t op_workspace_bytes(int op, int geom, int st, int64_t W, int64_t H, int64_
   if (op == MEXCT_OP_BP || op == MEXCT_OP_FDK) {
     const double hw = (double)(W / 2 - 1);
     const int rows = st != MEXCT_F32 && (geom != 2 || (double)sad > 2.02 * hw) ? 4 : 2;
-    b += slot((size_t)(W + 4) * (H + 3) * rows * es * n * nitems);
+    bool zc;
+    const int zh = bp_row_halo(W, H, geom, sad, rows, st, &zc);
+    b += slot((size_t)(W + 4) * (H + 2 * zh - 1) * rows * es * n * nitems);
     if (geom == 2 && st == MEXCT_F32) {
       size_t cfv = 0;  // the largest chain of any tuning candidate
       for (int sup : {2, 4, 8})
@@ -862,7 +902,9 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   // kernel takes views in chunks below 2 GiB, so one view must fit
   const double hw = (double)(W / 2 - 1);
   const int rows = st != MEXCT_F32 && (geom != 2 || (double)sad > 2.02 * hw) ? 4 : 2;
-  const size_t rp_bytes = (size_t)(W + 4) * (H + 3) * rows * store_bytes(st);
+  bool zclamp;
+  const int zh = bp_row_halo(W, H, geom, sad, rows, st, &zclamp);
+  const size_t rp_bytes = (size_t)(W + 4) * (H + 2 * zh - 1) * rows * store_bytes(st);
   if ((double)rp_bytes >= 2147483648.0)
     return fail(MEXCT_EINVAL, "detector too large (%lld x %lld)", (long long)W, (long long)H);
   if ((double)W * W * H >= 4294967296.0) return fail(MEXCT_EINVAL, "volume too large");
@@ -880,7 +922,7 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
       HIPCHK(launch_pad2(in_ptrs ? in_ptrs[i] : (const char*)sino + i * sel * in_es, sino_dtype,
                          (char*)pad + i * sino_stride,
                          st, (int)W, (int)H, (int)nProj, ADDR_CLAMP, c->stream,
-                         smax ? smax + i : nullptr, rows));
+                         smax ? smax + i : nullptr, rows, zh));
   const float2* sc = nullptr;
   RC(c->sincos_table(angles, (int)nProj, &sc));
   BPLaunch a;
@@ -896,6 +938,8 @@ int bp_run(mexct_ctx* c, const void* sino, int32_t sino_dtype, int items, int64_
   a.sad = sad;
   a.zr = pick_zrange(opts, geom, H);
   a.rows = rows;
+  a.zh = zh;
+  a.zclamp = zclamp;
   a.chain = nullptr;
   a.chain_views = 0;
   a.items = items;

@@ -81,6 +81,7 @@ struct BPKArgs {
   uint64_t out_stride;   // floats between the items' volumes
   float* const* outs;    // non-null: item i's volume at outs[i]
   int loads_only;        // launch the loads-only form
+  int zh;                // rows of halo above and below in the sinogram copy
 };
 
 // Voxel column (ix, iy) of lane `lane` of tile `tile`: 8x8 tiles ordered in
@@ -167,7 +168,10 @@ __device__ __forceinline__ uint32_t load_bits(const u4v& q) { return q.x ^ q.y ^
 // LOADS: the loads-only form (the backprojector's gather ceiling): the same
 // blocks, offsets and loads, each load's bits XOR-folded instead of
 // interpolated; every voxel of the range receives the fold.
-template <int GEOM, int ZR, typename S, bool SMALL, int RQ, bool LOADS = false>
+// ZC: clamp each detector row to the copy's rows; without it the host has
+// shown that every in-circle voxel's rows lie within the zh-row halo, whose
+// edge copies give the clamped taps.
+template <int GEOM, int ZR, typename S, bool SMALL, int RQ, bool ZC, bool LOADS = false>
 __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
   // Block order: the z-ranges one after another over the whole chip (a
   // phase each), and in a phase (batch item, tile), each XCD taking a
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
     // byte offset of (u = i0, v = j0) is (i0+2)*es + (j0+2)*row_bytes, an
     // integer below 2^24 and therefore exact in float
     const uint32_t soff = (uint32_t)v * (uint32_t)a.PV * es;
-    const float colf = (fi0 + (float)HALO) * (float)es + (float)(HALO * row_bytes);
+    const float colf = (fi0 + (float)HALO) * (float)es + (float)(a.zh * row_bytes);
     const float rbf = (float)row_bytes;
     const uint32_t colb = (uint32_t)((int)fi0 + HALO) * es;  // integer form (large slices)
     // groups of 8 z-voxels: offsets and loads of the whole group are issued
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       // is below 2, which the host checks), so the pair (j1, j1+1) of each
       // column is the 4-row run shifted down by j1 - j0 halves.
       float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
-      const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(HALO * row_bytes);
+      const uint32_t cb = (uint32_t)((int)fi0 + HALO) * es + (uint32_t)(a.zh * row_bytes);
       const uint32_t ax2 = pk_h2(ax);
       // Groups of GP voxel pairs, software-pipelined: the loads of group
       // g+1 are issued before group g is interpolated, so loads stay in
@@ -258,8 +262,8 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
           const float zb1 = zb + dz;
           az0[b][g] = __builtin_amdgcn_fractf(zb);
           az1[b][g] = __builtin_amdgcn_fractf(zb1);
-          const int j0 = clamp_m2(cvt_flr(zb), a.H);
-          const int j1 = clamp_m2(cvt_flr(zb1), a.H);
+          const int j0 = ZC ? clamp_m2(cvt_flr(zb), a.H) : cvt_flr(zb);
+          const int j1 = ZC ? clamp_m2(cvt_flr(zb1), a.H) : cvt_flr(zb1);
           sh[b][g] = (uint32_t)(j1 - j0) << 4;
           q[b][g] = __builtin_amdgcn_raw_buffer_load_b128(r, (uint32_t)__mul24(j0, row_bytes) + cb,
                                                           soff, 0);
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
       // texel coordinates (no replay of the reference's running sum); its
       // rounding differs by a few ulps, far below the storage rounding
       float zb = __builtin_fmaf((float)z0, dz, pz) - 0.5f;
-      const uint32_t cb = colb + (uint32_t)(HALO * row_bytes);
+      const uint32_t cb = colb + (uint32_t)(a.zh * row_bytes);
 #pragma unroll
       for (int m0 = 0; m0 < ZR; m0 += G) {
         float az[G];
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           az[g] = __builtin_amdgcn_fractf(zb);
-          const int j0 = clamp_m2(cvt_flr(zb), a.H);
+          const int j0 = ZC ? clamp_m2(cvt_flr(zb), a.H) : cvt_flr(zb);
           q[g] = bld4<S>(r, (uint32_t)__mul24(j0, row_bytes) + cb, soff);
           zb = zb + dz;
         }
@@ -363,10 +367,10 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
         const float zb = pz - 0.5f;
         const float fz = floorf(zb);
         az[g] = zb - fz;
-        const float fj0 = __builtin_amdgcn_fmed3f(fz, -2.f, (float)a.H);
+        const float fj0 = ZC ? __builtin_amdgcn_fmed3f(fz, -2.f, (float)a.H) : fz;
         const uint32_t voff =
             SMALL ? (uint32_t)__builtin_fmaf(fj0, rbf, colf)
-                  : (uint32_t)((int)fj0 + HALO) * (uint32_t)row_bytes + colb;
+                  : (uint32_t)((int)fj0 + a.zh) * (uint32_t)row_bytes + colb;
         q[g] = bld4<S>(r, voff, soff);
         pz = pz + dz;
       }
@@ -401,42 +405,52 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
 
 static unsigned padded_grid(uint32_t n_logical) { return (n_logical + 7u) & ~7u; }
 
-template <int ZR, typename S, bool SMALL, int RQ>
+template <int ZR, typename S, bool SMALL, int RQ, bool ZC>
 static void bp_dispatch_g(const BPKArgs& k0, int geom, hipStream_t st) {
   BPKArgs k = k0;  // z-ranges outermost (see bp_kernel)
   k.phase_logical = k.item_blocks / k.nz * (k.n_logical / k.item_blocks);
   k.phase_blocks = padded_grid(k.phase_logical);
   const unsigned g = k.phase_blocks * k.nz;
   if (k.loads_only) {  // cone only (the host checks)
-    bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ, true><<<g, 64, 0, st>>>(k);
+    bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ, ZC, true><<<g, 64, 0, st>>>(k);
     return;
   }
   switch (geom) {
-    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
-    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
-    default: bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ><<<g, 64, 0, st>>>(k); break;
+    case GEOM_PAR: bp_kernel<GEOM_PAR, ZR, S, SMALL, RQ, ZC><<<g, 64, 0, st>>>(k); break;
+    case GEOM_FAN: bp_kernel<GEOM_FAN, ZR, S, SMALL, RQ, ZC><<<g, 64, 0, st>>>(k); break;
+    default: bp_kernel<GEOM_CONE, ZR, S, SMALL, RQ, ZC><<<g, 64, 0, st>>>(k); break;
   }
 }
 
-template <int ZR, typename S>
+template <int ZR, typename S, bool ZC>
 static void bp_dispatch(const BPKArgs& k, int geom, int rows, hipStream_t st) {
   if constexpr (sizeof(S) == 2) {
-    if (rows == 4) return bp_dispatch_g<ZR, S, false, 1>(k, geom, st);
+    if (rows == 4) return bp_dispatch_g<ZR, S, false, 1, ZC>(k, geom, st);
   }
   if ((uint64_t)k.PV * 2u * sizeof(S) < (1u << 24))
-    bp_dispatch_g<ZR, S, true, 0>(k, geom, st);
+    bp_dispatch_g<ZR, S, true, 0, ZC>(k, geom, st);
   else
-    bp_dispatch_g<ZR, S, false, 0>(k, geom, st);
+    bp_dispatch_g<ZR, S, false, 0, ZC>(k, geom, st);
 }
 
-template <typename S>
+template <typename S, bool ZC>
 static void bp_dispatch_zr(const BPKArgs& k, int geom, int zr, int rows, hipStream_t st) {
   if (zr == 64)
-    bp_dispatch<64, S>(k, geom, rows, st);
+    bp_dispatch<64, S, ZC>(k, geom, rows, st);
   else if (zr == 48)
-    bp_dispatch<48, S>(k, geom, rows, st);
+    bp_dispatch<48, S, ZC>(k, geom, rows, st);
   else
-    bp_dispatch<32, S>(k, geom, rows, st);
+    bp_dispatch<32, S, ZC>(k, geom, rows, st);
+}
+
+// the unclamped rows for 2-byte storage only (the host keeps f32 clamped)
+template <typename S>
+static void bp_dispatch_zc(const BPKArgs& k, int geom, int zr, int rows, bool zclamp,
+                           hipStream_t st) {
+  if (zclamp || sizeof(S) == 4)
+    bp_dispatch_zr<S, true>(k, geom, zr, rows, st);
+  else if constexpr (sizeof(S) == 2)
+    bp_dispatch_zr<S, false>(k, geom, zr, rows, st);
 }
 
 size_t bp_chain_floats_per_view(int W, int H, int zr, int super_) {
@@ -460,7 +474,8 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
   k.W = a.W;
   k.H = a.H;
   k.PW = a.W + 2 * HALO;
-  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * HALO - 1);
+  k.zh = a.zh > HALO ? a.zh : HALO;
+  k.PV = (int64_t)(a.W + 2 * HALO) * (a.H + 2 * k.zh - 1);
   k.v0 = float(a.H - 1) / 2;
   k.SR = a.sad;
   k.tiles_x = (uint32_t)((a.W + 7) / 8);
@@ -504,11 +519,11 @@ hipError_t launch_bp(const BPLaunch& a, hipStream_t st) {
       k.chain = a.chain;
     }
     if (a.st_dt == 0)
-      bp_dispatch_zr<float>(k, a.geom, zr, rows, st);
+      bp_dispatch_zc<float>(k, a.geom, zr, rows, a.zclamp || a.zh < HALO, st);
     else if (a.st_dt == 1)
-      bp_dispatch_zr<_Float16>(k, a.geom, zr, rows, st);
+      bp_dispatch_zc<_Float16>(k, a.geom, zr, rows, a.zclamp || a.zh < HALO, st);
     else
-      bp_dispatch_zr<bf16_t>(k, a.geom, zr, rows, st);
+      bp_dispatch_zc<bf16_t>(k, a.geom, zr, rows, a.zclamp || a.zh < HALO, st);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     v0 += cn;

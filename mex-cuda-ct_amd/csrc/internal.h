@@ -33,12 +33,12 @@ hipError_t launch_pad3d(const void* in, int in_dt, int64_t in_sy, int64_t in_sz,
 hipError_t launch_pad3q(const void* in, int in_dt, int64_t in_sy, int64_t in_sz, void* out,
                         int st_dt, int nx, int ny, int nz, int mode, hipStream_t st,
                         const uint32_t* scale_max = nullptr);
-// 2D stack, row-grouped: in [w,h,n] -> out [rows, w+4, h+3, n] where element
-// (px, py) holds the haloed values at rows py .. py+rows-1 of column px
-// (rows = 2, or 4 for 2-byte storage).
+// 2D stack, row-grouped: in [w,h,n] -> out [rows, w+4, h+2*zh-1, n] where
+// element (px, py) holds the haloed values at rows py-zh .. py-zh+rows-1 of
+// column px (rows = 2, or 4 for 2-byte storage; zh >= 2 rows of halo).
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
                        int mode, hipStream_t st, const uint32_t* scale_max = nullptr,
-                       int rows = 2);
+                       int rows = 2, int zh = 2);
 // scale_max (pad3z, pad3q, pad2): when non-null, the bits of max|in|; the
 // copy holds in * 2^-pow2_exp(scale_max) (see common.h).
 // 3D interior copy between a padded f32 volume and a plain one.
@@ -116,7 +116,7 @@ hipError_t launch_osc_fp(const OSCFPLaunch& a, hipStream_t st);
 
 // ---- backprojection (bp.hip)
 struct BPLaunch {
-  const void* sino;  // row-paired haloed sinogram [2, W+4, H+3, nProj] (launch_pad2, edge halo)
+  const void* sino;  // row-grouped haloed sinogram [rows, W+4, H+2*zh-1, nProj] (launch_pad2, edge halo)
   int st_dt;
   const uint32_t* sino_max;  // non-null: sino holds P * 2^-e (pow2_exp); sums are rescaled
   float* out;        // [W, W, H]
@@ -136,6 +136,11 @@ struct BPLaunch {
   // the loads-only form: the same blocks and loads, each load XOR-folded
   // instead of interpolated (cone only)
   bool loads_only = false;
+  // rows of halo above and below in the sinogram copy; zclamp false: every
+  // in-circle voxel's detector rows lie within them (the host has shown it),
+  // so the kernel reads them without clamping
+  int zh = 2;
+  bool zclamp = true;
 };
 // floats of BP chain scratch per view (cone f32, z-range zr, super-tile side)
 size_t bp_chain_floats_per_view(int W, int H, int zr, int super_);

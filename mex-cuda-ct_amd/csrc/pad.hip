@@ -40,19 +40,20 @@ __global__ __launch_bounds__(256) void pad3_kernel(const void* __restrict__ in, 
 }
 
 // Row-grouped haloed sinogram for the backprojector (R = 2 or 4 rows):
-// out[((view*(h+3) + py)*PX + px)*R + r] = P(px-2, py-2+r), r < R, py in
-// [0, h+3), PX = w+4, with clamp (edge copies) or border (zeros) outside.
+// out[((view*PR + py)*PX + px)*R + r] = P(px-2, py-zh+r), r < R, py in
+// [0, PR), PR = h+2*zh-1, PX = w+4, with clamp (edge copies) or border
+// (zeros) outside; zh >= 2 rows of halo above and below.
 // R = 2: element (py, px) and its neighbour (py, px+1) hold the four taps of
-// a bilinear sample at (px-2, py-2), one 16-byte (f32) or 8-byte load.
+// a bilinear sample at (px-2, py-zh), one 16-byte (f32) or 8-byte load.
 // R = 4 (2-byte storage): the 16-byte load of two elements holds rows
-// py-2 .. py+1 of two columns, the taps of two consecutive z-voxels whose
+// py-zh .. py-zh+3 of two columns, the taps of two consecutive z-voxels whose
 // rows differ by at most 2.
 template <typename S, int R>
 __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, int in_dt,
                                                    S* __restrict__ out, int w, int h, int n,
-                                                   int mode, const uint32_t* scale_max) {
+                                                   int mode, const uint32_t* scale_max, int zh) {
   const int sexp = pow2_exp(scale_max);
-  const int PX = w + 2 * HALO, PR = h + 2 * HALO - 1;
+  const int PX = w + 2 * HALO, PR = h + 2 * zh - 1;
   const size_t total = (size_t)PX * PR * n;
   for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
        o += (size_t)gridDim.x * blockDim.x) {
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void pad2_kernel(const void* __restrict__ in, 
     const size_t vbase = (size_t)w * h * view;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      int i = px - HALO, j = py - HALO + q;
+      int i = px - HALO, j = py - zh + q;
       float v;
       if (mode == ADDR_CLAMP) {
         i = clampi(i, 0, w - 1);
@@ -285,21 +286,22 @@ hipError_t launch_pad3(const void* in, int in_dt, void* out, int st_dt, int nx, 
 }
 
 hipError_t launch_pad2(const void* in, int in_dt, void* out, int st_dt, int w, int h, int n,
-                       int mode, hipStream_t st, const uint32_t* scale_max, int rows) {
-  size_t total = (size_t)(w + 2 * HALO) * (h + 2 * HALO - 1) * n;
+                       int mode, hipStream_t st, const uint32_t* scale_max, int rows, int zh) {
+  if (zh < HALO) zh = HALO;
+  size_t total = (size_t)(w + 2 * HALO) * (h + 2 * zh - 1) * n;
   unsigned g = grid_for(total);
   if (st_dt == 0)
-    pad2_kernel<float, 2><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode, scale_max);
+    pad2_kernel<float, 2><<<g, 256, 0, st>>>(in, in_dt, (float*)out, w, h, n, mode, scale_max, zh);
   else if (st_dt == 1 && rows == 4)
     pad2_kernel<_Float16, 4><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode,
-                                                scale_max);
+                                                scale_max, zh);
   else if (st_dt == 1)
     pad2_kernel<_Float16, 2><<<g, 256, 0, st>>>(in, in_dt, (_Float16*)out, w, h, n, mode,
-                                                scale_max);
+                                                scale_max, zh);
   else if (rows == 4)
-    pad2_kernel<bf16_t, 4><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
+    pad2_kernel<bf16_t, 4><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max, zh);
   else
-    pad2_kernel<bf16_t, 2><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max);
+    pad2_kernel<bf16_t, 2><<<g, 256, 0, st>>>(in, in_dt, (bf16_t*)out, w, h, n, mode, scale_max, zh);
   return hipGetLastError();
 }
 

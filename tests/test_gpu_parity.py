@@ -72,6 +72,32 @@ def test_bp_std(gpu, geom, zrange, sad):
     assert np.array_equal(got, ref), "BP is expected to be bit-identical to the oracle"
 
 
+@pytest.mark.parametrize("ratio", [1.02, 1.3, 1.8, 2.5, 10.0])
+@pytest.mark.parametrize("storage", [mexct.F32, mexct.F16])
+def test_bp_cone_source_near_circle(gpu, ratio, storage):
+    # SAD = ratio x (hw + 1): at 1.02 and 1.3 the detector rows of the circle's
+    # voxels reach far past the detector and the kernel clamps them; at 1.8
+    # and 2.5 a halo covering them would add more than a third of the rows,
+    # so the kernel still clamps (the f16 row-pair and row-quad paths); at 10
+    # the f16 sinogram copy's row halo covers them and no clamp runs (f32
+    # always clamps).  Either way the result is the reference's clamp
+    # addressing.
+    W, H, n = 40, 30, 9
+    sad = ratio * (W // 2)
+    sino = np.asfortranarray(_rng(77).uniform(0, 1, (W, H, n)).astype(np.float32))
+    ang = _angles(n)
+    got = mexct.CUDAmex_BP(sino, sad, ang, 2, storage=storage)
+    if storage == mexct.F32:
+        assert np.array_equal(got, O.bp(sino, sad, ang, 2)), f"sad={sad}"
+    else:  # against the oracle on the f16-rounded sinogram
+        ref = O.bp(_round_storage(sino, storage), sad, ang, 2)
+        # below 2.02 hw the magnification reaches 1 / (ratio - 1) at the
+        # source side (50 at 1.02), where a few heavily weighted samples make
+        # up each voxel's sum and the f16 lerp rounding averages out less
+        tol = F16_LERP_BOUND if ratio > 2.02 else (1e-3, 5e-4)
+        _check(got, ref, f"BP f16 sad={sad} vs rounded", *tol)
+
+
 def test_bp_odd_width(gpu):
     W, H, n = 33, 17, 5
     sino = np.asfortranarray(_rng(3).normal(size=(W, H, n)).astype(np.float32))
