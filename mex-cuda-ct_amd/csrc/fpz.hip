@@ -673,10 +673,18 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout_, const float2*
   return k;
 }
 
+// The f16 par, fan and cone projections (and their loads-only form) run at
+// most 5 workgroups (20 waves) per CU: each reserves 32 KB of LDS, which the
+// kernel does not use.  Its 56 VGPRs would otherwise let 8 run; at 384^3 x
+// 512 cone the median of ten f16 launches went 15.4 -> 15.0 ms (6 per CU:
+// 15.2 ms, 4: 15.0 ms).  f32 (119 VGPRs) runs 4 per CU anyway.
+constexpr unsigned kF16OccLds = 32768;
+
 template <int EPI, typename S, int LAYOUT>
 static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
   FPZArgs k = k0;
   unsigned g = (k.n_logical + 7u) & ~7u;
+  const unsigned lds = EPI != 1 && __is_same(S, _Float16) ? kF16OccLds : 0u;
   // row groups outermost (see fpz_kernel), except for general-3D rays and
   // the OSC subset projections: a few views, all of whose blocks are
   // resident at once (at 256^3 x 60 views, 10 subsets, the OSC-TV run took
@@ -687,9 +695,9 @@ static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
     g = k.phase_blocks * k.n_vg;
   }
   switch (geom) {
-    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
-    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
-    case GEOM_CONE: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, 0, st>>>(k); break;
+    case GEOM_PAR: fpz_kernel<GEOM_PAR, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
+    case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
+    case GEOM_CONE: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
     default:
       if constexpr (EPI == 0) fpz_kernel<GEOM_G3D, S, 0, LAYOUT><<<g, 256, 0, st>>>(k);
       break;
