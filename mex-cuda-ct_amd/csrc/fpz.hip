@@ -679,6 +679,12 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout_, const float2*
 // 512 cone the median of ten f16 launches went 15.4 -> 15.0 ms (6 per CU:
 // 15.2 ms, 4: 15.0 ms).  f32 (119 VGPRs) runs 4 per CU anyway.
 constexpr unsigned kF16OccLds = 32768;
+// The general-3D projection likewise, with fewer workgroups still: 41000 B
+// of LDS per workgroup for f16, 54000 B for f32, the fastest of 0, 23000,
+// 32000, 41000, 54000 and 81000 B at the helical 320^3 x 360 views x 256^2
+// workload (f16 15.4 -> 14.0 ms, f32 23.6 -> 22.8 ms).  The cone f32 FP ran
+// the same with any of them (26.6 ms).
+constexpr unsigned kG3DOccLdsF16 = 41000, kG3DOccLdsF32 = 54000;
 
 template <int EPI, typename S, int LAYOUT>
 static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
@@ -699,7 +705,12 @@ static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
     case GEOM_FAN: fpz_kernel<GEOM_FAN, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
     case GEOM_CONE: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
     default:
-      if constexpr (EPI == 0) fpz_kernel<GEOM_G3D, S, 0, LAYOUT><<<g, 256, 0, st>>>(k);
+      if constexpr (EPI == 0) {
+        const unsigned l3 = __is_same(S, _Float16) ? kG3DOccLdsF16
+                            : __is_same(S, float)  ? kG3DOccLdsF32
+                                                   : 0u;
+        fpz_kernel<GEOM_G3D, S, 0, LAYOUT><<<g, 256, l3, st>>>(k);
+      }
       break;
   }
 }
