@@ -452,12 +452,17 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
 
 // Samples in flight in the plain loop.  At 384^3 x 512 cone, f32: 32.3 ms
 // with 1, 31.5 with 2, 30.9 with 3, 29.9 with 5, 29.5 with 7 and 11 (119 and
-// 179 VGPRs); f16 19.8 ms with 1, 19.1-19.3 with 3 and 5.  An OSC subset's
-// projection (6 views at 256^2 x 256: 6 waves per SIMD) keeps 4, the most
-// that lets all 6 stay resident (80 VGPRs f32): 307 us per subset with 2,
-// 279 with 3, ~240 with 4.
+// 179 VGPRs); f16 19.8 ms with 1, 19.1-19.3 with 3 and 5; with the f16
+// projections' workgroups per CU capped (fpz_dispatch_l), the cone f16 FP
+// ran 15.3 ms with 3, 5 and 7 and the helical general-3D one 13.4, 12.9 and
+// 12.4 ms, so f16 keeps 7 (bf16, uncapped, 3).  An OSC subset's projection
+// (6 views at 256^2 x 256: 6 waves per SIMD) keeps 4, the most that lets all
+// 6 stay resident (80 VGPRs f32): 307 us per subset with 2, 279 with 3, ~240
+// with 4.
 template <typename S, int EPI>
-constexpr int fetch_ahead() { return EPI == 1 ? 4 : sizeof(S) == 2 ? 3 : 7; }
+constexpr int fetch_ahead() {
+  return EPI == 1 ? 4 : __is_same(S, bf16_t) ? 3 : 7;
+}
 
 // EPI: 0 store the ray sum, 1 the OSC ratio terms, 2 the loads-only form.
 constexpr bool loads_only(int epi) { return epi == 2; }
