@@ -455,13 +455,14 @@ __device__ __forceinline__ float march_loopd(__amdgpu_buffer_rsrc_t r, const FPZ
 // 179 VGPRs); f16 19.8 ms with 1, 19.1-19.3 with 3 and 5; with the f16
 // projections' workgroups per CU capped (fpz_dispatch_l), the cone f16 FP
 // ran 15.3 ms with 3, 5 and 7 and the helical general-3D one 13.4, 12.9 and
-// 12.4 ms, so f16 keeps 7 (bf16, uncapped, 3).  An OSC subset's projection
+// 12.4 ms, so 2-byte storage keeps 7 (bf16 with 7 and the caps: cone 20.0 ->
+// 19.4 ms, general-3D 15.2 -> 14.0 ms, against 3 uncapped).  An OSC subset's projection
 // (6 views at 256^2 x 256: 6 waves per SIMD) keeps 4, the most that lets all
 // 6 stay resident (80 VGPRs f32): 307 us per subset with 2, 279 with 3, ~240
 // with 4.
 template <typename S, int EPI>
 constexpr int fetch_ahead() {
-  return EPI == 1 ? 4 : __is_same(S, bf16_t) ? 3 : 7;
+  return EPI == 1 ? 4 : 7;
 }
 
 // EPI: 0 store the ray sum, 1 the OSC ratio terms, 2 the loads-only form.
@@ -678,14 +679,14 @@ static FPZArgs make_zargs(const void* vol, int st_dt, int layout_, const float2*
   return k;
 }
 
-// The f16 par, fan and cone projections (and their loads-only form) run at
-// most 5 workgroups (20 waves) per CU: each reserves 32 KB of LDS, which the
-// kernel does not use.  Its 56 VGPRs would otherwise let 8 run; at 384^3 x
-// 512 cone the median of ten f16 launches went 15.4 -> 15.0 ms (6 per CU:
+// The 2-byte par, fan and cone projections (and their loads-only form) run
+// at most 5 workgroups (20 waves) per CU: each reserves 32 KB of LDS, which
+// the kernel does not use.  Their VGPRs would otherwise let 8 run; at 384^3
+// x 512 cone the median of ten f16 launches went 15.4 -> 15.0 ms (6 per CU:
 // 15.2 ms, 4: 15.0 ms).  f32 (119 VGPRs) runs 4 per CU anyway.
 constexpr unsigned kF16OccLds = 32768;
 // The general-3D projection likewise, with fewer workgroups still: 41000 B
-// of LDS per workgroup for f16, 54000 B for f32, the fastest of 0, 23000,
+// of LDS per workgroup for 2-byte storage, 54000 B for f32, the fastest of 0, 23000,
 // 32000, 41000, 54000 and 81000 B at the helical 320^3 x 360 views x 256^2
 // workload (f16 15.4 -> 14.0 ms, f32 23.6 -> 22.8 ms).  The cone f32 FP ran
 // the same with any of them (26.6 ms).
@@ -695,7 +696,7 @@ template <int EPI, typename S, int LAYOUT>
 static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
   FPZArgs k = k0;
   unsigned g = (k.n_logical + 7u) & ~7u;
-  const unsigned lds = EPI != 1 && __is_same(S, _Float16) ? kF16OccLds : 0u;
+  const unsigned lds = EPI != 1 && sizeof(S) == 2 ? kF16OccLds : 0u;
   // row groups outermost (see fpz_kernel), except for general-3D rays and
   // the OSC subset projections: a few views, all of whose blocks are
   // resident at once (at 256^3 x 60 views, 10 subsets, the OSC-TV run took
@@ -711,9 +712,7 @@ static void fpz_dispatch_l(const FPZArgs& k0, int geom, hipStream_t st) {
     case GEOM_CONE: fpz_kernel<GEOM_CONE, S, EPI, LAYOUT><<<g, 256, lds, st>>>(k); break;
     default:
       if constexpr (EPI == 0) {
-        const unsigned l3 = __is_same(S, _Float16) ? kG3DOccLdsF16
-                            : __is_same(S, float)  ? kG3DOccLdsF32
-                                                   : 0u;
+        const unsigned l3 = sizeof(S) == 2 ? kG3DOccLdsF16 : kG3DOccLdsF32;
         fpz_kernel<GEOM_G3D, S, 0, LAYOUT><<<g, 256, l3, st>>>(k);
       }
       break;
