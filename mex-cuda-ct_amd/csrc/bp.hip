@@ -279,16 +279,21 @@ __global__ __launch_bounds__(64) void bp_kernel(BPKArgs a) {
             fold ^= load_bits(v);
             continue;
           }
+          if constexpr (__is_same(S, _Float16) && kPackedF16Lerp) {
+            // all four rows lerped across the columns, then the second
+            // voxel's pair shifted out of them (the lerp is elementwise, so
+            // this equals lerping the shifted rows: one shift instead of two)
+            const uint32_t h01 = pk_lerp_h(v.x, v.z, ax2);  // rows j0, j0+1
+            const uint32_t h23 = pk_lerp_h(v.y, v.w, ax2);  // rows j0+2, j0+3
+            const uint32_t h1 = (uint32_t)((((uint64_t)h23 << 32) | h01) >> sh[b][g]);
+            acc[m0 + 2 * g] = __builtin_fmaf(UU, lerp_x(h01, az0[b][g]), acc[m0 + 2 * g]);
+            acc[m0 + 2 * g + 1] = __builtin_fmaf(UU, lerp_x(h1, az1[b][g]), acc[m0 + 2 * g + 1]);
+            continue;
+          }
           const uint32_t A1 = (uint32_t)((((uint64_t)v.y << 32) | v.x) >> sh[b][g]);
           const uint32_t B1 = (uint32_t)((((uint64_t)v.w << 32) | v.z) >> sh[b][g]);
           float c0, c1, d0, d1;
-          if constexpr (__is_same(S, _Float16) && kPackedF16Lerp) {
-            acc[m0 + 2 * g] =
-                __builtin_fmaf(UU, lerp_x(pk_lerp_h(v.x, v.z, ax2), az0[b][g]), acc[m0 + 2 * g]);
-            acc[m0 + 2 * g + 1] =
-                __builtin_fmaf(UU, lerp_x(pk_lerp_h(A1, B1, ax2), az1[b][g]), acc[m0 + 2 * g + 1]);
-            continue;
-          } else if constexpr (__is_same(S, _Float16)) {
+          if constexpr (__is_same(S, _Float16)) {
             c0 = lerp_h<0>(v.x, v.z, ax);
             c1 = lerp_h<1>(v.x, v.z, ax);
             d0 = lerp_h<0>(A1, B1, ax);
